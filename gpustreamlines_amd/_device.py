@@ -1,0 +1,169 @@
+"""One tracking problem resident on one HIP device, driven through the C ABI.
+
+``DeviceTracker`` owns a ``cuslines_tracker`` handle (its device buffers and
+HIP stream) and runs the two-phase seed protocol of the reference
+(cuslines/cuda_python/cu_propagate_seeds.py:206-233): phase 1 (initial peak
+directions per seed), an on-device exclusive scan of the per-seed counts,
+phase 2 (tracking into padded slots) and an on-device compaction of the
+slots into one packed point array, so only valid points cross PCIe.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+
+class DeviceBuffer:
+    """Device allocation owned by a ``DeviceTracker``."""
+
+    def __init__(self, dt: "DeviceTracker", nbytes: int):
+        self.dt = dt
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        _lib.check(dt.L.cuslines_device_alloc(dt.h, max(self.nbytes, 4), C.byref(p)), "device_alloc")
+        self.ptr = p.value
+
+    def free(self):
+        if self.ptr is not None and self.dt.h:
+            _lib.check(self.dt.L.cuslines_device_free(self.dt.h, C.c_void_p(self.ptr)), "device_free", hard=False)
+        self.ptr = None
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise ValueError("host array larger than the device buffer")
+        _lib.check(self.dt.L.cuslines_memcpy_htod(self.dt.h, C.c_void_p(self.ptr), a.ctypes.data, a.nbytes), "htod")
+
+    def download(self, out: np.ndarray):
+        if out.nbytes > self.nbytes:
+            raise ValueError("host array larger than the device buffer")
+        _lib.check(self.dt.L.cuslines_memcpy_dtoh(self.dt.h, out.ctypes.data, C.c_void_p(self.ptr), out.nbytes), "dtoh")
+        return out
+
+
+class DeviceTracker:
+    def __init__(self, params: dict, device: int = 0):
+        self.L = _lib.load()
+        self.params = dict(params)
+        self._ps = _lib.params_struct(params)
+        h = C.c_void_p()
+        _lib.check(self.L.cuslines_tracker_create(C.byref(self._ps), int(device), C.byref(h)), "tracker_create")
+        self.h = h.value
+        self.device = device
+
+    # -- setup ---------------------------------------------------------------
+    def upload(self, dataf, stop_map, vertices, edges, boot=None, ptt_lut=None):
+        d = np.ascontiguousarray(dataf, dtype=np.float32)
+        s = np.ascontiguousarray(stop_map, dtype=np.float32)
+        v = np.ascontiguousarray(vertices, dtype=np.float32)
+        e = np.ascontiguousarray(edges, dtype=np.int32)
+        _lib.check(self.L.cuslines_set_data(self.h, d.ctypes.data), "set_data")
+        _lib.check(self.L.cuslines_set_stop_map(self.h, s.ctypes.data), "set_stop_map")
+        _lib.check(self.L.cuslines_set_sphere(self.h, v.ctypes.data, e.ctypes.data), "set_sphere")
+        if boot is not None:
+            self.set_boot(**boot)
+        if ptt_lut is not None:
+            self.set_lut(ptt_lut)
+
+    def set_boot(self, H, R, delta_b, delta_q, sampling_matrix, b0s_mask, **_):
+        mats = [np.ascontiguousarray(m, dtype=np.float32) for m in (H, R, delta_b, delta_q, sampling_matrix)]
+        b0 = np.ascontiguousarray(b0s_mask, dtype=np.int32)
+        _lib.check(self.L.cuslines_set_boot_matrices(self.h, *[m.ctypes.data for m in mats], b0.ctypes.data),
+                   "set_boot_matrices")
+
+    def set_lut(self, lut):
+        lut = np.ascontiguousarray(lut, dtype=np.int32)
+        _lib.check(self.L.cuslines_set_ptt_lut(self.h, lut.ctypes.data), "set_ptt_lut")
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.check(self.L.cuslines_tracker_destroy(self.h), "tracker_destroy", hard=False)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- buffers and stream -------------------------------------------------
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    def synchronize(self):
+        _lib.check(self.L.cuslines_synchronize(self.h), "synchronize")
+
+    def stream(self) -> int:
+        s = C.c_void_p()
+        _lib.check(self.L.cuslines_get_stream(self.h, C.byref(s)), "get_stream")
+        return s.value or 0
+
+    # -- phases on device buffers -----------------------------------------
+    def phase1_async(self, seeds_d, n, rng_chunk, counts_d, dirs_d):
+        _lib.check(self.L.cuslines_phase1_async(self.h, C.c_void_p(seeds_d), int(n), int(rng_chunk),
+                                                C.c_void_p(counts_d), C.c_void_p(dirs_d)), "phase1")
+
+    def scan_async(self, counts_d, n, offsets_d):
+        _lib.check(self.L.cuslines_scan_async(self.h, C.c_void_p(counts_d), int(n), C.c_void_p(offsets_d)), "scan")
+
+    def phase2_async(self, seeds_d, n, rng_chunk, offsets_d, dirs_d, sseed_d, slen_d, sline_d):
+        _lib.check(self.L.cuslines_phase2_async(self.h, C.c_void_p(seeds_d), int(n), int(rng_chunk),
+                                                C.c_void_p(offsets_d), C.c_void_p(dirs_d), C.c_void_p(sseed_d),
+                                                C.c_void_p(slen_d), C.c_void_p(sline_d)), "phase2")
+
+    def compact_async(self, slen_d, nsl, sline_d, poff_d, pts_d):
+        _lib.check(self.L.cuslines_compact_async(self.h, C.c_void_p(slen_d), int(nsl), C.c_void_p(sline_d),
+                                                 C.c_void_p(poff_d), C.c_void_p(pts_d)), "compact")
+
+    # -- the whole protocol on host arrays ----------------------------------
+    def track(self, seeds, rng_chunk: int = 0, keep_dirs: bool = False, keep_slots: bool = False) -> dict:
+        """Phase 1, scan, phase 2 and compaction for ``seeds`` (float32 [n,3]).
+
+        Returns host arrays: ``counts`` [n], ``offsets`` [n+1],
+        ``sline_seed``/``sline_len`` [nsl], ``points`` [sum(len),3] and
+        ``point_offsets`` int64 [nsl+1]; ``dirs`` [n, nverts] with
+        ``keep_dirs``; padded ``slots`` with ``keep_slots``.
+        """
+        seeds = np.ascontiguousarray(seeds, dtype=np.float32).reshape(-1, 3)
+        n = len(seeds)
+        nv = self.params["nverts"]
+        L = self.params.get("max_sline_len", 501)
+        out = {}
+        bufs = []
+        try:
+            sd = self.alloc(seeds.nbytes); bufs.append(sd)
+            cd = self.alloc(4 * (n + 1)); bufs.append(cd)
+            dd = self.alloc(4 * n * nv); bufs.append(dd)
+            od = self.alloc(4 * (n + 1)); bufs.append(od)
+            sd.upload(seeds)
+            self.phase1_async(sd.ptr, n, rng_chunk, cd.ptr, dd.ptr)
+            self.scan_async(cd.ptr, n, od.ptr)
+            offsets = od.download(np.zeros(n + 1, dtype=np.int32))
+            counts = cd.download(np.zeros(n, dtype=np.int32))
+            nsl = int(offsets[n])
+            out["counts"], out["offsets"] = counts, offsets
+            if keep_dirs:
+                out["dirs"] = dd.download(np.zeros((n, nv), dtype=np.int32))
+            ssd = self.alloc(4 * nsl); bufs.append(ssd)
+            sld = self.alloc(4 * nsl); bufs.append(sld)
+            sl = self.alloc(4 * 3 * 2 * L * nsl); bufs.append(sl)
+            self.phase2_async(sd.ptr, n, rng_chunk, od.ptr, dd.ptr, ssd.ptr, sld.ptr, sl.ptr)
+            slen = sld.download(np.zeros(nsl, dtype=np.int32))
+            out["sline_seed"] = ssd.download(np.zeros(nsl, dtype=np.int32))
+            out["sline_len"] = slen
+            if keep_slots:
+                out["slots"] = sl.download(np.zeros((nsl, 2 * L, 3), dtype=np.float32))
+            npts = int(slen.astype(np.int64).sum())
+            pod = self.alloc(8 * (nsl + 1)); bufs.append(pod)
+            ptd = self.alloc(4 * 3 * npts); bufs.append(ptd)
+            self.compact_async(sld.ptr, nsl, sl.ptr, pod.ptr, ptd.ptr)
+            out["point_offsets"] = pod.download(np.zeros(nsl + 1, dtype=np.int64))
+            out["points"] = ptd.download(np.zeros((npts, 3), dtype=np.float32))
+        finally:
+            for b in bufs:
+                b.free()
+        return out

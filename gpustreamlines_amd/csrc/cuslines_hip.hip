@@ -1,0 +1,1556 @@
+// cuslines_hip.hip -- gfx950 kernels and C ABI of the per-seed tracking hot
+// path: bootstrap (OPDT/CSA), probabilistic and PTT direction getters, the
+// stepping loop and the two-phase seed protocol of dipy/GPUStreamlines.
+//
+// Layout in HBM (one tracker):
+//   prob/boot data  float [X][Y][Z][T32]   channels padded to a multiple of
+//                                          32 so each half-wave gather of 32
+//                                          channels is one aligned 128-B line
+//   PTT ODF         float [V][X][Y][Z]     vertex-major: the 8 texels of one
+//                                          probe tap share z-adjacent lines
+//   stop map        float [X][Y][Z]
+//   sphere          float [V][3], edges packed (u | v << 16)
+//   LUT             uint16 [R][R][R]       nearest sphere vertex
+//
+// Execution model: wave64, two seed slots per wave (one per 32-lane half).
+// The 32 lanes of a slot hold the reference's 32 per-lane random streams and
+// reproduce its reduction orders, so the results are bit-identical to the
+// CPU oracle (oracle/cuslines_oracle.cpp) on the same inputs.
+#include "device_math.h"
+
+#include "../../include/cuslines_hip.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace cusl;
+
+namespace {
+
+constexpr double PMF_THRESHOLD_P = 0.05;               // _globals.py:4
+constexpr float REAL_MAX = 3.40282346638528859812e+38f;
+constexpr float REAL_MIN = -REAL_MAX;                  // globals.h:41-42
+constexpr float K_SMALL = 0.0001f;                     // ptt.cuh:10
+constexpr float PTT_NORM_EPS = 1e-8f;                  // ptt.cuh:12
+constexpr float PTT_INV_NORM_EPS = 1e8f;               // ptt.cuh:13
+constexpr float CSA_C0 = 0.28209479177387814f;         // boot.cu:363
+constexpr int DISC_VERT_CNT = 24;                      // disc.h:20
+constexpr int DISC_FACE_CNT = 31;                      // disc.h:21
+constexpr int SORT_LAYERS = 15;
+
+// PTT disc tables (quality 2): numeric data of cuslines/cuda_c/disc.h:20-84
+// (nibrary, BSD-3-Clause, Copyright (c) 2024 Dogu Baran Aydogan).
+__constant__ float c_disc_vert[DISC_VERT_CNT * 2] = {
+    -0.99680788f, -0.07983759f, -0.94276539f, 0.33345677f, -0.87928469f, -0.47629658f,
+    -0.72856617f, 0.68497542f,  -0.60006556f, -0.79995082f, -0.54129995f, -0.02761342f,
+    -0.39271207f, 0.37117272f,  -0.39217391f, 0.91989110f,  -0.36362884f, -0.40757367f,
+    -0.22391316f, -0.97460910f, -0.00130022f, 0.53966106f,  0.00000000f,  0.00000000f,
+    0.00973999f,  0.99995257f,  0.01606516f,  -0.54289908f, 0.21342395f,  -0.97695968f,
+    0.38192071f,  -0.38666136f, 0.38897094f,  0.37442837f,  0.40696681f,  0.91344295f,
+    0.54387161f,  -0.01477123f, 0.59119367f,  -0.80652963f, 0.73955688f,  0.67309406f,
+    0.87601150f,  -0.48229022f, 0.94617928f,  0.32364298f,  0.99585368f,  -0.09096944f};
+__constant__ int c_disc_face[DISC_FACE_CNT * 3] = {
+    9, 8, 4,    11, 16, 10, 5, 8, 11,   5, 1, 0,    18, 16, 11, 11, 15, 18, 13, 8, 9,  11, 8, 13,
+    13, 15, 11, 22, 18, 23, 22, 20, 16, 16, 18, 22, 16, 20, 17, 12, 10, 17, 17, 10, 16, 15, 19, 21,
+    23, 18, 21, 21, 18, 15, 2, 4, 8,    2, 5, 0,    8, 5, 2,    7, 10, 12,  6, 7, 3,    10, 7, 6,
+    3, 1, 6,    1, 5, 6,    11, 10, 6,  6, 5, 11,   14, 19, 15, 15, 13, 14, 14, 13, 9};
+
+// Partner lane of every layer of the 32-key sort network; filled at tracker
+// creation from the generator below.
+__constant__ unsigned char c_sortnet[SORT_LAYERS][32];
+
+// Knuth's merge exchange (TAOCP 5.2.2 Algorithm M) for 32 keys with each
+// comparator placed in the earliest layer its inputs allow: the network of
+// the reference's warp sort tables (cuwsort.cuh:37-52).
+void make_sortnet(unsigned char out[SORT_LAYERS][32], int *nlayers) {
+    int ready[32] = {0};
+    for (int L = 0; L < SORT_LAYERS; ++L)
+        for (int l = 0; l < 32; ++l) out[L][l] = (unsigned char)l;
+    int nl = 0;
+    const int n = 32, t = 5;
+    for (int p = 1 << (t - 1); p > 0; p >>= 1) {
+        int q = 1 << (t - 1), r = 0, d = p;
+        for (;;) {
+            for (int i = 0; i < n - d; ++i) {
+                if ((i & p) != r) continue;
+                const int a = i, b = i + d;
+                const int L = std::max(ready[a], ready[b]);
+                out[L][a] = (unsigned char)b;
+                out[L][b] = (unsigned char)a;
+                ready[a] = ready[b] = L + 1;
+                nl = std::max(nl, L + 1);
+            }
+            if (q != p) { d = q - p; q >>= 1; r = p; } else break;
+        }
+    }
+    *nlayers = nl;
+}
+
+struct DevParams {
+    int dimx, dimy, dimz, dimt, t32;
+    int nverts, nv32, nedges, sphere_symm, max_sline_len, model;
+    int hr, hr32, delta_nr;
+    uint32_t seed_lo, seed_hi;
+    float tc_threshold, rel_peak, cos_max_angle, cos_min_sep, min_signal;
+    double step_d;
+    // PTT
+    int step_frac, probe_quality, lut_res, nx, ny, nz, log2x, log2y;
+    float probe_step, max_curv, min_support_f, ptt_seed_thr_f;
+    float ptt_sub_step;  // (float)(STEP_SIZE / STEP_FRAC)
+};
+
+struct DevBufs {
+    const float *data;
+    const float *stop;
+    const float *verts;
+    const uint32_t *edges;
+    const float *H, *R, *db, *dq, *smat;
+    const int *b0;
+    const int *dwi_idx;
+    const uint16_t *lut;
+};
+
+// -------------------------------------------------------------------------
+// Sorting, scans and peak extraction (per half-wave)
+// -------------------------------------------------------------------------
+
+// warp_sort<32,32,WSORT_DIR_DEC> (cuwsort.cuh:104-168) on (key, value).
+__device__ __forceinline__ void sort_dec(float &k, int &v) {
+    const int g = lane32();
+#pragma unroll
+    for (int L = 0; L < SORT_LAYERS; ++L) {
+        const int src = c_sortnet[L][g];
+        const float a = shfl32(k, src);
+        const int b = shfl32(v, src);
+        if (g >= src) {
+            if (a < k) { k = a; v = b; }
+        } else {
+            if (a > k) { k = a; v = b; }
+        }
+    }
+}
+
+// prefix_sum_sh_d (utils.cu:81-120): blocks of 32, lane 0 adds the previous
+// block's last value, Hillis-Steele inside the block.
+__device__ void prefix_sum(float *a, int len) {
+    const int l = lane32();
+    float carry = 0.0f;
+    for (int j = 0; j < len; j += 32) {
+        float v = (j + l < len) ? a[j + l] : 0.0f;
+        if (j != 0 && l == 0) v = v + carry;
+#pragma unroll
+        for (int i = 1; i < 32; i <<= 1) {
+            const float o = shfl_up32(v, i);
+            if (l >= i && j + l < len) v = v + o;
+        }
+        if (j + l < len) a[j + l] = v;
+        carry = shfl32(v, 31);
+    }
+    WAVE_SYNC();
+}
+
+// peak_directions_d (tracking_helpers.cu:75-245).  odf, marks and cand are
+// this slot's LDS arrays; the kept vertex indices end in cand[0..return).
+__device__ int peak_directions(const DevParams &P, const DevBufs &B, const float *odf, int *marks,
+                               int *cand, const float *sv) {
+    const int l = lane32();
+    const int nv = P.nverts;
+    float mn = REAL_MAX;
+    for (int j = l; j < nv; j += 32) mn = (mn < odf[j]) ? mn : odf[j];
+    mn = min32(mn);
+    mn = (0.0f > mn) ? 0.0f : mn;
+    for (int j = l; j < nv; j += 32) marks[j] = 0;
+    WAVE_SYNC();
+    // bit 1: has a strictly larger neighbour; bit 0: has a strictly smaller one.
+    // The reference's positive mark is exactly "bit 0 only".
+    for (int e = l; e < P.nedges; e += 32) {
+        const uint32_t uv = B.edges[e];
+        const int u = (int)(uv & 0xffffu), v = (int)(uv >> 16);
+        const float a = odf[u], b = odf[v];
+        if (a < b) { atomicOr(&marks[u], 2); atomicOr(&marks[v], 1); }
+        else if (b < a) { atomicOr(&marks[v], 2); atomicOr(&marks[u], 1); }
+    }
+    WAVE_SYNC();
+    float m = REAL_MIN;
+    for (int j = l; j < nv; j += 32)
+        if (marks[j] == 1) { const float x = odf[j] + (-mn); m = (m > x) ? m : x; }
+    m = max32(m);
+    const float comp = (float)((double)P.rel_peak * (double)m);
+    int n = 0;
+    for (int j0 = 0; j0 < nv; j0 += 32) {
+        const int j = j0 + l;
+        const bool keep = (j < nv) && (marks[j] == 1) && ((odf[j] - mn) >= comp);
+        const uint32_t msk = half_ballot(keep);
+        if (keep) cand[n + __popc(msk & ((1u << l) - 1u))] = j;
+        n += __popc(msk);
+    }
+    WAVE_SYNC();
+    if (n < 32) {
+        float k = REAL_MIN;
+        int v = 0;
+        if (l < n) { v = cand[l]; k = odf[v]; }
+        sort_dec(k, v);
+        if (l < n) cand[l] = v;
+        WAVE_SYNC();
+    }
+    if (n == 0) return 0;
+    int kept = 1;
+    if (l == 0) {
+        // remove_similar_vertices, single lane (tracking_helpers.cu:195-223),
+        // compacting the kept indices in place.
+        for (int i = 1; i < n; ++i) {
+            const int ci = cand[i];
+            const float ax = sv[3 * ci], ay = sv[3 * ci + 1], az = sv[3 * ci + 2];
+            int j = 0;
+            for (; j < kept; ++j) {
+                const int cj = cand[j];
+                float c = dot3(ax, ay, az, sv[3 * cj], sv[3 * cj + 1], sv[3 * cj + 2]);
+                if (P.sphere_symm) c = fabsf(c);
+                if (c > P.cos_min_sep) break;
+            }
+            if (j == kept) cand[kept++] = ci;
+        }
+    }
+    kept = shfl32(kept, 0);
+    WAVE_SYNC();
+    return kept;
+}
+
+// -------------------------------------------------------------------------
+// Interpolation and the stop check
+// -------------------------------------------------------------------------
+
+// trilinear_interp_d (tracking_helpers.cu:3-61) over `dimt` channels with
+// channel stride `ld`; writes out[0..dimt).
+__device__ int trilinear(const DevParams &P, const float *__restrict__ data, int ld, int dimt,
+                         float px, float py, float pz, float *out) {
+    if (px < -0.5f || px + 0.5f >= (float)P.dimx || py < -0.5f || py + 0.5f >= (float)P.dimy ||
+        pz < -0.5f || pz + 0.5f >= (float)P.dimz)
+        return -1;
+    const float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
+    const float wx1 = px - fx, wy1 = py - fy, wz1 = pz - fz;
+    const float wx0 = 1.0f - wx1, wy0 = 1.0f - wy1, wz0 = 1.0f - wz1;
+    const int x0 = (int)((0.0f > fx) ? 0.0f : fx), y0 = (int)((0.0f > fy) ? 0.0f : fy),
+              z0 = (int)((0.0f > fz) ? 0.0f : fz);
+    const int x1 = min(P.dimx - 1, x0 + 1), y1 = min(P.dimy - 1, y0 + 1), z1 = min(P.dimz - 1, z0 + 1);
+    const float w[8] = {(wx0 * wy0) * wz0, (wx0 * wy0) * wz1, (wx0 * wy1) * wz0, (wx0 * wy1) * wz1,
+                        (wx1 * wy0) * wz0, (wx1 * wy0) * wz1, (wx1 * wy1) * wz0, (wx1 * wy1) * wz1};
+    const size_t DY = (size_t)P.dimy, DZ = (size_t)P.dimz;
+    const float *c[8] = {
+        data + (((size_t)x0 * DY + y0) * DZ + z0) * ld, data + (((size_t)x0 * DY + y0) * DZ + z1) * ld,
+        data + (((size_t)x0 * DY + y1) * DZ + z0) * ld, data + (((size_t)x0 * DY + y1) * DZ + z1) * ld,
+        data + (((size_t)x1 * DY + y0) * DZ + z0) * ld, data + (((size_t)x1 * DY + y0) * DZ + z1) * ld,
+        data + (((size_t)x1 * DY + y1) * DZ + z0) * ld, data + (((size_t)x1 * DY + y1) * DZ + z1) * ld};
+    const int l = lane32();
+    for (int t = l; t < dimt; t += 32) {
+        float v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = __builtin_nontemporal_load(c[k] + t);
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc = __builtin_fmaf(w[k], v[k], acc);
+        out[t] = acc;
+    }
+    return 0;
+}
+
+// Texture coordinate -> (first texel, 8-bit weight of the second texel),
+// after the CUDA guide's linear filtering: xB = x - 0.5, i = floor(xB),
+// weight = frac(xB) with 8 fractional bits.
+__device__ __forceinline__ float tex_weight(float c, int *i0) {
+    const float b = c - 0.5f;
+    const float fl = floorf(b);
+    *i0 = (int)fl;
+    return __builtin_rintf((b - fl) * 256.0f) * 0.00390625f;
+}
+
+// check_point_d (tracking_helpers.cu:63-73): linear filter of the stop map,
+// border colour -1, texture x = array z (cu_tractography.py:237-241).
+__device__ int check_point(const DevParams &P, const float *__restrict__ stop, float px, float py,
+                           float pz) {
+    int iz, iy, ix;
+    const float a = tex_weight(pz, &iz), b = tex_weight(py, &iy), g = tex_weight(px, &ix);
+    const float a0 = 1.0f - a, b0 = 1.0f - b, g0 = 1.0f - g;
+    const int DX = P.dimx, DY = P.dimy, DZ = P.dimz;
+    auto fetch = [&](int z, int y, int x) -> float {
+        if (z < 0 || z >= DZ || y < 0 || y >= DY || x < 0 || x >= DX) return -1.0f;
+        return stop[((size_t)x * DY + y) * DZ + z];
+    };
+    float acc = ((a0 * b0) * g0) * fetch(iz, iy, ix);
+    acc = __builtin_fmaf((a * b0) * g0, fetch(iz + 1, iy, ix), acc);
+    acc = __builtin_fmaf((a0 * b) * g0, fetch(iz, iy + 1, ix), acc);
+    acc = __builtin_fmaf((a * b) * g0, fetch(iz + 1, iy + 1, ix), acc);
+    acc = __builtin_fmaf((a0 * b0) * g, fetch(iz, iy, ix + 1), acc);
+    acc = __builtin_fmaf((a * b0) * g, fetch(iz + 1, iy, ix + 1), acc);
+    acc = __builtin_fmaf((a0 * b) * g, fetch(iz, iy + 1, ix + 1), acc);
+    acc = __builtin_fmaf((a * b) * g, fetch(iz + 1, iy + 1, ix + 1), acc);
+    if (acc == -1.0f) return CUSLINES_OUTSIDEIMAGE;
+    return (acc > P.tc_threshold) ? CUSLINES_TRACKPOINT : CUSLINES_ENDPOINT;
+}
+
+// Texel index I of one axis of the vertex-tiled PTT texture -> (tile, local),
+// clamp address mode; `hint` is the tile the query was built from.
+__device__ __forceinline__ void axis_split(int I, int D, int ntiles, int hint, int *tile, int *loc) {
+    I = min(max(I, 0), ntiles * D - 1);
+    int t = hint, c = I - hint * D;
+    while (c < 0) { --t; c += D; }
+    while (c >= D) { ++t; c -= D; }
+    *tile = t;
+    *loc = c;
+}
+
+// Linear-filtered sample of the (nx,ny,nz)-tiled ODF texture at big-texture
+// coordinates (xq,yq,zq) built from tile (tx,ty,tz); slots >= nverts read 0.
+__device__ float tiled_sample(const DevParams &P, const float *__restrict__ odfv, float xq, float yq,
+                              float zq, int tx, int ty, int tz) {
+    int i, j, k;
+    const float a = tex_weight(xq, &i), b = tex_weight(yq, &j), g = tex_weight(zq, &k);
+    const float a0 = 1.0f - a, b0 = 1.0f - b, g0 = 1.0f - g;
+    int txa, txb, lxa, lxb, tya, tyb, lya, lyb, tza, tzb, lza, lzb;
+    axis_split(i, P.dimx, P.nx, tx, &txa, &lxa);
+    axis_split(i + 1, P.dimx, P.nx, tx, &txb, &lxb);
+    axis_split(j, P.dimy, P.ny, ty, &tya, &lya);
+    axis_split(j + 1, P.dimy, P.ny, ty, &tyb, &lyb);
+    axis_split(k, P.dimz, P.nz, tz, &tza, &lza);
+    axis_split(k + 1, P.dimz, P.nz, tz, &tzb, &lzb);
+    const int DX = P.dimx, DY = P.dimy, DZ = P.dimz, NV = P.nverts, nx = P.nx, ny = P.ny;
+    auto fetch = [&](int ti, int li, int tj, int lj, int tk, int lk) -> float {
+        const int v = ti + nx * (tj + ny * tk);
+        if (v >= NV) return 0.0f;
+        return odfv[(((size_t)v * DX + li) * DY + lj) * DZ + lk];
+    };
+    float acc = ((a0 * b0) * g0) * fetch(txa, lxa, tya, lya, tza, lza);
+    acc = __builtin_fmaf((a * b0) * g0, fetch(txb, lxb, tya, lya, tza, lza), acc);
+    acc = __builtin_fmaf((a0 * b) * g0, fetch(txa, lxa, tyb, lyb, tza, lza), acc);
+    acc = __builtin_fmaf((a * b) * g0, fetch(txb, lxb, tyb, lyb, tza, lza), acc);
+    acc = __builtin_fmaf((a0 * b0) * g, fetch(txa, lxa, tya, lya, tzb, lzb), acc);
+    acc = __builtin_fmaf((a * b0) * g, fetch(txb, lxb, tya, lya, tzb, lzb), acc);
+    acc = __builtin_fmaf((a0 * b) * g, fetch(txa, lxa, tyb, lyb, tzb, lzb), acc);
+    acc = __builtin_fmaf((a * b) * g, fetch(txb, lxb, tyb, lyb, tzb, lzb), acc);
+    return acc;
+}
+
+// interp4_d (ptt.cu:24-42): nearest vertex of the tangent from the LUT
+// (point sampling, normalised coordinates, clamp), then the tiled sample.
+__device__ __forceinline__ float interp4(const DevParams &P, const DevBufs &B, float px, float py,
+                                         float pz, float f0, float f1, float f2) {
+    const int res = P.lut_res;
+    const float rf = (float)res, rmax = (float)(res - 1);
+    const float u0 = __builtin_fmaf(f0, 0.5f, 0.5f), u1 = __builtin_fmaf(f1, 0.5f, 0.5f),
+                u2 = __builtin_fmaf(f2, 0.5f, 0.5f);
+    const int i0 = (int)fminf(fmaxf(floorf(u0 * rf), 0.0f), rmax);
+    const int i1 = (int)fminf(fmaxf(floorf(u1 * rf), 0.0f), rmax);
+    const int i2 = (int)fminf(fmaxf(floorf(u2 * rf), 0.0f), rmax);
+    const int idx = (int)B.lut[((size_t)i0 * res + i1) * res + i2];
+    const int tx = idx & (P.nx - 1);
+    const int ty = (idx >> P.log2x) & (P.ny - 1);
+    const int tz = idx >> (P.log2x + P.log2y);
+    const float xq = (float)(tx * P.dimx) + px;
+    const float yq = (float)(ty * P.dimy) + py;
+    const float zq = (float)(tz * P.dimz) + pz;
+    return tiled_sample(P, B.data, xq, yq, zq, tx, ty, tz);
+}
+
+// -------------------------------------------------------------------------
+// Half-wave matrix-vector products with the reference's per-row summation:
+// lane partials over j = l, l+32, ... (fma), then the 16/8/4/2/1 pair tree.
+// Rows are processed 32 at a time and reduced by a butterfly reduce-scatter
+// (31 shuffles per 32 rows); lane r ends with row r's sum.
+// XF selects the elementwise transform of the vector: 0 none, 1 OPDT, 2 CSA.
+// -------------------------------------------------------------------------
+template <int XF>
+__device__ __forceinline__ float gemv_xf(float x) {
+    if (XF == 1) {
+        const float lx = ulog(x);
+        return ((-lx) * (1.5f + lx)) * x;
+    } else if (XF == 2) {
+        const float c = fminf(fmaxf(x, 0.001f), 0.999f);
+        return ulog(-ulog(c));
+    }
+    return x;
+}
+
+template <int XF>
+__device__ void gemv_rows(int N, int M, const float *v, const float *__restrict__ mat, float *out) {
+    const int l = lane32();
+    for (int i0 = 0; i0 < N; i0 += 32) {
+        float acc[32];
+#pragma unroll
+        for (int r = 0; r < 32; ++r) acc[r] = 0.0f;
+        for (int j = l; j < M; j += 32) {
+            const float x = gemv_xf<XF>(v[j]);
+#pragma unroll
+            for (int r = 0; r < 32; ++r)
+                if (i0 + r < N) acc[r] = __builtin_fmaf(x, mat[(size_t)(i0 + r) * M + j], acc[r]);
+        }
+        // reduce-scatter: after the stage with distance d each lane keeps d rows
+#pragma unroll
+        for (int d = 16; d >= 1; d >>= 1) {
+            const bool up = (l & d) != 0;
+#pragma unroll
+            for (int r = 0; r < d; ++r) {
+                const float send = up ? acc[r] : acc[r + d];
+                const float keep = up ? acc[r + d] : acc[r];
+                acc[r] = keep + shfl_xor32(send, d);
+            }
+        }
+        if (i0 + l < N) out[i0 + l] = acc[0];
+    }
+    WAVE_SYNC();
+}
+
+// -------------------------------------------------------------------------
+// Probabilistic getter (generate_streamlines_cuda.cu:49-275)
+// -------------------------------------------------------------------------
+struct ProbLds {
+    float *pmf;
+    int *marks, *cand;
+};
+
+template <bool IS_START>
+__device__ int get_direction_prob(const DevParams &P, const DevBufs &B, const float *sv, ProbLds &S,
+                                  Philox &rng, float dx, float dy, float dz, float px, float py,
+                                  float pz, float *ndx, float *ndy, float *ndz) {
+    const int l = lane32();
+    const int nv = P.nverts;
+    float *pmf = S.pmf;
+    if (trilinear(P, B.data, P.t32, P.dimt, px, py, pz, pmf) != 0) return 0;
+    WAVE_SYNC();
+    float mx = REAL_MIN;
+    for (int i = l; i < nv; i += 32) mx = (mx > pmf[i]) ? mx : pmf[i];
+    mx = max32(mx);
+    const float thr = (float)(PMF_THRESHOLD_P * (double)mx);
+    if (IS_START) {
+        for (int i = l; i < nv; i += 32)
+            if (pmf[i] < thr) pmf[i] = 0.0f;
+        WAVE_SYNC();
+        return peak_directions(P, B, pmf, S.marks, S.cand, sv);
+    }
+    for (int i = l; i < nv; i += 32) {
+        float v = pmf[i];
+        if (v < thr) v = 0.0f;
+        float d = dot3(dx, dy, dz, sv[3 * i], sv[3 * i + 1], sv[3 * i + 2]);
+        if (P.sphere_symm) d = fabsf(d);
+        if (d < P.cos_max_angle) v = 0.0f;
+        pmf[i] = v;
+    }
+    WAVE_SYNC();
+    prefix_sum(pmf, nv);
+    const float last = pmf[nv - 1];
+    if (last == 0.0f) return 0;
+    float sel = 0.0f;
+    if (l == 0) sel = rng.uniform() * last;
+    sel = shfl32(sel, 0);
+    int low = 0, high = nv - 1;
+    while ((high - low) >= 32) {
+        const int mid = (low + high) / 2;
+        if (pmf[mid] < sel) low = mid; else high = mid;
+    }
+    const bool pr = (low + l <= high) ? (sel < pmf[low + l]) : false;
+    const uint32_t msk = half_ballot(pr);
+    int ind = low + __ffs(msk) - 1;
+    if (ind < 0) ind = 0;
+    const float vx = sv[3 * ind], vy = sv[3 * ind + 1], vz = sv[3 * ind + 2];
+    if (dot3(dx, dy, dz, vx, vy, vz) > 0.0f) { *ndx = vx; *ndy = vy; *ndz = vz; }
+    else { *ndx = -vx; *ndy = -vy; *ndz = -vz; }
+    WAVE_SYNC();
+    return 1;
+}
+
+// -------------------------------------------------------------------------
+// Bootstrap getter (boot.cu:19-113, 115-193, 330-608)
+// -------------------------------------------------------------------------
+struct BootLds {
+    float *vox, *msk;  // region A: vox [t32], msk [hr32]
+    float *r;          // region B: R.d / model coefficients; later peak candidates
+    float *h;          // region C: H.d / ODF (pmf)
+    int *marks;        // region A reused as peak marks
+    int *cand;         // region B reused as peak candidates
+};
+
+// closest_peak_d (boot.cu:115-193): lane-strided best |dot| with strict
+// improvement, then an xor tree with the same rule; lane 0's winner counts.
+__device__ int closest_peak(const DevParams &P, const float *sv, float dx, float dy, float dz, int npeaks,
+                            const int *peaks, float *ox, float *oy, float *oz) {
+    const int l = lane32();
+    float cd = 0.0f;
+    int ci = -1;
+    for (int j = l; j < npeaks; j += 32) {
+        const int v = peaks[j];
+        const float d = dot3(dx, dy, dz, sv[3 * v], sv[3 * v + 1], sv[3 * v + 2]);
+        if (fabsf(d) > fabsf(cd)) { cd = d; ci = j; }
+    }
+    for (int m = 16; m; m >>= 1) {
+        const float od = shfl_xor32(cd, m);
+        const int oi = shfl_xor32(ci, m);
+        if (fabsf(od) > fabsf(cd)) { cd = od; ci = oi; }
+    }
+    cd = shfl32(cd, 0);
+    ci = shfl32(ci, 0);
+    if (ci >= 0) {
+        const int v = peaks[ci];
+        if (cd >= P.cos_max_angle) { *ox = sv[3 * v]; *oy = sv[3 * v + 1]; *oz = sv[3 * v + 2]; return 1; }
+        if (cd <= -P.cos_max_angle) { *ox = -sv[3 * v]; *oy = -sv[3 * v + 1]; *oz = -sv[3 * v + 2]; return 1; }
+    }
+    return 0;
+}
+
+template <int MODEL>
+__device__ int get_direction_boot(const DevParams &P, const DevBufs &B, const float *sv, BootLds &S,
+                                  Philox &rng, int nattempts, float dx, float dy, float dz, float px,
+                                  float py, float pz, float *ndx, float *ndy, float *ndz) {
+    const int l = lane32();
+    const int hr = P.hr, dn = P.delta_nr, nv = P.nverts, T = P.dimt;
+    for (int att = 0; att < nattempts; ++att) {
+        const int rv = trilinear(P, B.data, P.t32, T, px, py, pz, S.vox);
+        WAVE_SYNC();
+        if (rv == 0) {
+            for (int m = l; m < hr; m += 32) S.msk[m] = S.vox[B.dwi_idx[m]];
+            WAVE_SYNC();
+            gemv_rows<0>(hr, hr, S.msk, B.R, S.r);
+            gemv_rows<0>(hr, hr, S.msk, B.H, S.h);
+            for (int j = l; j < hr; j += 32) {
+                const int src = (int)(rng.next() % (uint32_t)hr);
+                S.h[j] = S.h[j] + S.r[src];
+            }
+            WAVE_SYNC();
+            for (int m = l; m < hr; m += 32) S.vox[B.dwi_idx[m]] = S.h[m];
+            WAVE_SYNC();
+            float part = 0.0f;
+            int cnt = 0;
+            for (int j = l; j < T; j += 32) {
+                float v = S.vox[j];
+                v = (P.min_signal > v) ? P.min_signal : v;
+                S.vox[j] = v;
+                if (B.b0[j]) { ++cnt; part = part + v; }
+            }
+            const float sum = tree_sum32(part);
+            for (int m = 16; m; m >>= 1) cnt += shfl_xor32(cnt, m);
+            const float denom = sum / (float)cnt;
+            for (int j = l; j < T; j += 32) S.vox[j] = S.vox[j] / denom;
+            WAVE_SYNC();
+            for (int m = l; m < hr; m += 32) S.msk[m] = S.vox[B.dwi_idx[m]];
+            WAVE_SYNC();
+            if (MODEL == CUSLINES_OPDT) {
+                gemv_rows<1>(dn, hr, S.msk, B.dq, S.r);
+                gemv_rows<0>(dn, hr, S.msk, B.db, S.h);
+                for (int j = l; j < dn; j += 32) S.r[j] = S.r[j] - S.h[j];
+            } else {
+                gemv_rows<2>(dn, hr, S.msk, B.dq, S.r);
+                if (l == 0) S.r[0] = CSA_C0;
+            }
+            WAVE_SYNC();
+            gemv_rows<0>(nv, dn, S.r, B.smat, S.h);
+        } else {
+            for (int j = l; j < nv; j += 32) S.h[j] = 0.0f;
+            WAVE_SYNC();
+        }
+        float mx = REAL_MIN;
+        for (int j = l; j < nv; j += 32) mx = (mx > S.h[j]) ? mx : S.h[j];
+        mx = max32(mx);
+        const float thr = (float)(PMF_THRESHOLD_P * (double)mx);
+        for (int j = l; j < nv; j += 32)
+            if (S.h[j] < thr) S.h[j] = 0.0f;
+        WAVE_SYNC();
+        const int ndir = peak_directions(P, B, S.h, S.marks, S.cand, sv);
+        if (nattempts == 1) return ndir;
+        if (ndir > 0 && closest_peak(P, sv, dx, dy, dz, ndir, S.cand, ndx, ndy, ndz)) {
+            WAVE_SYNC();
+            return 1;
+        }
+        WAVE_SYNC();
+    }
+    return 0;
+}
+
+// -------------------------------------------------------------------------
+// PTT getter (ptt.cu)
+// -------------------------------------------------------------------------
+template <int FAIL>
+__device__ __forceinline__ void norm3(float &x, float &y, float &z) {
+    const float inv = 1.0f / __builtin_sqrtf(__builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x)));
+    if (__builtin_isfinite(inv) && inv > 0.0f && inv < PTT_INV_NORM_EPS) {
+        x *= inv; y *= inv; z *= inv;
+    } else {
+        x = (FAIL == 0) ? 1.0f : 0.0f;
+        y = (FAIL == 1) ? 1.0f : 0.0f;
+        z = (FAIL == 2) ? 1.0f : 0.0f;
+    }
+}
+
+template <int FAIL>
+__device__ __forceinline__ void crossnorm3(float *d, const float *a, const float *b) {
+    d[0] = __builtin_fmaf(a[1], b[2], -(a[2] * b[1]));
+    d[1] = __builtin_fmaf(a[2], b[0], -(a[0] * b[2]));
+    d[2] = __builtin_fmaf(a[0], b[1], -(a[1] * b[0]));
+    norm3<FAIL>(d[0], d[1], d[2]);
+}
+
+__device__ __forceinline__ void prepare_propagator(float k1, float k2, float arclen, float *p) {
+    if (fabsf(k1) < K_SMALL && fabsf(k2) < K_SMALL) {
+        p[0] = arclen; p[1] = 0.0f; p[2] = 0.0f; p[3] = 1.0f; p[4] = 0.0f;
+        p[5] = 0.0f; p[6] = 0.0f; p[7] = 0.0f; p[8] = 1.0f;
+        return;
+    }
+    if (fabsf(k1) < K_SMALL) k1 = K_SMALL;
+    if (fabsf(k2) < K_SMALL) k2 = K_SMALL;
+    const float k = __builtin_sqrtf(__builtin_fmaf(k2, k2, k1 * k1));
+    float sinkt, coskt;
+    usincos(k * arclen, &sinkt, &coskt);
+    const float kk = 1.0f / (k * k);
+    p[0] = sinkt / k;
+    p[1] = (k1 * (1.0f - coskt)) * kk;
+    p[2] = (k2 * (1.0f - coskt)) * kk;
+    p[3] = coskt;
+    p[4] = (k1 * sinkt) / k;
+    p[5] = (k2 * sinkt) / k;
+    p[6] = -p[5];
+    p[7] = ((k1 * k2) * (coskt - 1.0f)) * kk;
+    p[8] = __builtin_fmaf(k2 * k2, coskt, k1 * k1) * kk;
+}
+
+__device__ __forceinline__ void propagate_frame(const float *p, float *f, float *direc) {
+    float tmp[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        direc[i] = __builtin_fmaf(p[2], f[6 + i], __builtin_fmaf(p[1], f[3 + i], p[0] * f[i]));
+        tmp[i] = __builtin_fmaf(p[5], f[6 + i], __builtin_fmaf(p[4], f[3 + i], p[3] * f[i]));
+        f[6 + i] = __builtin_fmaf(p[8], f[6 + i], __builtin_fmaf(p[7], f[3 + i], p[6] * f[i]));
+    }
+    norm3<0>(tmp[0], tmp[1], tmp[2]);
+    crossnorm3<1>(f + 3, f + 6, tmp);
+    crossnorm3<2>(f + 6, tmp, f + 3);
+    f[0] = tmp[0]; f[1] = tmp[1]; f[2] = tmp[2];
+}
+
+template <bool INIT>
+__device__ __forceinline__ void probing_frame(const float *frame, Philox &rng, float *pf) {
+    if (INIT) {
+        pf[0] = frame[0]; pf[1] = frame[1]; pf[2] = frame[2];
+        norm3<0>(pf[0], pf[1], pf[2]);
+        pf[3] = rng.normal();
+        pf[4] = rng.normal();
+        pf[5] = rng.normal();
+        const float dt = __builtin_fmaf(pf[5], pf[2], __builtin_fmaf(pf[4], pf[1], pf[3] * pf[0]));
+        pf[3] = __builtin_fmaf(-dt, pf[0], pf[3]);
+        pf[4] = __builtin_fmaf(-dt, pf[1], pf[4]);
+        pf[5] = __builtin_fmaf(-dt, pf[2], pf[5]);
+        const float n2 = __builtin_fmaf(pf[5], pf[5], __builtin_fmaf(pf[4], pf[4], pf[3] * pf[3]));
+        if (n2 < PTT_NORM_EPS) {
+            const float ax = fabsf(pf[0]), ay = fabsf(pf[1]), az = fabsf(pf[2]);
+            if (ax <= ay && ax <= az) { pf[3] = 0.0f; pf[4] = pf[2]; pf[5] = -pf[1]; }
+            else if (ay <= az) { pf[3] = -pf[2]; pf[4] = 0.0f; pf[5] = pf[0]; }
+            else { pf[3] = pf[1]; pf[4] = -pf[0]; pf[5] = 0.0f; }
+        }
+        norm3<1>(pf[3], pf[4], pf[5]);
+        crossnorm3<2>(pf + 6, pf, pf + 3);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) pf[i] = frame[i];
+    }
+}
+
+// calculate_data_support_d (ptt.cu:156-192)
+__device__ __forceinline__ float data_support(const DevParams &P, const DevBufs &B, float support,
+                                              float px, float py, float pz, float k1, float k2, float *pf) {
+    float prop[9], direc[3];
+    prepare_propagator(k1, k2, P.probe_step, prop);
+    for (int i = 0; i < P.probe_quality; ++i) {
+        propagate_frame(prop, pf, direc);
+        px += direc[0]; py += direc[1]; pz += direc[2];
+        support += interp4(P, B, px, py, pz, pf[0], pf[1], pf[2]);
+    }
+    return support;
+}
+
+struct PttLds {
+    float *frame;     // 18 floats: [0..8] backward frame, [9..17] forward frame
+    float *vert_pdf;  // 32
+    float *face_cdf;  // 32
+};
+
+// get_direction_ptt_d (ptt.cu:194-412); `frame` is the 9-float frame in use.
+template <bool INIT>
+__device__ int get_direction_ptt(const DevParams &P, const DevBufs &B, PttLds &S, float *frame,
+                                 Philox &rng, float dx, float dy, float dz, float px, float py, float pz,
+                                 float *ndx, float *ndy, float *ndz) {
+    const int l = lane32();
+    if (INIT) {
+        if (l == 0) { frame[0] = dx; frame[1] = dy; frame[2] = dz; }
+        WAVE_SYNC();
+    }
+    float fr[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) fr[i] = frame[i];
+    const float first_val = interp4(P, B, px, py, pz, fr[0], fr[1], fr[2]);
+    bool found = false;
+    if (l < DISC_VERT_CNT) {
+        const float k1 = (float)((double)c_disc_vert[2 * l] * (double)P.max_curv);
+        const float k2 = (float)((double)c_disc_vert[2 * l + 1] * (double)P.max_curv);
+        float pf[9];
+        probing_frame<INIT>(fr, rng, pf);
+        const float sup = data_support(P, B, first_val, px, py, pz, k1, k2, pf);
+        const bool ok = !(sup < P.min_support_f);
+        S.vert_pdf[l] = ok ? sup : 0.0f;
+        found = ok;
+    }
+    if (half_ballot(found) == 0u) return 0;
+    WAVE_SYNC();
+    float face = 0.0f;
+    if (l < DISC_FACE_CNT) {
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj) {
+            const float vv = S.vert_pdf[c_disc_face[l * 3 + jj]];
+            if (vv == 0.0f) { face = 0.0f; break; }
+            face = face + vv;
+        }
+    }
+    // prefix_sum_sh_d over 31 faces: a single block of 32
+#pragma unroll
+    for (int i = 1; i < 32; i <<= 1) {
+        const float o = shfl_up32(face, i);
+        if (l >= i && l < DISC_FACE_CNT) face = face + o;
+    }
+    if (l < DISC_FACE_CNT) S.face_cdf[l] = face;
+    const float last = shfl32(face, DISC_FACE_CNT - 1);
+    WAVE_SYNC();
+    if (last == 0.0f) return 0;
+    const double mc = (double)P.max_curv;
+    for (int round = 0; round < 1024 / 32; ++round) {
+        float r1 = rng.uniform(), r2 = rng.uniform();
+        if (r1 + r2 > 1.0f) { r1 = 1.0f - r1; r2 = 1.0f - r2; }
+        const float tmp = rng.uniform() * last;
+        int jj = 0;
+        for (; jj < DISC_FACE_CNT - 1; ++jj)
+            if (S.face_cdf[jj] >= tmp) break;
+        const int f0 = c_disc_face[jj * 3], f1 = c_disc_face[jj * 3 + 1], f2 = c_disc_face[jj * 3 + 2];
+        const float vx0 = (float)(mc * (double)c_disc_vert[f0 * 2]);
+        const float vx1 = (float)(mc * (double)c_disc_vert[f1 * 2]);
+        const float vx2 = (float)(mc * (double)c_disc_vert[f2 * 2]);
+        const float vy0 = (float)(mc * (double)c_disc_vert[f0 * 2 + 1]);
+        const float vy1 = (float)(mc * (double)c_disc_vert[f1 * 2 + 1]);
+        const float vy2 = (float)(mc * (double)c_disc_vert[f2 * 2 + 1]);
+        const float k1 = __builtin_fmaf(r2, vx2 - vx0, __builtin_fmaf(r1, vx1 - vx0, vx0));
+        const float k2 = __builtin_fmaf(r2, vy2 - vy0, __builtin_fmaf(r1, vy1 - vy0, vy0));
+        float pf[9];
+        probing_frame<INIT>(fr, rng, pf);
+        const float sup = data_support(P, B, first_val, px, py, pz, k1, k2, pf);
+        uint32_t msk;
+        if (INIT) {
+            float mx = sup;
+            for (int m = 16; m; m >>= 1) mx = fmaxf(mx, shfl_xor32(mx, m));
+            msk = half_ballot(sup == mx && !(mx < P.min_support_f));
+        } else {
+            msk = half_ballot(!(sup < P.min_support_f));
+        }
+        if (msk != 0u) {
+            const int win = __ffs(msk) - 1;
+            float ox = dx, oy = dy, oz = dz;
+            if (l == win) {
+                if (!INIT) {
+                    float prop[9], d[3];
+                    prepare_propagator(k1, k2, P.ptt_sub_step, prop);
+#pragma unroll
+                    for (int i = 0; i < 9; ++i) pf[i] = fr[i];
+                    propagate_frame(prop, pf, d);
+                    norm3<0>(d[0], d[1], d[2]);
+                    ox = d[0]; oy = d[1]; oz = d[2];
+                }
+#pragma unroll
+                for (int i = 0; i < 9; ++i) frame[i] = pf[i];
+            }
+            *ndx = shfl32(ox, win);
+            *ndy = shfl32(oy, win);
+            *ndz = shfl32(oz, win);
+            WAVE_SYNC();
+            return 1;
+        }
+    }
+    return 0;
+}
+
+// init_frame_ptt_d (ptt.cu:415-474); frame[15..17] is set to the binormal.
+__device__ bool init_frame_ptt(const DevParams &P, const DevBufs &B, PttLds &S, Philox &rng,
+                               float fx, float fy, float fz, float sx, float sy, float sz) {
+    const int l = lane32();
+    float tx, ty, tz;
+    bool ok = get_direction_ptt<true>(P, B, S, S.frame, rng, fx, fy, fz, sx, sy, sz, &tx, &ty, &tz);
+    if (!ok) {
+        ok = get_direction_ptt<true>(P, B, S, S.frame, rng, -fx, -fy, -fz, sx, sy, sz, &tx, &ty, &tz);
+        if (!ok) return false;
+    } else {
+        if (l == 0)
+            for (int i = 0; i < 6; ++i) S.frame[i] = -S.frame[i];
+        WAVE_SYNC();
+    }
+    if (l == 0) {
+        for (int i = 0; i < 6; ++i) S.frame[9 + i] = -S.frame[i];
+        for (int i = 0; i < 3; ++i) S.frame[15 + i] = S.frame[6 + i];
+    }
+    WAVE_SYNC();
+    return true;
+}
+
+// -------------------------------------------------------------------------
+// Tracker (generate_streamlines_cuda.cu:277-421, boot.cu:720-824)
+// -------------------------------------------------------------------------
+struct SlotLds {
+    ProbLds prob;
+    BootLds boot;
+    PttLds ptt;
+};
+
+template <int MODEL>
+__device__ int tracker(const DevParams &P, const DevBufs &B, const float *sv, SlotLds &S, Philox &rng,
+                       float sx, float sy, float sz, float dx, float dy, float dz, float *frame,
+                       float *__restrict__ sl) {
+    const int l = lane32();
+    const int step_frac = (MODEL == CUSLINES_PTT) ? P.step_frac : 1;
+    const int MAXL = P.max_sline_len;
+    const double sub = P.step_d / (double)step_frac;
+    float px = sx, py = sy, pz = sz;
+    if (l == 0) { sl[0] = px; sl[1] = py; sl[2] = pz; }
+    int i;
+    for (i = 1; i < MAXL * step_frac; ++i) {
+        float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+        int ndir;
+        if (MODEL == CUSLINES_PROB)
+            ndir = get_direction_prob<false>(P, B, sv, S.prob, rng, dx, dy, dz, px, py, pz, &nx, &ny, &nz);
+        else if (MODEL == CUSLINES_PTT)
+            ndir = get_direction_ptt<false>(P, B, S.ptt, frame, rng, dx, dy, dz, px, py, pz, &nx, &ny, &nz);
+        else
+            ndir = get_direction_boot<MODEL>(P, B, sv, S.boot, rng, 5, dx, dy, dz, px, py, pz, &nx, &ny, &nz);
+        if (ndir == 0) break;
+        dx = nx; dy = ny; dz = nz;
+        px = (float)__builtin_fma((double)(dx / 1.0f), sub, (double)px);
+        py = (float)__builtin_fma((double)(dy / 1.0f), sub, (double)py);
+        pz = (float)__builtin_fma((double)(dz / 1.0f), sub, (double)pz);
+        if (l == 0 && (i % step_frac) == 0) {
+            float *o = sl + 3 * (size_t)(i / step_frac);
+            o[0] = px; o[1] = py; o[2] = pz;
+        }
+        const int tissue = check_point(P, B.stop, px, py, pz);
+        if (tissue == CUSLINES_ENDPOINT || tissue == CUSLINES_INVALIDPOINT || tissue == CUSLINES_OUTSIDEIMAGE)
+            break;
+    }
+    int nsteps = i / step_frac;
+    if ((i % step_frac) != 0 && i < step_frac * (MAXL - 1)) {
+        ++nsteps;
+        if (l == 0) {
+            float *o = sl + 3 * (size_t)nsteps;
+            o[0] = px; o[1] = py; o[2] = pz;
+        }
+    }
+    return nsteps;
+}
+
+// -------------------------------------------------------------------------
+// Kernels
+// -------------------------------------------------------------------------
+
+// Orders memory traffic of one wave around cross-lane reuse of global data
+// (points written by lane 0 and read back by every lane of the slot).
+#define MEM_SYNC()                                             \
+    do {                                                       \
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); \
+        __builtin_amdgcn_wave_barrier();                       \
+    } while (0)
+
+__host__ __device__ inline int boot_region_a(int nv32, int t32, int hr32) {
+    return (nv32 > t32 + hr32) ? nv32 : t32 + hr32;
+}
+
+// Floats of LDS per seed slot.
+__host__ __device__ inline int slot_floats(int model, bool phase1, int nv32, int t32, int hr32) {
+    if (model == CUSLINES_PTT) return phase1 ? 3 * nv32 : 96;
+    if (model == CUSLINES_PROB) return phase1 ? 3 * nv32 : nv32;
+    const int bc = (nv32 > hr32) ? nv32 : hr32;
+    return boot_region_a(nv32, t32, hr32) + 2 * bc;
+}
+
+template <int MODEL>
+__device__ void bind_slot(const DevParams &P, bool phase1, float *base, SlotLds &S) {
+    if (MODEL == CUSLINES_PTT && !phase1) {
+        S.ptt.frame = base;
+        S.ptt.vert_pdf = base + 32;
+        S.ptt.face_cdf = base + 64;
+    } else if (MODEL == CUSLINES_PROB || MODEL == CUSLINES_PTT) {
+        S.prob.pmf = base;
+        S.prob.marks = reinterpret_cast<int *>(base + P.nv32);
+        S.prob.cand = reinterpret_cast<int *>(base + 2 * P.nv32);
+    } else {
+        const int a = boot_region_a(P.nv32, P.t32, P.hr32);
+        const int bc = (P.nv32 > P.hr32) ? P.nv32 : P.hr32;
+        S.boot.vox = base;
+        S.boot.msk = base + P.t32;
+        S.boot.marks = reinterpret_cast<int *>(base);
+        S.boot.r = base + a;
+        S.boot.cand = reinterpret_cast<int *>(base + a);
+        S.boot.h = base + a + bc;
+    }
+}
+
+__device__ __forceinline__ int verts_floats(const DevParams &P) { return (3 * P.nverts + 3) & ~3; }
+
+// getNumStreamlines{Prob,Boot,Ptt}_k (generate_streamlines_cuda.cu:423-462,
+// boot.cu:610-718, ptt_init.cu:1-63): initial peak directions per seed.
+template <int MODEL>
+__global__ void __launch_bounds__(256) phase1_kernel(DevParams P, DevBufs B, const float *__restrict__ seeds,
+                                                     int nseeds, int rng_chunk, int *__restrict__ counts,
+                                                     int *__restrict__ dirs) {
+    extern __shared__ float smem[];
+    float *sv = smem;
+    for (int i = threadIdx.x; i < 3 * P.nverts; i += blockDim.x) sv[i] = B.verts[i];
+    __syncthreads();
+    const int nslots = blockDim.x >> 5;
+    const int slot = threadIdx.x >> 5;
+    const int s = blockIdx.x * nslots + slot;
+    if (s >= nseeds) return;
+    const int l = lane32();
+    float *base = smem + verts_floats(P) + slot * slot_floats(MODEL, true, P.nv32, P.t32, P.hr32);
+    SlotLds S;
+    bind_slot<MODEL>(P, true, base, S);
+    const float px = seeds[3 * s], py = seeds[3 * s + 1], pz = seeds[3 * s + 2];
+    Philox rng;
+    rng.init(((uint64_t)P.seed_hi << 32) | P.seed_lo, (uint64_t)32 * (uint64_t)(s % rng_chunk) + l);
+    int n;
+    const int *kept;
+    if (MODEL == CUSLINES_PROB) {
+        float a, b, c;
+        n = get_direction_prob<true>(P, B, sv, S.prob, rng, 0.0f, 0.0f, 0.0f, px, py, pz, &a, &b, &c);
+        kept = S.prob.cand;
+    } else if (MODEL == CUSLINES_PTT) {
+        float *pmf = S.prob.pmf;
+        for (int i = l; i < P.nverts; i += 32) {
+            const int tx = i & (P.nx - 1), ty = (i >> P.log2x) & (P.ny - 1), tz = i >> (P.log2x + P.log2y);
+            const float v = tiled_sample(P, B.data, (float)(tx * P.dimx) + px, (float)(ty * P.dimy) + py,
+                                         (float)(tz * P.dimz) + pz, tx, ty, tz);
+            pmf[i] = (v < P.ptt_seed_thr_f) ? 0.0f : v;
+        }
+        WAVE_SYNC();
+        n = peak_directions(P, B, pmf, S.prob.marks, S.prob.cand, sv);
+        kept = S.prob.cand;
+    } else {
+        float a, b, c;
+        n = get_direction_boot<MODEL>(P, B, sv, S.boot, rng, 1, 0.0f, 0.0f, 0.0f, px, py, pz, &a, &b, &c);
+        kept = S.boot.cand;
+    }
+    int *out = dirs + (size_t)s * P.nverts;
+    for (int k = l; k < n; k += 32) out[k] = kept[k];
+    if (l == 0) counts[s] = n;
+}
+
+// genStreamlinesMerge{Prob,Boot}_k (generate_streamlines_cuda.cu:464-622,
+// boot.cu:826-980): per direction, backward track, reverse, forward track.
+template <int MODEL>
+__global__ void __launch_bounds__(256) phase2_kernel(DevParams P, DevBufs B, const float *__restrict__ seeds,
+                                                     int nseeds, int rng_chunk, const int *__restrict__ offsets,
+                                                     const int *__restrict__ dirs, int *__restrict__ sline_seed,
+                                                     int *__restrict__ sline_len, float *__restrict__ sline) {
+    extern __shared__ float smem[];
+    float *sv = smem;
+    int svf = 0;
+    if (MODEL != CUSLINES_PTT) {
+        for (int i = threadIdx.x; i < 3 * P.nverts; i += blockDim.x) sv[i] = B.verts[i];
+        svf = verts_floats(P);
+    }
+    __syncthreads();
+    const int nslots = blockDim.x >> 5;
+    const int slot = threadIdx.x >> 5;
+    const int s = blockIdx.x * nslots + slot;
+    if (s >= nseeds) return;
+    const int l = lane32();
+    float *base = smem + svf + slot * slot_floats(MODEL, false, P.nv32, P.t32, P.hr32);
+    SlotLds S;
+    bind_slot<MODEL>(P, false, base, S);
+    const float sx = seeds[3 * s], sy = seeds[3 * s + 1], sz = seeds[3 * s + 2];
+    Philox rng;
+    rng.init(((uint64_t)P.seed_hi << 32) | P.seed_lo, (uint64_t)32 * (uint64_t)(s % rng_chunk) + l + 1);
+    const int ndir = offsets[s + 1] - offsets[s];
+    int off = offsets[s];
+    const size_t slot_f = (size_t)2 * P.max_sline_len * 3;
+    for (int i = 0; i < ndir; ++i) {
+        const int vi = dirs[(size_t)s * P.nverts + i];
+        const float fx = B.verts[3 * vi], fy = B.verts[3 * vi + 1], fz = B.verts[3 * vi + 2];
+        float *cur = sline + (size_t)off * slot_f;
+        if (l == 0) sline_seed[off] = s;
+        if (MODEL == CUSLINES_PTT) {
+            if (!init_frame_ptt(P, B, S.ptt, rng, fx, fy, fz, sx, sy, sz)) {
+                if (l == 0) {
+                    sline_len[off] = 1;
+                    cur[0] = sx; cur[1] = sy; cur[2] = sz;
+                }
+                ++off;
+                continue;
+            }
+        }
+        const int nb = tracker<MODEL>(P, B, sv, S, rng, sx, sy, sz, -fx, -fy, -fz, S.ptt.frame, cur);
+        MEM_SYNC();
+        for (int j = l; j < nb / 2; j += 32) {
+            float *a = cur + 3 * (size_t)j, *b = cur + 3 * (size_t)(nb - 1 - j);
+            const float a0 = a[0], a1 = a[1], a2 = a[2];
+            a[0] = b[0]; a[1] = b[1]; a[2] = b[2];
+            b[0] = a0; b[1] = a1; b[2] = a2;
+        }
+        MEM_SYNC();
+        const int nf = tracker<MODEL>(P, B, sv, S, rng, sx, sy, sz, fx, fy, fz,
+                                      (MODEL == CUSLINES_PTT) ? S.ptt.frame + 9 : nullptr,
+                                      cur + 3 * (size_t)(nb - 1));
+        if (l == 0) sline_len[off] = nb - 1 + nf;
+        ++off;
+    }
+}
+
+// PTT data: [X][Y][Z][T] -> vertex-major [T][X][Y][Z] (32x32 LDS tiles).
+__global__ void transpose_vox_t(const float *__restrict__ src, float *__restrict__ dst, long long nvox, int T) {
+    __shared__ float tile[32][33];
+    const long long v0 = (long long)blockIdx.x * 32;
+    const int t0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    for (int k = ty; k < 32; k += 8) {
+        const long long v = v0 + k;
+        const int t = t0 + tx;
+        if (v < nvox && t < T) tile[k][tx] = src[v * T + t];
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int t = t0 + k;
+        const long long v = v0 + tx;
+        if (v < nvox && t < T) dst[(long long)t * nvox + v] = tile[tx][k];
+    }
+}
+
+// Exclusive scans of int32 counts (1024 elements per block, three passes).
+template <typename TO>
+__device__ TO block_incl_scan(TO x, TO *wsum) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const TO o = __shfl_up(x, d, 64);
+        if (lane >= d) x += o;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    if (w == 0) {
+        TO y = (lane < (int)(blockDim.x >> 6)) ? wsum[lane] : (TO)0;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const TO o = __shfl_up(y, d, 64);
+            if (lane >= d) y += o;
+        }
+        if (lane < (int)(blockDim.x >> 6)) wsum[lane] = y;
+    }
+    __syncthreads();
+    if (w > 0) x += wsum[w - 1];
+    __syncthreads();
+    return x;
+}
+
+template <typename TO>
+__global__ void __launch_bounds__(1024) scan_pass1(const int *__restrict__ in, int n, TO *__restrict__ bsum) {
+    __shared__ TO wsum[16];
+    const long long i = (long long)blockIdx.x * 1024 + threadIdx.x;
+    const TO x = (i < n) ? (TO)in[i] : (TO)0;
+    const TO inc = block_incl_scan<TO>(x, wsum);
+    if (threadIdx.x == 1023) bsum[blockIdx.x] = inc;
+}
+
+template <typename TO>
+__global__ void __launch_bounds__(1024) scan_pass2(TO *__restrict__ bsum, int nb) {
+    __shared__ TO wsum[16];
+    __shared__ TO carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int b0 = 0; b0 < nb; b0 += 1024) {
+        const int i = b0 + threadIdx.x;
+        const TO x = (i < nb) ? bsum[i] : (TO)0;
+        const TO inc = block_incl_scan<TO>(x, wsum);
+        const TO c = carry;
+        if (i < nb) bsum[i] = c + inc - x;  // exclusive
+        __syncthreads();
+        if (threadIdx.x == 1023) carry = c + inc;
+        __syncthreads();
+    }
+}
+
+template <typename TO>
+__global__ void __launch_bounds__(1024) scan_pass3(const int *__restrict__ in, int n, const TO *__restrict__ bsum,
+                                                   TO *__restrict__ out) {
+    __shared__ TO wsum[16];
+    const long long i = (long long)blockIdx.x * 1024 + threadIdx.x;
+    const TO x = (i < n) ? (TO)in[i] : (TO)0;
+    const TO inc = block_incl_scan<TO>(x, wsum);
+    const TO base = bsum[blockIdx.x];
+    if (i < n) out[i] = base + inc - x;
+    if (i == n - 1) out[n] = base + inc;
+}
+
+// Copy each streamline's points from its padded slot into the packed array.
+__global__ void compact_kernel(const int *__restrict__ len, int nsl, const float *__restrict__ sline,
+                               size_t slot_f, const long long *__restrict__ off, float *__restrict__ pts) {
+    const int lane = threadIdx.x & 63;
+    const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nw = (gridDim.x * blockDim.x) >> 6;
+    for (int k = w; k < nsl; k += nw) {
+        const int nf = 3 * len[k];
+        const float *src = sline + (size_t)k * slot_f;
+        float *dst = pts + 3 * (size_t)off[k];
+        for (int p = lane; p < nf; p += 64) dst[p] = src[p];
+    }
+}
+
+}  // namespace
+
+// =========================================================================
+// Host side: C ABI
+// =========================================================================
+thread_local std::string g_last_error;
+
+static int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIPC(expr)                                                                                  \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess) return fail(CUSLINES_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct cuslines_tracker {
+    cuslines_params p{};
+    DevParams dp{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    float *d_data = nullptr, *d_stop = nullptr, *d_verts = nullptr;
+    uint32_t *d_edges = nullptr;
+    float *d_H = nullptr, *d_R = nullptr, *d_db = nullptr, *d_dq = nullptr, *d_smat = nullptr;
+    int *d_b0 = nullptr, *d_dwi = nullptr;
+    uint16_t *d_lut = nullptr;
+    bool has_data = false, has_stop = false, has_sphere = false, has_boot = false, has_lut = false;
+};
+
+static float smallest_float_at_least(double x) {
+    float f = (float)x;
+    if ((double)f < x) f = std::nextafter(f, INFINITY);
+    return f;
+}
+
+static int ceil32(int x) { return (x + 31) & ~31; }
+
+static DevBufs bufs_of(const cuslines_tracker *t) {
+    DevBufs b;
+    b.data = t->d_data;
+    b.stop = t->d_stop;
+    b.verts = t->d_verts;
+    b.edges = t->d_edges;
+    b.H = t->d_H; b.R = t->d_R; b.db = t->d_db; b.dq = t->d_dq; b.smat = t->d_smat;
+    b.b0 = t->d_b0;
+    b.dwi_idx = t->d_dwi;
+    b.lut = t->d_lut;
+    return b;
+}
+
+static void dfree(void *p) {
+    if (p) (void)hipFree(p);
+}
+
+static int check_ready(const cuslines_tracker *t) {
+    if (!t->has_data || !t->has_stop || !t->has_sphere)
+        return fail(CUSLINES_ESTATE, "data, stop map and sphere must be uploaded first");
+    if ((t->p.model_type == CUSLINES_OPDT || t->p.model_type == CUSLINES_CSA) && !t->has_boot)
+        return fail(CUSLINES_ESTATE, "bootstrap matrices must be uploaded first");
+    if (t->p.model_type == CUSLINES_PTT && !t->has_lut) return fail(CUSLINES_ESTATE, "PTT LUT must be uploaded first");
+    return CUSLINES_OK;
+}
+
+template <int MODEL>
+static int launch_phase1(cuslines_tracker *t, const float *seeds, int n, int chunk, int *counts, int *dirs) {
+    const DevParams &d = t->dp;
+    const int threads = 128;
+    const int nslots = threads / 32;
+    const size_t lds = sizeof(float) * ((size_t)((3 * d.nverts + 3) & ~3) +
+                                        (size_t)nslots * slot_floats(MODEL, true, d.nv32, d.t32, d.hr32));
+    if (lds > 160 * 1024) return fail(CUSLINES_EINVAL, "sphere too large for LDS");
+    HIPC(hipFuncSetAttribute((const void *)phase1_kernel<MODEL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const unsigned grid = (unsigned)((n + nslots - 1) / nslots);
+    hipLaunchKernelGGL(phase1_kernel<MODEL>, dim3(grid), dim3(threads), lds, t->stream, d, bufs_of(t), seeds, n,
+                       chunk, counts, dirs);
+    HIPC(hipGetLastError());
+    return CUSLINES_OK;
+}
+
+template <int MODEL>
+static int launch_phase2(cuslines_tracker *t, const float *seeds, int n, int chunk, const int *offsets,
+                         const int *dirs, int *sseed, int *slen, float *sl) {
+    const DevParams &d = t->dp;
+    const int threads = (MODEL == CUSLINES_OPDT || MODEL == CUSLINES_CSA) ? 128 : 256;
+    const int nslots = threads / 32;
+    const size_t vf = (MODEL == CUSLINES_PTT) ? 0 : (size_t)((3 * d.nverts + 3) & ~3);
+    const size_t lds = sizeof(float) * (vf + (size_t)nslots * slot_floats(MODEL, false, d.nv32, d.t32, d.hr32));
+    if (lds > 160 * 1024) return fail(CUSLINES_EINVAL, "sphere too large for LDS");
+    HIPC(hipFuncSetAttribute((const void *)phase2_kernel<MODEL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const unsigned grid = (unsigned)((n + nslots - 1) / nslots);
+    hipLaunchKernelGGL(phase2_kernel<MODEL>, dim3(grid), dim3(threads), lds, t->stream, d, bufs_of(t), seeds, n,
+                       chunk, offsets, dirs, sseed, slen, sl);
+    HIPC(hipGetLastError());
+    return CUSLINES_OK;
+}
+
+template <typename TO>
+static int run_scan(cuslines_tracker *t, const int *in, int n, TO *out) {
+    if (n <= 0) {
+        HIPC(hipMemsetAsync(out, 0, sizeof(TO), t->stream));
+        return CUSLINES_OK;
+    }
+    const int nb = (n + 1023) / 1024;
+    TO *bsum = nullptr;
+    HIPC(hipMallocAsync((void **)&bsum, (size_t)nb * sizeof(TO), t->stream));
+    hipLaunchKernelGGL(scan_pass1<TO>, dim3(nb), dim3(1024), 0, t->stream, in, n, bsum);
+    hipLaunchKernelGGL(scan_pass2<TO>, dim3(1), dim3(1024), 0, t->stream, bsum, nb);
+    hipLaunchKernelGGL(scan_pass3<TO>, dim3(nb), dim3(1024), 0, t->stream, in, n, bsum, out);
+    hipError_t e = hipGetLastError();
+    (void)hipFreeAsync(bsum, t->stream);
+    if (e != hipSuccess) return fail(CUSLINES_EHIP, std::string("scan: ") + hipGetErrorString(e));
+    return CUSLINES_OK;
+}
+
+extern "C" {
+
+int cuslines_abi_version(void) { return CUSLINES_ABI_VERSION; }
+
+int cuslines_last_error(char *buf, size_t len) {
+    if (!buf || len == 0) return CUSLINES_EINVAL;
+    const size_t n = std::min(len - 1, g_last_error.size());
+    std::memcpy(buf, g_last_error.data(), n);
+    buf[n] = '\0';
+    return CUSLINES_OK;
+}
+
+int cuslines_device_count(int *count) {
+    if (!count) return fail(CUSLINES_EINVAL, "count is null");
+    int n = 0;
+    HIPC(hipGetDeviceCount(&n));
+    *count = n;
+    return CUSLINES_OK;
+}
+
+int cuslines_tracker_create(const cuslines_params *p, int device, cuslines_tracker **out) {
+    if (!p || !out) return fail(CUSLINES_EINVAL, "null argument");
+    *out = nullptr;
+    if (p->dimx <= 0 || p->dimy <= 0 || p->dimz <= 0 || p->dimt <= 0)
+        return fail(CUSLINES_EINVAL, "volume dimensions must be positive");
+    if (p->model_type < CUSLINES_OPDT || p->model_type > CUSLINES_PTT)
+        return fail(CUSLINES_EINVAL, "invalid model type");
+    if (p->nverts <= 0 || p->nverts > 65535 || p->nedges <= 0)
+        return fail(CUSLINES_EINVAL, "sphere must have 1..65535 vertices and at least one edge");
+    if (p->max_sline_len < 2) return fail(CUSLINES_EINVAL, "max_sline_len must be >= 2");
+    if ((p->model_type == CUSLINES_PROB || p->model_type == CUSLINES_PTT) && p->nverts != p->dimt)
+        return fail(CUSLINES_EINVAL, "prob/PTT data must hold one ODF value per sphere vertex");
+    if (p->model_type == CUSLINES_PTT) {
+        if (p->step_frac < 1 || p->probe_quality < 1 || p->lut_res < 2)
+            return fail(CUSLINES_EINVAL, "invalid PTT parameters");
+        const long long slots = 1LL << (p->tile_log2_x + p->tile_log2_y + p->tile_log2_z);
+        if (slots < p->dimt) return fail(CUSLINES_EINVAL, "PTT tiling has fewer slots than vertices");
+    }
+    if ((p->model_type == CUSLINES_OPDT || p->model_type == CUSLINES_CSA) && p->delta_nr <= 0)
+        return fail(CUSLINES_EINVAL, "bootstrap needs delta_nr > 0");
+    int ndev = 0;
+    HIPC(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(CUSLINES_EINVAL, "device index out of range");
+    HIPC(hipSetDevice(device));
+    cuslines_tracker *t = new cuslines_tracker();
+    t->p = *p;
+    t->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete t;
+        return fail(CUSLINES_EHIP, std::string("hipStreamCreateWithFlags: ") + hipGetErrorString(e));
+    }
+    unsigned char net[SORT_LAYERS][32];
+    int nl = 0;
+    make_sortnet(net, &nl);
+    e = hipMemcpyToSymbol(HIP_SYMBOL(c_sortnet), net, sizeof(net));
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(t->stream);
+        delete t;
+        return fail(CUSLINES_EHIP, std::string("sort network upload: ") + hipGetErrorString(e));
+    }
+    DevParams &d = t->dp;
+    d.dimx = p->dimx; d.dimy = p->dimy; d.dimz = p->dimz; d.dimt = p->dimt; d.t32 = ceil32(p->dimt);
+    d.nverts = p->nverts; d.nv32 = ceil32(p->nverts); d.nedges = p->nedges;
+    d.sphere_symm = p->sphere_symm ? 1 : 0;
+    d.max_sline_len = p->max_sline_len;
+    d.model = p->model_type;
+    d.hr = 0; d.hr32 = 0; d.delta_nr = p->delta_nr;
+    d.seed_lo = (uint32_t)p->rng_seed; d.seed_hi = (uint32_t)(p->rng_seed >> 32);
+    d.tc_threshold = p->tc_threshold;
+    d.rel_peak = p->relative_peak_thresh;
+    d.cos_max_angle = (float)std::cos((double)p->max_angle);
+    d.cos_min_sep = (float)std::cos((double)p->min_separation_angle);
+    d.min_signal = p->min_signal;
+    d.step_d = (double)p->step_size;
+    d.step_frac = (p->model_type == CUSLINES_PTT) ? p->step_frac : 1;
+    d.probe_quality = p->probe_quality;
+    d.lut_res = p->lut_res;
+    d.log2x = p->tile_log2_x; d.log2y = p->tile_log2_y;
+    d.nx = 1 << p->tile_log2_x; d.ny = 1 << p->tile_log2_y; d.nz = 1 << p->tile_log2_z;
+    d.probe_step = p->probe_step_size;
+    d.max_curv = p->max_curvature;
+    d.min_support_f = smallest_float_at_least((double)p->probe_quality * PMF_THRESHOLD_P);
+    d.ptt_seed_thr_f = smallest_float_at_least(PMF_THRESHOLD_P);
+    d.ptt_sub_step = (float)(d.step_d / (double)d.step_frac);
+    *out = t;
+    return CUSLINES_OK;
+}
+
+int cuslines_tracker_destroy(cuslines_tracker *t) {
+    if (!t) return CUSLINES_OK;
+    (void)hipSetDevice(t->device);
+    if (t->stream) (void)hipStreamSynchronize(t->stream);
+    dfree(t->d_data); dfree(t->d_stop); dfree(t->d_verts); dfree(t->d_edges);
+    dfree(t->d_H); dfree(t->d_R); dfree(t->d_db); dfree(t->d_dq); dfree(t->d_smat);
+    dfree(t->d_b0); dfree(t->d_dwi); dfree(t->d_lut);
+    if (t->stream) (void)hipStreamDestroy(t->stream);
+    delete t;
+    return CUSLINES_OK;
+}
+
+int cuslines_set_data(cuslines_tracker *t, const float *dataf) {
+    if (!t || !dataf) return fail(CUSLINES_EINVAL, "null argument");
+    HIPC(hipSetDevice(t->device));
+    const DevParams &d = t->dp;
+    const size_t nvox = (size_t)d.dimx * d.dimy * d.dimz;
+    dfree(t->d_data);
+    t->d_data = nullptr;
+    t->has_data = false;
+    if (t->p.model_type == CUSLINES_PTT) {
+        float *tmp = nullptr;
+        const size_t bytes = nvox * d.dimt * sizeof(float);
+        HIPC(hipMalloc(&tmp, bytes));
+        hipError_t e = hipMalloc(&t->d_data, bytes);
+        if (e != hipSuccess) { dfree(tmp); return fail(CUSLINES_ENOMEM, "PTT data allocation failed"); }
+        e = hipMemcpy(tmp, dataf, bytes, hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            dim3 grid((unsigned)((nvox + 31) / 32), (unsigned)((d.dimt + 31) / 32));
+            hipLaunchKernelGGL(transpose_vox_t, grid, dim3(256), 0, t->stream, tmp, t->d_data, (long long)nvox, d.dimt);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+        dfree(tmp);
+        if (e != hipSuccess) return fail(CUSLINES_EHIP, std::string("PTT data upload: ") + hipGetErrorString(e));
+    } else {
+        const size_t pitch = (size_t)d.t32 * sizeof(float);
+        HIPC(hipMalloc(&t->d_data, nvox * pitch));
+        HIPC(hipMemset(t->d_data, 0, nvox * pitch));
+        HIPC(hipMemcpy2D(t->d_data, pitch, dataf, (size_t)d.dimt * sizeof(float), (size_t)d.dimt * sizeof(float),
+                         nvox, hipMemcpyHostToDevice));
+    }
+    t->has_data = true;
+    return CUSLINES_OK;
+}
+
+int cuslines_set_stop_map(cuslines_tracker *t, const float *stop_map) {
+    if (!t || !stop_map) return fail(CUSLINES_EINVAL, "null argument");
+    HIPC(hipSetDevice(t->device));
+    const size_t n = (size_t)t->dp.dimx * t->dp.dimy * t->dp.dimz;
+    dfree(t->d_stop);
+    t->d_stop = nullptr;
+    HIPC(hipMalloc(&t->d_stop, n * sizeof(float)));
+    HIPC(hipMemcpy(t->d_stop, stop_map, n * sizeof(float), hipMemcpyHostToDevice));
+    t->has_stop = true;
+    return CUSLINES_OK;
+}
+
+int cuslines_set_sphere(cuslines_tracker *t, const float *vertices, const int32_t *edges) {
+    if (!t || !vertices || !edges) return fail(CUSLINES_EINVAL, "null argument");
+    HIPC(hipSetDevice(t->device));
+    const int nv = t->dp.nverts, ne = t->dp.nedges;
+    std::vector<uint32_t> packed(ne);
+    for (int e = 0; e < ne; ++e) {
+        const int u = edges[2 * e], v = edges[2 * e + 1];
+        if (u < 0 || u >= nv || v < 0 || v >= nv) return fail(CUSLINES_EINVAL, "edge vertex index out of range");
+        packed[e] = (uint32_t)u | ((uint32_t)v << 16);
+    }
+    dfree(t->d_verts); dfree(t->d_edges);
+    t->d_verts = nullptr; t->d_edges = nullptr;
+    HIPC(hipMalloc(&t->d_verts, (size_t)nv * 3 * sizeof(float)));
+    HIPC(hipMemcpy(t->d_verts, vertices, (size_t)nv * 3 * sizeof(float), hipMemcpyHostToDevice));
+    HIPC(hipMalloc(&t->d_edges, (size_t)ne * sizeof(uint32_t)));
+    HIPC(hipMemcpy(t->d_edges, packed.data(), (size_t)ne * sizeof(uint32_t), hipMemcpyHostToDevice));
+    t->has_sphere = true;
+    return CUSLINES_OK;
+}
+
+int cuslines_set_boot_matrices(cuslines_tracker *t, const float *H, const float *R, const float *delta_b,
+                               const float *delta_q, const float *sampling_matrix, const int32_t *b0s_mask) {
+    if (!t || !H || !R || !delta_b || !delta_q || !sampling_matrix || !b0s_mask)
+        return fail(CUSLINES_EINVAL, "null argument");
+    if (t->p.model_type != CUSLINES_OPDT && t->p.model_type != CUSLINES_CSA)
+        return fail(CUSLINES_ESTATE, "bootstrap matrices need an OPDT or CSA tracker");
+    HIPC(hipSetDevice(t->device));
+    const int T = t->dp.dimt, dn = t->dp.delta_nr, nv = t->dp.nverts;
+    std::vector<int> b0(T), dwi;
+    int nb0 = 0;
+    for (int j = 0; j < T; ++j) {
+        b0[j] = b0s_mask[j] ? 1 : 0;
+        if (b0[j]) ++nb0; else dwi.push_back(j);
+    }
+    const int hr = (int)dwi.size();
+    if (hr == 0 || nb0 == 0) return fail(CUSLINES_EINVAL, "bootstrap needs at least one b0 and one DWI volume");
+    t->dp.hr = hr;
+    t->dp.hr32 = ceil32(hr);
+    float **dst[5] = {&t->d_H, &t->d_R, &t->d_db, &t->d_dq, &t->d_smat};
+    const float *src[5] = {H, R, delta_b, delta_q, sampling_matrix};
+    const size_t cnt[5] = {(size_t)hr * hr, (size_t)hr * hr, (size_t)dn * hr, (size_t)dn * hr, (size_t)nv * dn};
+    for (int k = 0; k < 5; ++k) {
+        dfree(*dst[k]);
+        *dst[k] = nullptr;
+        HIPC(hipMalloc(dst[k], cnt[k] * sizeof(float)));
+        HIPC(hipMemcpy(*dst[k], src[k], cnt[k] * sizeof(float), hipMemcpyHostToDevice));
+    }
+    dfree(t->d_b0); dfree(t->d_dwi);
+    t->d_b0 = nullptr; t->d_dwi = nullptr;
+    HIPC(hipMalloc(&t->d_b0, (size_t)T * sizeof(int)));
+    HIPC(hipMemcpy(t->d_b0, b0.data(), (size_t)T * sizeof(int), hipMemcpyHostToDevice));
+    HIPC(hipMalloc(&t->d_dwi, (size_t)hr * sizeof(int)));
+    HIPC(hipMemcpy(t->d_dwi, dwi.data(), (size_t)hr * sizeof(int), hipMemcpyHostToDevice));
+    t->has_boot = true;
+    return CUSLINES_OK;
+}
+
+int cuslines_set_ptt_lut(cuslines_tracker *t, const int32_t *lut) {
+    if (!t || !lut) return fail(CUSLINES_EINVAL, "null argument");
+    if (t->p.model_type != CUSLINES_PTT) return fail(CUSLINES_ESTATE, "LUT needs a PTT tracker");
+    HIPC(hipSetDevice(t->device));
+    const size_t r = (size_t)t->dp.lut_res;
+    std::vector<uint16_t> h(r * r * r);
+    for (size_t i = 0; i < h.size(); ++i) {
+        if (lut[i] < 0 || lut[i] >= t->dp.nverts) return fail(CUSLINES_EINVAL, "LUT vertex index out of range");
+        h[i] = (uint16_t)lut[i];
+    }
+    dfree(t->d_lut);
+    t->d_lut = nullptr;
+    HIPC(hipMalloc(&t->d_lut, h.size() * sizeof(uint16_t)));
+    HIPC(hipMemcpy(t->d_lut, h.data(), h.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    t->has_lut = true;
+    return CUSLINES_OK;
+}
+
+int cuslines_device_alloc(cuslines_tracker *t, size_t bytes, void **dptr) {
+    if (!t || !dptr) return fail(CUSLINES_EINVAL, "null argument");
+    HIPC(hipSetDevice(t->device));
+    *dptr = nullptr;
+    if (bytes == 0) bytes = 4;
+    hipError_t e = hipMalloc(dptr, bytes);
+    if (e != hipSuccess) return fail(CUSLINES_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return CUSLINES_OK;
+}
+
+int cuslines_device_free(cuslines_tracker *t, void *dptr) {
+    if (!t) return fail(CUSLINES_EINVAL, "null tracker");
+    HIPC(hipSetDevice(t->device));
+    if (dptr) HIPC(hipFree(dptr));
+    return CUSLINES_OK;
+}
+
+int cuslines_memcpy_htod(cuslines_tracker *t, void *dst, const void *src, size_t bytes) {
+    if (!t || (bytes && (!dst || !src))) return fail(CUSLINES_EINVAL, "null argument");
+    HIPC(hipSetDevice(t->device));
+    if (bytes) HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, t->stream));
+    HIPC(hipStreamSynchronize(t->stream));
+    return CUSLINES_OK;
+}
+
+int cuslines_memcpy_dtoh(cuslines_tracker *t, void *dst, const void *src, size_t bytes) {
+    if (!t || (bytes && (!dst || !src))) return fail(CUSLINES_EINVAL, "null argument");
+    HIPC(hipSetDevice(t->device));
+    if (bytes) HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, t->stream));
+    HIPC(hipStreamSynchronize(t->stream));
+    return CUSLINES_OK;
+}
+
+int cuslines_synchronize(cuslines_tracker *t) {
+    if (!t) return fail(CUSLINES_EINVAL, "null tracker");
+    HIPC(hipSetDevice(t->device));
+    HIPC(hipStreamSynchronize(t->stream));
+    return CUSLINES_OK;
+}
+
+int cuslines_get_stream(cuslines_tracker *t, void **stream) {
+    if (!t || !stream) return fail(CUSLINES_EINVAL, "null argument");
+    *stream = (void *)t->stream;
+    return CUSLINES_OK;
+}
+
+int cuslines_phase1_async(cuslines_tracker *t, const float *seeds_d, int32_t nseeds, int32_t rng_chunk,
+                          int32_t *counts_d, int32_t *dirs_d) {
+    if (!t) return fail(CUSLINES_EINVAL, "null tracker");
+    if (nseeds < 0 || rng_chunk < 0) return fail(CUSLINES_EINVAL, "negative size");
+    if (nseeds == 0) return CUSLINES_OK;
+    if (!seeds_d || !counts_d || !dirs_d) return fail(CUSLINES_EINVAL, "null buffer");
+    int rc = check_ready(t);
+    if (rc) return rc;
+    HIPC(hipSetDevice(t->device));
+    const int chunk = rng_chunk > 0 ? rng_chunk : nseeds;
+    switch (t->p.model_type) {
+        case CUSLINES_OPDT: return launch_phase1<CUSLINES_OPDT>(t, seeds_d, nseeds, chunk, counts_d, dirs_d);
+        case CUSLINES_CSA: return launch_phase1<CUSLINES_CSA>(t, seeds_d, nseeds, chunk, counts_d, dirs_d);
+        case CUSLINES_PROB: return launch_phase1<CUSLINES_PROB>(t, seeds_d, nseeds, chunk, counts_d, dirs_d);
+        default: return launch_phase1<CUSLINES_PTT>(t, seeds_d, nseeds, chunk, counts_d, dirs_d);
+    }
+}
+
+int cuslines_scan_async(cuslines_tracker *t, const int32_t *counts_d, int32_t nseeds, int32_t *offsets_d) {
+    if (!t || !offsets_d || (nseeds > 0 && !counts_d)) return fail(CUSLINES_EINVAL, "null argument");
+    if (nseeds < 0) return fail(CUSLINES_EINVAL, "negative size");
+    HIPC(hipSetDevice(t->device));
+    return run_scan<int>(t, counts_d, nseeds, offsets_d);
+}
+
+int cuslines_phase2_async(cuslines_tracker *t, const float *seeds_d, int32_t nseeds, int32_t rng_chunk,
+                          const int32_t *offsets_d, const int32_t *dirs_d, int32_t *sline_seed_d,
+                          int32_t *sline_len_d, float *sline_d) {
+    if (!t) return fail(CUSLINES_EINVAL, "null tracker");
+    if (nseeds < 0 || rng_chunk < 0) return fail(CUSLINES_EINVAL, "negative size");
+    if (nseeds == 0) return CUSLINES_OK;
+    if (!seeds_d || !offsets_d || !dirs_d || !sline_seed_d || !sline_len_d || !sline_d)
+        return fail(CUSLINES_EINVAL, "null buffer");
+    int rc = check_ready(t);
+    if (rc) return rc;
+    HIPC(hipSetDevice(t->device));
+    const int chunk = rng_chunk > 0 ? rng_chunk : nseeds;
+    switch (t->p.model_type) {
+        case CUSLINES_OPDT:
+            return launch_phase2<CUSLINES_OPDT>(t, seeds_d, nseeds, chunk, offsets_d, dirs_d, sline_seed_d, sline_len_d, sline_d);
+        case CUSLINES_CSA:
+            return launch_phase2<CUSLINES_CSA>(t, seeds_d, nseeds, chunk, offsets_d, dirs_d, sline_seed_d, sline_len_d, sline_d);
+        case CUSLINES_PROB:
+            return launch_phase2<CUSLINES_PROB>(t, seeds_d, nseeds, chunk, offsets_d, dirs_d, sline_seed_d, sline_len_d, sline_d);
+        default:
+            return launch_phase2<CUSLINES_PTT>(t, seeds_d, nseeds, chunk, offsets_d, dirs_d, sline_seed_d, sline_len_d, sline_d);
+    }
+}
+
+int cuslines_compact_async(cuslines_tracker *t, const int32_t *sline_len_d, int32_t nslines, const float *sline_d,
+                           int64_t *point_off_d, float *points_d) {
+    if (!t || !point_off_d || (nslines > 0 && (!sline_len_d || !sline_d || !points_d)))
+        return fail(CUSLINES_EINVAL, "null argument");
+    if (nslines < 0) return fail(CUSLINES_EINVAL, "negative size");
+    HIPC(hipSetDevice(t->device));
+    int rc = run_scan<long long>(t, sline_len_d, nslines, reinterpret_cast<long long *>(point_off_d));
+    if (rc) return rc;
+    if (nslines == 0) return CUSLINES_OK;
+    const size_t slot_f = (size_t)2 * t->dp.max_sline_len * 3;
+    const unsigned grid = (unsigned)std::min<long long>(((long long)nslines + 3) / 4, 65536);
+    hipLaunchKernelGGL(compact_kernel, dim3(grid), dim3(256), 0, t->stream, sline_len_d, nslines, sline_d, slot_f,
+                       reinterpret_cast<const long long *>(point_off_d), points_d);
+    HIPC(hipGetLastError());
+    return CUSLINES_OK;
+}
+
+}  // extern "C"

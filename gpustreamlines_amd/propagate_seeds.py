@@ -1,0 +1,171 @@
+"""Seed-chunk propagation (cuslines/cuda_python/cu_propagate_seeds.py).
+
+Per chunk and per GPU share: upload seeds, phase 1, exclusive scan of the
+per-seed direction counts, phase 2, then compaction of the padded point
+slots.  Differences from the reference, both on the device side of the
+boundary: the scan runs on the GPU (the reference copies counts to the host
+and scans in a Python loop, :95-129) and only valid points are copied back
+(the reference copies every 1,002-point slot, :171-191).  ``slines[n]`` is
+therefore the packed float32 [n_points, 3] array of GPU ``n`` and
+``sline_offs[n]`` its int64 [nSlines+1] point offsets.
+"""
+
+from __future__ import annotations
+
+import logging
+import math
+
+import numpy as np
+
+from ._globals import MAX_SLINE_LEN, REAL_SIZE, THR_X_BL, THR_X_SL
+from .array_sequence import MEGABYTE, make_array_sequence
+
+logger = logging.getLogger("GPUStreamlines")
+
+
+def div_up(a, b):
+    return (a + b - 1) // b
+
+
+class SeedBatchPropagator:
+    def __init__(self, gpu_tracker, minlen=0, maxlen=np.inf):
+        self.gpu_tracker = gpu_tracker
+        self.ngpus = gpu_tracker.ngpus
+        self.minlen = minlen
+        self.maxlen = maxlen
+        self.nSlines = np.zeros(self.ngpus, dtype=np.int32)
+        self.slines = [None] * self.ngpus
+        self.sline_lens = [None] * self.ngpus
+        self.sline_offs = [None] * self.ngpus
+        self.sline_seeds = [None] * self.ngpus
+        self._reset_buffers()
+
+    def _reset_buffers(self):
+        n = self.ngpus
+        self.seeds_d = [None] * n
+        self.slinesOffs_d = [None] * n   # phase-1 counts
+        self.slinesScan_d = [None] * n   # exclusive offsets
+        self.shDirTemp0_d = [None] * n
+        self.slineSeed_d = [None] * n
+        self.slineLen_d = [None] * n
+        self.sline_d = [None] * n
+
+    def _switch_device(self, n):
+        nseeds_gpu = min(self.nseeds_per_gpu, max(0, self.nseeds - n * self.nseeds_per_gpu))
+        block = (THR_X_SL, THR_X_BL // THR_X_SL, 1)
+        grid = (div_up(nseeds_gpu, THR_X_BL // THR_X_SL), 1, 1)
+        return nseeds_gpu, block, grid
+
+    def _free(self):
+        for lst in (self.seeds_d, self.slinesOffs_d, self.slinesScan_d, self.shDirTemp0_d,
+                    self.slineSeed_d, self.slineLen_d, self.sline_d):
+            for b in lst:
+                if b is not None:
+                    b.free()
+        self._reset_buffers()
+
+    def propagate(self, seeds):
+        seeds = np.ascontiguousarray(seeds, dtype=np.float32).reshape(-1, 3)
+        self.nseeds = len(seeds)
+        self.nseeds_per_gpu = (self.nseeds + self.gpu_tracker.ngpus - 1) // self.gpu_tracker.ngpus
+        tr = self.gpu_tracker
+        dg = tr.dg
+        try:
+            for ii in range(self.ngpus):
+                nseeds_gpu, block, grid = self._switch_device(ii)
+                dt = tr.device_trackers[ii]
+                part = seeds[ii * self.nseeds_per_gpu: ii * self.nseeds_per_gpu + nseeds_gpu]
+                self.seeds_d[ii] = dt.alloc(part.nbytes)
+                self.slinesOffs_d[ii] = dt.alloc(4 * (nseeds_gpu + 1))
+                self.slinesScan_d[ii] = dt.alloc(4 * (nseeds_gpu + 1))
+                self.shDirTemp0_d[ii] = dt.alloc(4 * max(1, nseeds_gpu) * tr.samplm_nr)
+                if nseeds_gpu:
+                    self.seeds_d[ii].upload(part)
+            for ii in range(self.ngpus):
+                nseeds_gpu, block, grid = self._switch_device(ii)
+                if nseeds_gpu == 0:
+                    continue
+                dg.getNumStreamlines(ii, nseeds_gpu, block, grid, self)
+            for ii in range(self.ngpus):
+                nseeds_gpu, _, _ = self._switch_device(ii)
+                dt = tr.device_trackers[ii]
+                if nseeds_gpu == 0:
+                    self.nSlines[ii] = 0
+                    continue
+                dt.scan_async(self.slinesOffs_d[ii].ptr, nseeds_gpu, self.slinesScan_d[ii].ptr)
+                tot = np.zeros(nseeds_gpu + 1, dtype=np.int32)
+                self.slinesScan_d[ii].download(tot)
+                self.nSlines[ii] = int(tot[nseeds_gpu])
+            for ii in range(self.ngpus):
+                nseeds_gpu, block, grid = self._switch_device(ii)
+                dt = tr.device_trackers[ii]
+                nsl = int(self.nSlines[ii])
+                self.slineSeed_d[ii] = dt.alloc(4 * nsl)
+                self.slineLen_d[ii] = dt.alloc(4 * nsl)
+                self.sline_d[ii] = dt.alloc(REAL_SIZE * 2 * 3 * MAX_SLINE_LEN * nsl)
+                if nseeds_gpu and nsl:
+                    dg.generateStreamlines(ii, nseeds_gpu, block, grid, self)
+            self._cleanup()
+        finally:
+            self._free()
+
+    def _cleanup(self):
+        tr = self.gpu_tracker
+        for ii in range(self.ngpus):
+            dt = tr.device_trackers[ii]
+            nsl = int(self.nSlines[ii])
+            lens = np.zeros(nsl, dtype=np.int32)
+            seeds_of = np.zeros(nsl, dtype=np.int32)
+            if nsl:
+                self.slineLen_d[ii].download(lens)
+                self.slineSeed_d[ii].download(seeds_of)
+            npts = int(lens.astype(np.int64).sum())
+            offs = np.zeros(nsl + 1, dtype=np.int64)
+            pts = np.zeros((npts, 3), dtype=np.float32)
+            if nsl:
+                pod = dt.alloc(8 * (nsl + 1))
+                ptd = dt.alloc(12 * max(npts, 1))
+                try:
+                    dt.compact_async(self.slineLen_d[ii].ptr, nsl, self.sline_d[ii].ptr, pod.ptr, ptd.ptr)
+                    pod.download(offs)
+                    if npts:
+                        ptd.download(pts)
+                finally:
+                    pod.free()
+                    ptd.free()
+            self.slines[ii] = pts
+            self.sline_lens[ii] = lens
+            self.sline_offs[ii] = offs
+            self.sline_seeds[ii] = seeds_of
+        self.gpu_tracker.rng_offset += self.nseeds
+
+    def _keep(self, npts):
+        return not (npts < self.minlen or npts > self.maxlen)
+
+    def get_buffer_size(self):
+        """Megabytes needed for the kept streamlines (:235-243)."""
+        size = 0
+        for ii in range(self.ngpus):
+            lens = self.sline_lens[ii]
+            if lens is None:
+                continue
+            keep = (lens >= self.minlen) & (lens <= self.maxlen)
+            size += int(lens[keep].astype(np.int64).sum()) * 3 * REAL_SIZE
+        return math.ceil(size / MEGABYTE)
+
+    def as_generator(self):
+        def _yield_slines():
+            for ii in range(self.ngpus):
+                pts, lens, offs = self.slines[ii], self.sline_lens[ii], self.sline_offs[ii]
+                if lens is None:
+                    continue
+                for jj in range(int(self.nSlines[ii])):
+                    npts = int(lens[jj])
+                    if not self._keep(npts):
+                        continue
+                    yield pts[offs[jj]: offs[jj] + npts]
+
+        return _yield_slines()
+
+    def as_array_sequence(self):
+        return make_array_sequence(self.as_generator(), self.get_buffer_size())

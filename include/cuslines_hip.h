@@ -1,0 +1,185 @@
+/*
+ * cuslines_hip.h -- C ABI of libcuslines_hip.so, the gfx950 tracking library.
+ *
+ * The reference (dipy/GPUStreamlines, package ``cuslines``) reaches its
+ * tracking kernels from Python through cuda-python: one NVRTC-compiled module
+ * per direction getter, two kernel launches per seed chunk, device buffers
+ * owned by ``GPUTracker``.  This header is the plain C replacement of that
+ * boundary.  Every entry point takes plain pointers and sizes, returns an int
+ * status (0 = success) and leaves a message for cuslines_last_error().
+ *
+ * Threading: one host thread per tracker handle; each handle owns one HIP
+ * stream (default priority); calls are synchronous unless documented as
+ * "_async" (those enqueue on the handle's stream and return).
+ */
+#ifndef CUSLINES_HIP_H
+#define CUSLINES_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CUSLINES_ABI_VERSION 1
+
+/* Direction getter / model kinds; values of the reference's ModelType enum
+ * (cuslines/cuda_c/globals.h:82-87, cuslines/cuda_python/cutils.py:14-19). */
+enum {
+    CUSLINES_OPDT = 0,
+    CUSLINES_CSA = 1,
+    CUSLINES_PROB = 2,
+    CUSLINES_PTT = 3
+};
+
+/* Tissue classes returned by the stop check (cuslines/cuda_c/globals.h:89). */
+enum {
+    CUSLINES_OUTSIDEIMAGE = 0,
+    CUSLINES_INVALIDPOINT = 1,
+    CUSLINES_TRACKPOINT = 2,
+    CUSLINES_ENDPOINT = 3
+};
+
+/* Status codes. */
+enum {
+    CUSLINES_OK = 0,
+    CUSLINES_EINVAL = 1,   /* bad argument (shape, range, null pointer)     */
+    CUSLINES_EHIP = 2,     /* HIP runtime error                              */
+    CUSLINES_ESTATE = 3,   /* call out of order (e.g. phase 2 before data)   */
+    CUSLINES_ENOMEM = 4    /* device allocation failed                       */
+};
+
+/*
+ * Tracking parameters.  Replaces the NVRTC macro set built in
+ * cuslines/cuda_python/cu_direction_getters.py:70-110 (common macros) and
+ * :456-468 (PTT macros), plus GPUTracker's constructor arguments
+ * (cuslines/cuda_python/cu_tractography.py:37-56).  Float fields hold the
+ * float32 values the reference rounds its arguments to.
+ */
+typedef struct cuslines_params {
+    int32_t dimx, dimy, dimz, dimt;  /* data volume [X][Y][Z][T] (C order)      */
+    int32_t model_type;              /* CUSLINES_OPDT/CSA/PROB/PTT              */
+    int32_t nverts;                  /* sphere vertices (SAMPLM_NR)             */
+    int32_t nedges;                  /* sphere edges (NUM_EDGES)                */
+    int32_t sphere_symm;             /* SPHERE_SYMM                              */
+    int32_t max_sline_len;           /* MAX_SLINE_LEN (501)                      */
+    int32_t delta_nr;                /* bootstrap: SH coefficients               */
+    uint64_t rng_seed;               /* RNG_SEED                                 */
+    float step_size;                 /* STEP_SIZE, voxels                        */
+    float max_angle;                 /* MAX_ANGLE, radians                       */
+    float tc_threshold;              /* TC_THRESHOLD (stop-map threshold)        */
+    float relative_peak_thresh;      /* RELATIVE_PEAK_THRESH                     */
+    float min_separation_angle;      /* MIN_SEPARATION_ANGLE, radians            */
+    float min_signal;                /* bootstrap min_signal                     */
+    /* PTT only */
+    int32_t step_frac;               /* STEP_FRAC                                */
+    int32_t probe_quality;           /* PROBE_QUALITY (probe steps)              */
+    int32_t lut_res;                 /* nearest-vertex LUT resolution (128)      */
+    int32_t tile_log2_x;             /* LOG2_X of the (nx,ny,nz) vertex tiling   */
+    int32_t tile_log2_y;             /* LOG2_Y                                   */
+    int32_t tile_log2_z;             /* log2 nz                                  */
+    float probe_step_size;           /* PROBE_STEP_SIZE                          */
+    float max_curvature;             /* MAX_CURVATURE                            */
+} cuslines_params;
+
+typedef struct cuslines_tracker cuslines_tracker;
+
+/* ABI version of the loaded library (CUSLINES_ABI_VERSION). */
+int cuslines_abi_version(void);
+
+/* Copy the calling thread's last error message into buf (NUL-terminated).
+ * Mirrors the message half of checkCudaErrors, cuslines/cuda_python/cutils.py:56-75. */
+int cuslines_last_error(char *buf, size_t len);
+
+/* Number of visible HIP devices (cuslines/cuda_python/cu_tractography.py:161-165). */
+int cuslines_device_count(int *count);
+
+/* Create a tracker bound to one device.  Replaces GPUTracker.__init__'s
+ * context/stream setup and compile_program (cu_tractography.py:110-183,
+ * cu_direction_getters.py:53-144): the kernels are precompiled for gfx950 and
+ * specialised by the runtime parameters above. */
+int cuslines_tracker_create(const cuslines_params *params, int device, cuslines_tracker **out);
+
+/* Free every device buffer and the stream (cu_tractography.py:260-300). */
+int cuslines_tracker_destroy(cuslines_tracker *t);
+
+/* Upload the data volume: DWI for bootstrap, ODF amplitudes for prob, the
+ * prepared ODF (clip >= 0, zero outside the stop mask, max-normalised) for
+ * PTT.  Host float32 [dimx][dimy][dimz][dimt] (cu_tractography.py:204-221). */
+int cuslines_set_data(cuslines_tracker *t, const float *dataf);
+
+/* Upload the stop map, host float32 [dimx][dimy][dimz]
+ * (cu_tractography.py:237-241: linear-filtered texture, border -1). */
+int cuslines_set_stop_map(cuslines_tracker *t, const float *stop_map);
+
+/* Upload sphere vertices float32 [nverts][3] and edges int32 [nedges][2]
+ * (cu_tractography.py:223-255). */
+int cuslines_set_sphere(cuslines_tracker *t, const float *vertices, const int32_t *edges);
+
+/* Bootstrap matrices (BootDirectionGetter.allocate_on_gpu,
+ * cu_direction_getters.py:217-280): H, R float32 [hr][hr] with hr the number
+ * of non-b0 volumes; delta_b, delta_q float32 [delta_nr][hr];
+ * sampling_matrix float32 [nverts][delta_nr]; b0s_mask int32 [dimt]. */
+int cuslines_set_boot_matrices(cuslines_tracker *t, const float *H, const float *R,
+                               const float *delta_b, const float *delta_q,
+                               const float *sampling_matrix, const int32_t *b0s_mask);
+
+/* PTT nearest-vertex lookup table, int32 [lut_res]^3 indexed [ix][iy][iz]
+ * over linspace(-1, 1, lut_res) per axis (PttDirectionGetter.prepare_data,
+ * cu_direction_getters.py:601-623, uploaded as a point-sampled texture). */
+int cuslines_set_ptt_lut(cuslines_tracker *t, const int32_t *lut);
+
+/* Device memory helpers for callers that keep seeds and outputs resident. */
+int cuslines_device_alloc(cuslines_tracker *t, size_t bytes, void **dptr);
+int cuslines_device_free(cuslines_tracker *t, void *dptr);
+int cuslines_memcpy_htod(cuslines_tracker *t, void *dst, const void *src, size_t bytes);
+int cuslines_memcpy_dtoh(cuslines_tracker *t, void *dst, const void *src, size_t bytes);
+int cuslines_synchronize(cuslines_tracker *t);
+/* The handle's HIP stream (hipStream_t), for event timing by the caller. */
+int cuslines_get_stream(cuslines_tracker *t, void **stream);
+
+/*
+ * Phase 1 on device buffers (getNumStreamlines{Boot,Prob,Ptt}_k,
+ * cuslines/cuda_c/boot.cu:610-718, generate_streamlines_cuda.cu:423-462,
+ * ptt_init.cu:1-63): for every seed, the initial peak directions.
+ *   seeds_d    float32 [nseeds][3] voxel coordinates
+ *   counts_d   int32 [nseeds + 1]; counts_d[i] = number of directions of seed i
+ *   dirs_d     int32 [nseeds][nverts]; sphere-vertex index of each direction
+ * rng_chunk: seeds are numbered i % rng_chunk for their random streams
+ * (the reference's streams are chunk-relative); 0 means nseeds.
+ */
+int cuslines_phase1_async(cuslines_tracker *t, const float *seeds_d, int32_t nseeds,
+                          int32_t rng_chunk, int32_t *counts_d, int32_t *dirs_d);
+
+/* Exclusive scan of counts_d[0..nseeds) into offsets_d[0..nseeds] on device,
+ * replacing the host loop of cuslines/cuda_python/cu_propagate_seeds.py:95-129.
+ * The total is written to offsets_d[nseeds]. */
+int cuslines_scan_async(cuslines_tracker *t, const int32_t *counts_d, int32_t nseeds,
+                        int32_t *offsets_d);
+
+/*
+ * Phase 2 on device buffers (genStreamlinesMerge{Boot,Prob}_k,
+ * cuslines/cuda_c/boot.cu:826-980, generate_streamlines_cuda.cu:464-622):
+ * for every seed direction, track backward, reverse, track forward.
+ *   offsets_d  int32 [nseeds + 1] exclusive offsets (from cuslines_scan_async)
+ *   dirs_d     as written by phase 1
+ *   sline_seed_d int32 [nslines]   seed index of each streamline
+ *   sline_len_d  int32 [nslines]   points per streamline
+ *   sline_d      float32 [nslines][2*max_sline_len][3] padded point slots
+ */
+int cuslines_phase2_async(cuslines_tracker *t, const float *seeds_d, int32_t nseeds,
+                          int32_t rng_chunk, const int32_t *offsets_d,
+                          const int32_t *dirs_d, int32_t *sline_seed_d,
+                          int32_t *sline_len_d, float *sline_d);
+
+/* Compact padded slots into one point array: points_d[sum(len)][3], with
+ * point_off_d int64 [nslines + 1] the exclusive scan of sline_len_d. */
+int cuslines_compact_async(cuslines_tracker *t, const int32_t *sline_len_d, int32_t nslines,
+                           const float *sline_d, int64_t *point_off_d, float *points_d);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUSLINES_HIP_H */

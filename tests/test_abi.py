@@ -1,0 +1,69 @@
+"""The C ABI library loads on a host without a GPU and exports every entry
+point include/cuslines_hip.h declares (no compute calls here)."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "cuslines_hip.h")
+
+
+def declared():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^int (cuslines_\w+)\(", txt, re.M)))
+
+
+def test_header_declares_entry_points():
+    names = declared()
+    for must in ("cuslines_tracker_create", "cuslines_phase1_async", "cuslines_phase2_async",
+                 "cuslines_scan_async", "cuslines_compact_async", "cuslines_last_error"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    from gpustreamlines_amd import _lib
+
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.fail("libcuslines_hip.so is not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [n for n in declared() if not hasattr(L, n)]
+    assert not missing, missing
+    L.cuslines_abi_version.restype = ctypes.c_int
+    assert L.cuslines_abi_version() == _lib.ABI_VERSION
+
+
+def test_params_struct_layout_matches_header():
+    """Field order/size of cuslines_params as ctypes sees it == the C struct
+    (checked through a compiled probe of offsetof)."""
+    import subprocess
+    import tempfile
+
+    from gpustreamlines_amd._lib import cuslines_params
+
+    fields = [f for f, _ in cuslines_params._fields_]
+    src = '#include <stdio.h>\n#include <stddef.h>\n#include "%s"\nint main(){\n' % HEADER
+    src += 'printf("%zu\\n", sizeof(cuslines_params));\n'
+    for f in fields:
+        src += 'printf("%%zu\\n", offsetof(cuslines_params, %s));\n' % f
+    src += "return 0;}\n"
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "p.c")
+        exe = os.path.join(d, "p")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-o", exe, c], check=True)
+        out = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    assert out[0] == ctypes.sizeof(cuslines_params)
+    assert out[1:] == [getattr(cuslines_params, f).offset for f in fields]
+
+
+def test_errors_are_reported_without_gpu():
+    from gpustreamlines_amd import _lib
+
+    L = _lib.load()
+    h = ctypes.c_void_p()
+    bad = _lib.params_struct(dict(dimx=0, dimy=1, dimz=1, dimt=1, model_type=2, nverts=1, nedges=1))
+    assert L.cuslines_tracker_create(ctypes.byref(bad), 0, ctypes.byref(h)) == 1  # CUSLINES_EINVAL
+    assert "positive" in _lib.last_error()

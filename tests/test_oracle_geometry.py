@@ -1,0 +1,173 @@
+"""Interpolation, stop check, peak extraction and PTT texture emulation of
+the oracle, against closed forms and brute-force restatements."""
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+
+
+def toy_problem(shape=(6, 7, 8), T=4, data=None, stop=None, model=2, nverts=None, extra=None,
+                verts=None, edges=None, lut=None):
+    rng = np.random.default_rng(0)
+    if data is None:
+        data = rng.random(shape + (T,)).astype(np.float32)
+    if stop is None:
+        stop = np.ones(shape, dtype=np.float32)
+    nv = nverts or T
+    if verts is None:
+        verts = np.tile(np.array([[1, 0, 0]], dtype=np.float32), (nv, 1))
+        edges = np.array([[0, min(1, nv - 1)]], dtype=np.int32)
+    p = dict(dimx=shape[0], dimy=shape[1], dimz=shape[2], dimt=data.shape[3], model_type=model,
+             nverts=nv, nedges=len(edges), max_sline_len=501, step_size=0.5,
+             max_angle=np.radians(60), tc_threshold=0.1, relative_peak_thresh=0.25,
+             min_separation_angle=np.radians(45))
+    p.update(extra or {})
+    return Oracle(p, data, stop, verts, edges, ptt_lut=lut), data
+
+
+def test_trilinear_exact_at_voxels():
+    o, d = toy_problem()
+    for idx in [(0, 0, 0), (5, 6, 7), (2, 3, 4)]:
+        rv, out = o.trilinear(np.array(idx, dtype=np.float32))
+        assert rv == 0 and np.array_equal(out, d[idx])
+
+
+def test_trilinear_interior_matches_float64():
+    o, d = toy_problem()
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        p = rng.uniform([0, 0, 0], [4.99, 5.99, 6.99]).astype(np.float32)
+        rv, out = o.trilinear(p)
+        i = np.floor(p).astype(int)
+        f = p.astype(np.float64) - i
+        want = np.zeros(d.shape[3])
+        for a in (0, 1):
+            for b in (0, 1):
+                for c in (0, 1):
+                    w = (f[0] if a else 1 - f[0]) * (f[1] if b else 1 - f[1]) * (f[2] if c else 1 - f[2])
+                    want += w * d[i[0] + a, i[1] + b, i[2] + c]
+        assert rv == 0
+        np.testing.assert_allclose(out, want, rtol=2e-6, atol=1e-7)
+
+
+def test_trilinear_bounds_and_low_edge_quirk():
+    o, d = toy_problem()
+    assert o.trilinear(np.array([-0.51, 1, 1], np.float32))[0] == -1
+    assert o.trilinear(np.array([5.5, 1, 1], np.float32))[0] == -1  # p + 0.5 >= DIMX
+    rv, out = o.trilinear(np.array([-0.25, 2, 3], np.float32))
+    # floor = -1 is clamped to voxel 0 with weight 0.75 on voxel 1 (tracking_helpers.cu:24-41)
+    assert rv == 0
+    np.testing.assert_allclose(out, 0.25 * d[0, 2, 3] + 0.75 * d[1, 2, 3], rtol=1e-6)
+
+
+def test_stop_check_texture_semantics():
+    shape = (8, 6, 5)
+    ramp = np.broadcast_to((np.arange(8, dtype=np.float32) / 10.0)[:, None, None], shape).copy()
+    o, _ = toy_problem(shape=shape, stop=ramp)
+    # linear filtering samples at p - 0.5: FA(2.0) is the mean of voxels 1 and 2 = 0.15 > 0.1
+    assert o.check_point(np.array([2.0, 2.0, 2.0], np.float32)) == 2
+    # FA(1.5) is voxel 1 = 0.1, not > 0.1 -> ENDPOINT
+    assert o.check_point(np.array([1.5, 2.5, 2.5], np.float32)) == 3
+    # fully outside: only border texels (-1) -> OUTSIDEIMAGE
+    assert o.check_point(np.array([-1.2, 2.0, 2.0], np.float32)) == 0
+    assert o.check_point(np.array([9.0, 2.0, 2.0], np.float32)) == 0
+    # half outside: a blend with the border colour, below threshold -> ENDPOINT
+    assert o.check_point(np.array([2.0, -0.3, 2.0], np.float32)) == 3
+
+
+def brute_peaks(odf, verts, edges, rel=0.25, cos_sep=np.cos(np.radians(45)), symm=False):
+    odf = odf.astype(np.float32)
+    nv = len(odf)
+    larger = np.zeros(nv, bool)
+    smaller = np.zeros(nv, bool)
+    for u, v in edges:
+        if odf[u] < odf[v]:
+            larger[u] = True; smaller[v] = True
+        elif odf[v] < odf[u]:
+            larger[v] = True; smaller[u] = True
+    mn = max(np.float32(0), odf.min())
+    marked = (~larger) & smaller
+    if not marked.any():
+        return []
+    thr = np.float32(rel * float(np.max(odf[marked] - mn)))
+    cand = [j for j in range(nv) if marked[j] and odf[j] - mn >= thr]
+    if len(cand) < 32:
+        cand = sorted(cand, key=lambda j: -odf[j])
+    kept = []
+    for c in cand:
+        ok = True
+        for k in kept:
+            d = float(np.dot(verts[c].astype(np.float64), verts[k]))
+            if symm:
+                d = abs(d)
+            if d > cos_sep:
+                ok = False
+                break
+        if ok:
+            kept.append(c)
+    return kept
+
+
+@pytest.mark.parametrize("symm", [0, 1])
+def test_peaks_match_brute_force(symm):
+    from gpustreamlines_amd.sphere import symmetric_sphere
+
+    verts, edges = symmetric_sphere(724)
+    rng = np.random.default_rng(5)
+    o, _ = toy_problem(shape=(2, 2, 2), T=724, model=2, nverts=724, verts=verts, edges=edges,
+                       extra=dict(sphere_symm=symm))
+    for trial in range(40):
+        u = rng.normal(size=(3, 3))
+        u /= np.linalg.norm(u, axis=1, keepdims=True)
+        w = rng.uniform(0.3, 1.0, 3)
+        odf = sum(wk * np.abs(verts @ uk) ** rng.integers(4, 30) for wk, uk in zip(w, u))
+        odf = (odf + 1e-4 * rng.random(724)).astype(np.float32)
+        got = list(o.peaks(odf))
+        assert got == brute_peaks(odf, verts, edges, symm=bool(symm)), trial
+
+
+def test_single_fibre_gives_antipodal_pair():
+    from gpustreamlines_amd.sphere import symmetric_sphere
+
+    verts, edges = symmetric_sphere(724)
+    o, _ = toy_problem(shape=(2, 2, 2), T=724, model=2, nverts=724, verts=verts, edges=edges)
+    u = np.array([0.3, -0.5, 0.81]); u /= np.linalg.norm(u)
+    odf = (np.abs(verts @ u) ** 20).astype(np.float32) + 1e-5 * np.arange(724, dtype=np.float32) / 724
+    pk = o.peaks(odf)
+    assert len(pk) == 2
+    assert np.all(np.abs(verts[pk] @ u) > 0.99)
+    assert np.dot(verts[pk[0]], verts[pk[1]]) < -0.99
+
+
+def test_ptt_tiled_texture_bleed():
+    """A constant ODF per vertex, g(v) = v: interior taps return g of the LUT
+    vertex; past the last texel centre along x the clamped/tiled texture
+    blends in the next tile's vertex (nx=4 tiles of 8 vertices here)."""
+    from gpustreamlines_amd.direction_getters import calculate_tight_fit, nearest_vertex_lut
+    from gpustreamlines_amd.sphere import symmetric_sphere
+
+    verts, edges = symmetric_sphere(8)
+    T = 8
+    nx, ny, nz = calculate_tight_fit(T)
+    shape = (5, 4, 3)
+    data = np.broadcast_to(np.arange(T, dtype=np.float32), shape + (T,)).copy()
+    lut = nearest_vertex_lut(verts, 16)
+    ex = dict(step_frac=2, probe_quality=4, lut_res=16, tile_log2_x=int(np.log2(nx)),
+              tile_log2_y=int(np.log2(ny)), tile_log2_z=int(np.log2(nz)), probe_step_size=0.25,
+              max_curvature=4.0)
+    o, _ = toy_problem(shape=shape, data=data, model=3, nverts=T, verts=verts, edges=edges, lut=lut, extra=ex)
+    for v in range(T):
+        frame = np.concatenate([verts[v], [0, 1, 0, 0, 0, 1]]).astype(np.float32)
+        res = 16
+        ix = [min(max(int(np.floor((f * 0.5 + 0.5) * res)), 0), res - 1) for f in frame[:3]]
+        vv = lut[ix[0], ix[1], ix[2]]
+        assert o.interp4(np.array([2.0, 1.7, 1.2], np.float32), frame) == pytest.approx(float(vv), abs=1e-5)
+        tx = vv & (nx - 1)
+        # x = DX - 0.2: texel DX-1 of this tile (weight 0.7) and texel 0 of tile tx+1
+        nxt = vv + 1 if tx + 1 < nx and vv + 1 < T else vv
+        if tx + 1 >= nx:
+            nxt = vv  # clamp at the texture edge
+        want = (1 - 0.30078125) * vv + 0.30078125 * nxt  # frac 0.3 held with 8 bits = 77/256
+        got = o.interp4(np.array([shape[0] - 0.2, 1.7, 1.2], np.float32), frame)
+        assert got == pytest.approx(want, abs=1e-5), (v, vv, got, want)

@@ -1,0 +1,89 @@
+"""Tracking-level properties of the oracle: determinism, chunk-relative
+random streams, the two-phase protocol's layout, and analytic lengths on a
+straight-fibre phantom."""
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+
+
+def _oracle(w):
+    return Oracle(w.params, w.dataf, w.stop_map, w.vertices, w.edges, boot=w.boot, ptt_lut=w.ptt_lut)
+
+
+@pytest.mark.parametrize("kind", ["prob", "csa"])
+def test_deterministic_and_layout(small_workloads, kind):
+    from gpustreamlines_amd import workloads as W
+
+    w = small_workloads[kind]
+    seeds = W.seeds_for(w, 64, placement="mixed", seed=3)
+    a = _oracle(w).track(seeds)
+    b = _oracle(w).track(seeds)
+    assert np.array_equal(a["sline_len"], b["sline_len"])
+    assert np.array_equal(a["slines"], b["slines"])
+    assert np.array_equal(a["offsets"][1:] - a["offsets"][:-1], a["counts"])
+    # every streamline's seed index is its owner, in seed order
+    owner = np.repeat(np.arange(len(seeds)), a["counts"])
+    assert np.array_equal(a["sline_seed"], owner)
+    L = a["sline_len"]
+    assert L.min() >= 1 and L.max() <= 2 * 501 - 1
+    # the seed point sits inside each streamline (junction of the two halves)
+    for k in range(len(L)):
+        pts = a["slines"][k, :L[k]]
+        assert np.any(np.all(pts == seeds[owner[k]], axis=1))
+
+
+def test_chunk_relative_streams(small_workloads):
+    from gpustreamlines_amd import workloads as W
+
+    w = small_workloads["prob"]
+    seeds = W.seeds_for(w, 96, placement="mask", seed=4)
+    o = _oracle(w)
+    whole = o.track(seeds, rng_chunk=48)
+    second = o.track(seeds[48:])
+    k = whole["offsets"][48]
+    assert np.array_equal(whole["sline_len"][k:], second["sline_len"])
+    assert not np.array_equal(o.track(seeds)["sline_len"], whole["sline_len"])
+
+
+def test_rng_seed_changes_result(small_workloads):
+    from gpustreamlines_amd import workloads as W
+
+    w = small_workloads["prob"]
+    seeds = W.seeds_for(w, 32, placement="mask", seed=5)
+    a = _oracle(w).track(seeds)
+    p = dict(w.params, rng_seed=12345)
+    b = Oracle(p, w.dataf, w.stop_map, w.vertices, w.edges).track(seeds)
+    assert np.array_equal(a["counts"], b["counts"])  # phase 1 draws nothing for prob
+    assert not np.array_equal(a["sline_len"], b["sline_len"])
+
+
+def test_straight_fibre_lengths():
+    """ODF sharply peaked along x inside an FA slab x in [4, 24): prob
+    streamlines run along x and stop at the slab walls, so their points lie
+    in the slab and their extent is the slab width (to one step plus the
+    stop map's half-voxel shift)."""
+    from gpustreamlines_amd.sphere import symmetric_sphere
+
+    verts, edges = symmetric_sphere(724)
+    shape = (28, 10, 10)
+    lobe = (np.abs(verts[:, 0]) ** 50).astype(np.float32)
+    data = np.broadcast_to(lobe, shape + (724,)).copy()
+    fa = np.zeros(shape, np.float32)
+    fa[4:24] = 0.8
+    step = 0.25
+    p = dict(dimx=shape[0], dimy=shape[1], dimz=shape[2], dimt=724, model_type=2, nverts=724,
+             nedges=len(edges), max_sline_len=501, step_size=step, max_angle=np.radians(60),
+             tc_threshold=0.1, relative_peak_thresh=0.25, min_separation_angle=np.radians(45))
+    o = Oracle(p, data, fa, verts, edges)
+    seeds = np.array([[10.0, 5.0, 5.0], [14.3, 4.6, 5.2], [20.0, 5.0, 4.0]], np.float32)
+    r = o.track(seeds)
+    assert np.all(r["counts"] == 2)
+    for k, L in enumerate(r["sline_len"]):
+        pts = r["slines"][k, :L]
+        ext = pts[:, 0].max() - pts[:, 0].min()
+        assert 17.0 < ext < 20.5, (k, ext)
+        assert pts[:, 0].min() > 3.0 and pts[:, 0].max() < 25.0
+        steps = np.linalg.norm(np.diff(pts, axis=0), axis=1)
+        np.testing.assert_allclose(steps, step, rtol=1e-4)
