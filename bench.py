@@ -1,0 +1,217 @@
+"""Throughput of the tracking hot path on one MI355X.
+
+A step is one pass of the hot path (phase 1: initial peak directions, the
+on-device scan of direction counts, phase 2: tracking) over one batch of
+synthetic seeds already resident in HBM.  Default workload: PTT on the
+81x106x76 multi-tensor phantom (160 DWI + 10 b0, CSA ODF on a 724-vertex
+sphere), 1M seeds, half in the FA > 0.2 mask and half in 0.05 <= FA <= 0.15
+border voxels, step 0.25 voxel (0.5 mm at 2 mm), random streams numbered per
+25,000-seed chunk.  Prints one JSON line.
+
+Timing: HIP events recorded on the tracker's own stream around each phase;
+steps = sum over streamlines of (points - 1).
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+CHUNK = 25000
+
+
+def alg_bytes_per_step(kind: str, p: dict) -> float:
+    """Compulsory HBM bytes per output step with no cross-step reuse: the
+    8-corner gather of the data channels the getter reads, the 8-texel stop
+    map tap and the 12-byte point store.  PTT counts every probe tap of one
+    rejection round (1 + 24*8 + 32*8 taps of 8 texels + LUT) per sub-step."""
+    stop_tap = 8 * 4
+    if kind == "ptt":
+        taps = 1 + 24 * 8 + 32 * 8
+        return p["step_frac"] * (taps * (8 * 4 + 4) + stop_tap) + 12
+    return 8 * p["dimt"] * 4 + stop_tap + 12
+
+
+class HipEvents:
+    """hipEvent timing on an explicit stream (the library's own)."""
+
+    def __init__(self):
+        self.hip = C.CDLL("libamdhip64.so.7")
+        self.hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def create(self):
+        e = C.c_void_p()
+        assert self.hip.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.hip.hipEventRecord(e, C.c_void_p(stream)) == 0
+
+    def ms(self, a, b):
+        assert self.hip.hipEventSynchronize(b) == 0
+        t = C.c_float()
+        assert self.hip.hipEventElapsedTime(C.byref(t), a, b) == 0
+        return float(t.value)
+
+
+def cpu_baseline(w, seeds, n_sample):
+    """The CPU oracle (C++/OpenMP restatement of the reference path) on the
+    first n_sample seeds of the same workload, all host threads."""
+    from oracle.oracle import Oracle, num_threads
+
+    o = Oracle(w.params, w.dataf, w.stop_map, w.vertices, w.edges, boot=w.boot, ptt_lut=w.ptt_lut)
+    sample = seeds[:n_sample]
+    t0 = time.perf_counter()
+    r = o.track(sample, rng_chunk=CHUNK)
+    dt = time.perf_counter() - t0
+    steps = int((r["sline_len"].astype(np.int64) - 1).clip(min=0).sum())
+    return dict(value=steps / dt, unit="steps/s", cores=num_threads(), kind="port",
+                sample=f"{n_sample} seeds of the same workload ({steps} steps, {len(r['sline_len'])} "
+                       f"streamlines) in {dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--getter", default="ptt", choices=["ptt", "prob", "csa", "opdt"])
+    ap.add_argument("--seeds", type=int, default=1_000_000)
+    ap.add_argument("--placement", default="mixed", choices=["mixed", "mask"])
+    ap.add_argument("--cpu-sample", type=int, default=None)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build runs no multi-GPU configuration "
+              "(seeds shard contiguously per GPU with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from gpustreamlines_amd import workloads as W
+    from gpustreamlines_amd._device import DeviceTracker
+
+    t_setup = time.perf_counter()
+    w = W.build(args.getter)
+    seeds = W.seeds_for(w, args.seeds, placement=args.placement)
+    n = len(seeds)
+    nv = w.params["nverts"]
+    L = w.params["max_sline_len"]
+    dt = DeviceTracker(w.params, device=0)
+    dt.upload(w.dataf, w.stop_map, w.vertices, w.edges, boot=w.boot, ptt_lut=w.ptt_lut)
+    sd = dt.alloc(seeds.nbytes)
+    sd.upload(seeds)
+    cd = dt.alloc(4 * (n + 1))
+    dd = dt.alloc(4 * n * nv)
+    od = dt.alloc(4 * (n + 1))
+    stream = dt.stream()
+    ev = HipEvents()
+    e = [ev.create() for _ in range(4)]
+    print(f"[bench] setup {time.perf_counter() - t_setup:.1f}s getter={args.getter} seeds={n}", file=sys.stderr,
+          flush=True)
+
+    # Phase 1 + scan once to size the streamline buffers (same seeds every step).
+    dt.phase1_async(sd.ptr, n, CHUNK, cd.ptr, dd.ptr)
+    dt.scan_async(cd.ptr, n, od.ptr)
+    offs = od.download(np.zeros(n + 1, dtype=np.int32))
+    nsl = int(offs[n])
+    ssd = dt.alloc(4 * nsl)
+    sld = dt.alloc(4 * nsl)
+    sl = dt.alloc(4 * 3 * 2 * L * nsl)
+    print(f"[bench] {nsl} streamlines, slot buffer {4 * 3 * 2 * L * nsl / 2**30:.1f} GiB", file=sys.stderr, flush=True)
+
+    def step():
+        ev.record(e[0], stream)
+        dt.phase1_async(sd.ptr, n, CHUNK, cd.ptr, dd.ptr)
+        ev.record(e[1], stream)
+        dt.scan_async(cd.ptr, n, od.ptr)
+        ev.record(e[2], stream)
+        dt.phase2_async(sd.ptr, n, CHUNK, od.ptr, dd.ptr, ssd.ptr, sld.ptr, sl.ptr)
+        ev.record(e[3], stream)
+        dt.synchronize()
+        return ev.ms(e[0], e[1]), ev.ms(e[1], e[2]), ev.ms(e[2], e[3]), ev.ms(e[0], e[3])
+
+    for i in range(args.warmup):
+        r = step()
+        print(f"[bench] warmup {i}: {r[3] / 1e3:.2f}s", file=sys.stderr, flush=True)
+    dt.synchronize()
+    times = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        r = step()
+        times.append(r)
+        print(f"[bench] step {i}: phase1 {r[0]:.1f} ms scan {r[1]:.2f} ms phase2 {r[2]:.1f} ms", file=sys.stderr,
+              flush=True)
+    wall = time.perf_counter() - t0
+    lens = sld.download(np.zeros(nsl, dtype=np.int32)).astype(np.int64)
+    steps_per_pass = int((lens - 1).clip(min=0).sum())
+    t_step = np.array([t[3] for t in times]) / 1e3
+    t_p2 = np.array([t[2] for t in times]) / 1e3
+    ms_per_step = float(t_step.mean() * 1e3)
+    value = steps_per_pass / float(t_step.mean())
+    abytes = alg_bytes_per_step(args.getter, w.params)
+    achieved = abytes * steps_per_pass / float(t_p2.mean()) / 1e9
+
+    res = {
+        "metric": "streamline steps/sec at 1M seeds, plain vs learned-proposal PTT, 1 MI355X",
+        "value": value,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.getter}_{n}_seeds_{args.placement}_81x106x76_step0.25vox",
+            "getter": args.getter,
+            "seeds": n,
+            "streamlines": nsl,
+            "points": int(lens.sum()),
+            "streamlines_per_s": nsl / float(t_step.mean()),
+            "phase1_ms": float(np.mean([t[0] for t in times])),
+            "scan_ms": float(np.mean([t[1] for t in times])),
+            "phase2_ms": float(t_p2.mean() * 1e3),
+            "wall_s": wall,
+            "proposal": "plain",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": f"phase2_kernel<{args.getter}>",
+            "alg_bytes_per_step": abytes,
+        },
+    }
+    for b in (sd, cd, dd, od, ssd, sld, sl):
+        b.free()
+    dt.close()
+    if not args.no_cpu:
+        ns = args.cpu_sample or {"ptt": 400, "prob": 4000, "csa": 1000, "opdt": 1000}[args.getter]
+        print(f"[bench] cpu baseline on {ns} seeds", file=sys.stderr, flush=True)
+        res["cpu_baseline"] = cpu_baseline(w, seeds, ns)
+    else:
+        res["cpu_baseline"] = None
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

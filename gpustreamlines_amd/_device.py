@@ -97,6 +97,14 @@ class DeviceTracker:
     def synchronize(self):
         _lib.check(self.L.cuslines_synchronize(self.h), "synchronize")
 
+    def set_launch(self, schedule: int = 0, threads: int = 0, blocks_per_cu: int = 0):
+        """Phase-2 launch: schedule 0 = persistent work queue, 1 = static."""
+        _lib.check(self.L.cuslines_set_launch(self.h, int(schedule), int(threads), int(blocks_per_cu)), "set_launch")
+
+    def set_ptt_layout(self, corner_blocks: bool = True):
+        """PTT taps from 32-byte corner blocks (True) or 8 texel gathers."""
+        _lib.check(self.L.cuslines_set_ptt_layout(self.h, int(bool(corner_blocks))), "set_ptt_layout")
+
     def stream(self) -> int:
         s = C.c_void_p()
         _lib.check(self.L.cuslines_get_stream(self.h, C.byref(s)), "get_stream")

@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcuslines_hip.so")
+LIB_PATH = os.environ.get("CUSLINES_HIP_LIB") or os.path.join(HERE, "libcuslines_hip.so")
 
 ABI_VERSION = 1
 
@@ -94,6 +94,8 @@ def load():
         "cuslines_memcpy_dtoh": [vp, vp, vp, sz],
         "cuslines_synchronize": [vp],
         "cuslines_get_stream": [vp, C.POINTER(vp)],
+        "cuslines_set_launch": [vp, C.c_int, C.c_int, C.c_int],
+        "cuslines_set_ptt_layout": [vp, C.c_int],
         "cuslines_phase1_async": [vp, vp, i32, i32, vp, vp],
         "cuslines_scan_async": [vp, vp, i32, vp],
         "cuslines_phase2_async": [vp, vp, i32, i32, vp, vp, vp, vp, vp],

@@ -112,6 +112,7 @@ struct DevBufs {
     const int *b0;
     const int *dwi_idx;
     const uint16_t *lut;
+    const float *blocks;  // PTT corner blocks (nullptr: gather 8 texels)
 };
 
 // -------------------------------------------------------------------------
@@ -249,7 +250,7 @@ __device__ int trilinear(const DevParams &P, const float *__restrict__ data, int
     for (int t = l; t < dimt; t += 32) {
         float v[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) v[k] = __builtin_nontemporal_load(c[k] + t);
+        for (int k = 0; k < 8; ++k) v[k] = c[k][t];
         float acc = 0.0f;
 #pragma unroll
         for (int k = 0; k < 8; ++k) acc = __builtin_fmaf(w[k], v[k], acc);
@@ -352,6 +353,28 @@ __device__ __forceinline__ float interp4(const DevParams &P, const DevBufs &B, f
     const float xq = (float)(tx * P.dimx) + px;
     const float yq = (float)(ty * P.dimy) + py;
     const float zq = (float)(tz * P.dimz) + pz;
+    if (B.blocks) {
+        int i, j, k;
+        const float a = tex_weight(xq, &i), b = tex_weight(yq, &j), g = tex_weight(zq, &k);
+        const int cx = i - tx * P.dimx, cy = j - ty * P.dimy, cz = k - tz * P.dimz;
+        if ((unsigned)(cx + 1) <= (unsigned)P.dimx && (unsigned)(cy + 1) <= (unsigned)P.dimy &&
+            (unsigned)(cz + 1) <= (unsigned)P.dimz) {
+            const size_t cell = (((size_t)idx * (P.dimx + 1) + (cx + 1)) * (P.dimy + 1) + (cy + 1)) * (P.dimz + 1) +
+                                (cz + 1);
+            const float4 *blk = reinterpret_cast<const float4 *>(B.blocks + cell * 8);
+            const float4 lo = blk[0], hi = blk[1];
+            const float a0 = 1.0f - a, b0 = 1.0f - b, g0 = 1.0f - g;
+            float acc = ((a0 * b0) * g0) * lo.x;
+            acc = __builtin_fmaf((a * b0) * g0, lo.y, acc);
+            acc = __builtin_fmaf((a0 * b) * g0, lo.z, acc);
+            acc = __builtin_fmaf((a * b) * g0, lo.w, acc);
+            acc = __builtin_fmaf((a0 * b0) * g, hi.x, acc);
+            acc = __builtin_fmaf((a * b0) * g, hi.y, acc);
+            acc = __builtin_fmaf((a0 * b) * g, hi.z, acc);
+            acc = __builtin_fmaf((a * b) * g, hi.w, acc);
+            return acc;
+        }
+    }
     return tiled_sample(P, B.data, xq, yq, zq, tx, ty, tz);
 }
 
@@ -381,11 +404,18 @@ __device__ void gemv_rows(int N, int M, const float *v, const float *__restrict_
         float acc[32];
 #pragma unroll
         for (int r = 0; r < 32; ++r) acc[r] = 0.0f;
+        // Rows past N read row N-1 (valid memory, result discarded) so the 32
+        // loads of one column step issue back to back without branches.
+        const float *rowp[32];
+#pragma unroll
+        for (int r = 0; r < 32; ++r) rowp[r] = mat + (size_t)min(i0 + r, N - 1) * M;
         for (int j = l; j < M; j += 32) {
+            float m[32];
+#pragma unroll
+            for (int r = 0; r < 32; ++r) m[r] = rowp[r][j];
             const float x = gemv_xf<XF>(v[j]);
 #pragma unroll
-            for (int r = 0; r < 32; ++r)
-                if (i0 + r < N) acc[r] = __builtin_fmaf(x, mat[(size_t)(i0 + r) * M + j], acc[r]);
+            for (int r = 0; r < 32; ++r) acc[r] = __builtin_fmaf(x, m[r], acc[r]);
         }
         // reduce-scatter: after the stage with distance d each lane keeps d rows
 #pragma unroll
@@ -778,16 +808,17 @@ __device__ bool init_frame_ptt(const DevParams &P, const DevBufs &B, PttLds &S, 
                                float fx, float fy, float fz, float sx, float sy, float sz) {
     const int l = lane32();
     float tx, ty, tz;
-    bool ok = get_direction_ptt<true>(P, B, S, S.frame, rng, fx, fy, fz, sx, sy, sz, &tx, &ty, &tz);
-    if (!ok) {
-        ok = get_direction_ptt<true>(P, B, S, S.frame, rng, -fx, -fy, -fz, sx, sy, sz, &tx, &ty, &tz);
-        if (!ok) return false;
-    } else {
-        if (l == 0)
-            for (int i = 0; i < 6; ++i) S.frame[i] = -S.frame[i];
-        WAVE_SYNC();
+    int att = 0;
+    bool ok = false;
+    for (; att < 2; ++att) {  // +dir first, then -dir
+        const float sg = att ? -1.0f : 1.0f;
+        ok = get_direction_ptt<true>(P, B, S, S.frame, rng, sg * fx, sg * fy, sg * fz, sx, sy, sz, &tx, &ty, &tz);
+        if (ok) break;
     }
+    if (!ok) return false;
     if (l == 0) {
+        if (att == 0)
+            for (int i = 0; i < 6; ++i) S.frame[i] = -S.frame[i];
         for (int i = 0; i < 6; ++i) S.frame[9 + i] = -S.frame[i];
         for (int i = 0; i < 3; ++i) S.frame[15 + i] = S.frame[6 + i];
     }
@@ -944,8 +975,59 @@ __global__ void __launch_bounds__(256) phase1_kernel(DevParams P, DevBufs B, con
     if (l == 0) counts[s] = n;
 }
 
-// genStreamlinesMerge{Prob,Boot}_k (generate_streamlines_cuda.cu:464-622,
-// boot.cu:826-980): per direction, backward track, reverse, forward track.
+// genStreamlinesMerge{Prob,Boot}_k body for one seed (generate_streamlines_cuda.cu:484-621,
+// boot.cu:855-979): per direction, backward track, reverse, forward track.
+template <int MODEL>
+__device__ void track_seed(const DevParams &P, const DevBufs &B, const float *sv, SlotLds &S, int s,
+                           const float *__restrict__ seeds, int rng_chunk, const int *__restrict__ offsets,
+                           const int *__restrict__ dirs, int *__restrict__ sline_seed, int *__restrict__ sline_len,
+                           float *__restrict__ sline) {
+    const int l = lane32();
+    const float sx = seeds[3 * s], sy = seeds[3 * s + 1], sz = seeds[3 * s + 2];
+    Philox rng;
+    rng.init(((uint64_t)P.seed_hi << 32) | P.seed_lo, (uint64_t)32 * (uint64_t)(s % rng_chunk) + l + 1);
+    const int ndir = offsets[s + 1] - offsets[s];
+    int off = offsets[s];
+    const size_t slot_f = (size_t)2 * P.max_sline_len * 3;
+    for (int i = 0; i < ndir; ++i, ++off) {
+        const int vi = dirs[(size_t)s * P.nverts + i];
+        const float fx = B.verts[3 * vi], fy = B.verts[3 * vi + 1], fz = B.verts[3 * vi + 2];
+        float *cur = sline + (size_t)off * slot_f;
+        if (l == 0) sline_seed[off] = s;
+        if (MODEL == CUSLINES_PTT) {
+            if (!init_frame_ptt(P, B, S.ptt, rng, fx, fy, fz, sx, sy, sz)) {
+                if (l == 0) {
+                    sline_len[off] = 1;
+                    cur[0] = sx; cur[1] = sy; cur[2] = sz;
+                }
+                continue;
+            }
+        }
+        int nb = 0, nf = 0;
+        for (int pass = 0; pass < 2; ++pass) {  // backward (-dir), then forward (+dir)
+            const float sg = pass ? 1.0f : -1.0f;
+            float *frame = (MODEL == CUSLINES_PTT) ? S.ptt.frame + 9 * pass : nullptr;
+            float *out = pass ? cur + 3 * (size_t)(nb - 1) : cur;
+            const int ns = tracker<MODEL>(P, B, sv, S, rng, sx, sy, sz, sg * fx, sg * fy, sg * fz, frame, out);
+            if (pass == 0) {
+                nb = ns;
+                MEM_SYNC();
+                for (int j = l; j < nb / 2; j += 32) {
+                    float *a = cur + 3 * (size_t)j, *b = cur + 3 * (size_t)(nb - 1 - j);
+                    const float a0 = a[0], a1 = a[1], a2 = a[2];
+                    a[0] = b[0]; a[1] = b[1]; a[2] = b[2];
+                    b[0] = a0; b[1] = a1; b[2] = a2;
+                }
+                MEM_SYNC();
+            } else {
+                nf = ns;
+            }
+        }
+        if (l == 0) sline_len[off] = nb - 1 + nf;
+    }
+}
+
+// One seed per 32-lane slot, static assignment.
 template <int MODEL>
 __global__ void __launch_bounds__(256) phase2_kernel(DevParams P, DevBufs B, const float *__restrict__ seeds,
                                                      int nseeds, int rng_chunk, const int *__restrict__ offsets,
@@ -963,46 +1045,82 @@ __global__ void __launch_bounds__(256) phase2_kernel(DevParams P, DevBufs B, con
     const int slot = threadIdx.x >> 5;
     const int s = blockIdx.x * nslots + slot;
     if (s >= nseeds) return;
-    const int l = lane32();
-    float *base = smem + svf + slot * slot_floats(MODEL, false, P.nv32, P.t32, P.hr32);
     SlotLds S;
-    bind_slot<MODEL>(P, false, base, S);
-    const float sx = seeds[3 * s], sy = seeds[3 * s + 1], sz = seeds[3 * s + 2];
-    Philox rng;
-    rng.init(((uint64_t)P.seed_hi << 32) | P.seed_lo, (uint64_t)32 * (uint64_t)(s % rng_chunk) + l + 1);
-    const int ndir = offsets[s + 1] - offsets[s];
-    int off = offsets[s];
-    const size_t slot_f = (size_t)2 * P.max_sline_len * 3;
-    for (int i = 0; i < ndir; ++i) {
-        const int vi = dirs[(size_t)s * P.nverts + i];
-        const float fx = B.verts[3 * vi], fy = B.verts[3 * vi + 1], fz = B.verts[3 * vi + 2];
-        float *cur = sline + (size_t)off * slot_f;
-        if (l == 0) sline_seed[off] = s;
-        if (MODEL == CUSLINES_PTT) {
-            if (!init_frame_ptt(P, B, S.ptt, rng, fx, fy, fz, sx, sy, sz)) {
-                if (l == 0) {
-                    sline_len[off] = 1;
-                    cur[0] = sx; cur[1] = sy; cur[2] = sz;
-                }
-                ++off;
-                continue;
-            }
-        }
-        const int nb = tracker<MODEL>(P, B, sv, S, rng, sx, sy, sz, -fx, -fy, -fz, S.ptt.frame, cur);
-        MEM_SYNC();
-        for (int j = l; j < nb / 2; j += 32) {
-            float *a = cur + 3 * (size_t)j, *b = cur + 3 * (size_t)(nb - 1 - j);
-            const float a0 = a[0], a1 = a[1], a2 = a[2];
-            a[0] = b[0]; a[1] = b[1]; a[2] = b[2];
-            b[0] = a0; b[1] = a1; b[2] = a2;
-        }
-        MEM_SYNC();
-        const int nf = tracker<MODEL>(P, B, sv, S, rng, sx, sy, sz, fx, fy, fz,
-                                      (MODEL == CUSLINES_PTT) ? S.ptt.frame + 9 : nullptr,
-                                      cur + 3 * (size_t)(nb - 1));
-        if (l == 0) sline_len[off] = nb - 1 + nf;
-        ++off;
+    bind_slot<MODEL>(P, false, smem + svf + slot * slot_floats(MODEL, false, P.nv32, P.t32, P.hr32), S);
+    track_seed<MODEL>(P, B, sv, S, s, seeds, rng_chunk, offsets, dirs, sline_seed, sline_len, sline);
+}
+
+// Persistent work queue: a fixed grid of slots; each 32-lane slot takes the
+// next seed from a global counter as soon as its previous seed is done, so
+// seeds of very uneven length keep every slot busy.  `work` must hold 0.
+template <int MODEL>
+__global__ void __launch_bounds__(256) phase2_queue_kernel(DevParams P, DevBufs B, const float *__restrict__ seeds,
+                                                           int nseeds, int rng_chunk, const int *__restrict__ offsets,
+                                                           const int *__restrict__ dirs, int *__restrict__ sline_seed,
+                                                           int *__restrict__ sline_len, float *__restrict__ sline,
+                                                           int *__restrict__ work) {
+    extern __shared__ float smem[];
+    float *sv = smem;
+    int svf = 0;
+    if (MODEL != CUSLINES_PTT) {
+        for (int i = threadIdx.x; i < 3 * P.nverts; i += blockDim.x) sv[i] = B.verts[i];
+        svf = verts_floats(P);
     }
+    __syncthreads();
+    const int slot = threadIdx.x >> 5;
+    const int l = lane32();
+    SlotLds S;
+    bind_slot<MODEL>(P, false, smem + svf + slot * slot_floats(MODEL, false, P.nv32, P.t32, P.hr32), S);
+    for (;;) {
+        int s = 0;
+        if (l == 0) s = atomicAdd(work, 1);
+        s = shfl32(s, 0);
+        if (s >= nseeds) break;
+        track_seed<MODEL>(P, B, sv, S, s, seeds, rng_chunk, offsets, dirs, sline_seed, sline_len, sline);
+    }
+}
+
+__global__ void zero_int(int *p) { *p = 0; }
+
+// PTT corner blocks: for vertex v and texel cell (cx,cy,cz) in [-1, D-1]^3 of
+// its tile, the 8 texels one linear-filtered tap reads (x fastest), with the
+// tiling's bleed into neighbouring tiles and the clamp at the texture edge.
+__global__ void build_corner_blocks(DevParams P, const float *__restrict__ odfv, float *__restrict__ blocks,
+                                    long long ncells) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ncells) return;
+    const int DX = P.dimx, DY = P.dimy, DZ = P.dimz;
+    long long t = idx;
+    const int cz = (int)(t % (DZ + 1)) - 1; t /= (DZ + 1);
+    const int cy = (int)(t % (DY + 1)) - 1; t /= (DY + 1);
+    const int cx = (int)(t % (DX + 1)) - 1; t /= (DX + 1);
+    const int v = (int)t;
+    const int tx = v & (P.nx - 1), ty = (v >> P.log2x) & (P.ny - 1), tz = v >> (P.log2x + P.log2y);
+    int txa, txb, lxa, lxb, tya, tyb, lya, lyb, tza, tzb, lza, lzb;
+    axis_split(tx * DX + cx, DX, P.nx, tx, &txa, &lxa);
+    axis_split(tx * DX + cx + 1, DX, P.nx, tx, &txb, &lxb);
+    axis_split(ty * DY + cy, DY, P.ny, ty, &tya, &lya);
+    axis_split(ty * DY + cy + 1, DY, P.ny, ty, &tyb, &lyb);
+    axis_split(tz * DZ + cz, DZ, P.nz, tz, &tza, &lza);
+    axis_split(tz * DZ + cz + 1, DZ, P.nz, tz, &tzb, &lzb);
+    const int NV = P.nverts, nx = P.nx, ny = P.ny;
+    auto fetch = [&](int ti, int li, int tj, int lj, int tk, int lk) -> float {
+        const int w = ti + nx * (tj + ny * tk);
+        if (w >= NV) return 0.0f;
+        return odfv[(((size_t)w * DX + li) * DY + lj) * DZ + lk];
+    };
+    float4 lo, hi;
+    lo.x = fetch(txa, lxa, tya, lya, tza, lza);
+    lo.y = fetch(txb, lxb, tya, lya, tza, lza);
+    lo.z = fetch(txa, lxa, tyb, lyb, tza, lza);
+    lo.w = fetch(txb, lxb, tyb, lyb, tza, lza);
+    hi.x = fetch(txa, lxa, tya, lya, tzb, lzb);
+    hi.y = fetch(txb, lxb, tya, lya, tzb, lzb);
+    hi.z = fetch(txa, lxa, tyb, lyb, tzb, lzb);
+    hi.w = fetch(txb, lxb, tyb, lyb, tzb, lzb);
+    float4 *out = reinterpret_cast<float4 *>(blocks + (size_t)idx * 8);
+    out[0] = lo;
+    out[1] = hi;
 }
 
 // PTT data: [X][Y][Z][T] -> vertex-major [T][X][Y][Z] (32x32 LDS tiles).
@@ -1131,6 +1249,13 @@ struct cuslines_tracker {
     float *d_H = nullptr, *d_R = nullptr, *d_db = nullptr, *d_dq = nullptr, *d_smat = nullptr;
     int *d_b0 = nullptr, *d_dwi = nullptr;
     uint16_t *d_lut = nullptr;
+    int *d_work = nullptr;      // work-queue counter of the persistent phase-2 kernel
+    float *d_blocks = nullptr;  // PTT corner blocks
+    int use_blocks = 1;
+    int schedule = CUSLINES_SCHED_QUEUE;
+    int threads = 0;        // 0: per-getter default
+    int blocks_per_cu = 0;  // 0: occupancy maximum
+    int num_cu = 1;
     bool has_data = false, has_stop = false, has_sphere = false, has_boot = false, has_lut = false;
 };
 
@@ -1152,6 +1277,7 @@ static DevBufs bufs_of(const cuslines_tracker *t) {
     b.b0 = t->d_b0;
     b.dwi_idx = t->d_dwi;
     b.lut = t->d_lut;
+    b.blocks = t->use_blocks ? t->d_blocks : nullptr;
     return b;
 }
 
@@ -1188,15 +1314,34 @@ template <int MODEL>
 static int launch_phase2(cuslines_tracker *t, const float *seeds, int n, int chunk, const int *offsets,
                          const int *dirs, int *sseed, int *slen, float *sl) {
     const DevParams &d = t->dp;
-    const int threads = (MODEL == CUSLINES_OPDT || MODEL == CUSLINES_CSA) ? 128 : 256;
+    int threads = t->threads;
+    if (threads <= 0) threads = (MODEL == CUSLINES_OPDT || MODEL == CUSLINES_CSA) ? 128 : 256;
     const int nslots = threads / 32;
     const size_t vf = (MODEL == CUSLINES_PTT) ? 0 : (size_t)((3 * d.nverts + 3) & ~3);
     const size_t lds = sizeof(float) * (vf + (size_t)nslots * slot_floats(MODEL, false, d.nv32, d.t32, d.hr32));
     if (lds > 160 * 1024) return fail(CUSLINES_EINVAL, "sphere too large for LDS");
-    HIPC(hipFuncSetAttribute((const void *)phase2_kernel<MODEL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    const unsigned grid = (unsigned)((n + nslots - 1) / nslots);
-    hipLaunchKernelGGL(phase2_kernel<MODEL>, dim3(grid), dim3(threads), lds, t->stream, d, bufs_of(t), seeds, n,
-                       chunk, offsets, dirs, sseed, slen, sl);
+    if (t->schedule == CUSLINES_SCHED_STATIC) {
+        HIPC(hipFuncSetAttribute((const void *)phase2_kernel<MODEL>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)lds));
+        const unsigned grid = (unsigned)((n + nslots - 1) / nslots);
+        hipLaunchKernelGGL(phase2_kernel<MODEL>, dim3(grid), dim3(threads), lds, t->stream, d, bufs_of(t), seeds, n,
+                           chunk, offsets, dirs, sseed, slen, sl);
+        HIPC(hipGetLastError());
+        return CUSLINES_OK;
+    }
+    const void *kfn = (const void *)phase2_queue_kernel<MODEL>;
+    HIPC(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = 0;
+    HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, threads, lds));
+    if (per_cu < 1) per_cu = 1;
+    if (t->blocks_per_cu > 0 && t->blocks_per_cu < per_cu) per_cu = t->blocks_per_cu;
+    long long grid = (long long)per_cu * t->num_cu;
+    const long long need = (n + nslots - 1) / nslots;
+    if (grid > need) grid = need;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(zero_int, dim3(1), dim3(1), 0, t->stream, t->d_work);
+    hipLaunchKernelGGL(phase2_queue_kernel<MODEL>, dim3((unsigned)grid), dim3(threads), lds, t->stream, d,
+                       bufs_of(t), seeds, n, chunk, offsets, dirs, sseed, slen, sl, t->d_work);
     HIPC(hipGetLastError());
     return CUSLINES_OK;
 }
@@ -1271,11 +1416,20 @@ int cuslines_tracker_create(const cuslines_params *p, int device, cuslines_track
         delete t;
         return fail(CUSLINES_EHIP, std::string("hipStreamCreateWithFlags: ") + hipGetErrorString(e));
     }
+    e = hipMalloc(&t->d_work, sizeof(int));
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&t->num_cu, hipDeviceAttributeMultiprocessorCount, device);
+    if (e != hipSuccess) {
+        dfree(t->d_work);
+        (void)hipStreamDestroy(t->stream);
+        delete t;
+        return fail(CUSLINES_EHIP, std::string("tracker setup: ") + hipGetErrorString(e));
+    }
     unsigned char net[SORT_LAYERS][32];
     int nl = 0;
     make_sortnet(net, &nl);
     e = hipMemcpyToSymbol(HIP_SYMBOL(c_sortnet), net, sizeof(net));
     if (e != hipSuccess) {
+        dfree(t->d_work);
         (void)hipStreamDestroy(t->stream);
         delete t;
         return fail(CUSLINES_EHIP, std::string("sort network upload: ") + hipGetErrorString(e));
@@ -1314,7 +1468,7 @@ int cuslines_tracker_destroy(cuslines_tracker *t) {
     if (t->stream) (void)hipStreamSynchronize(t->stream);
     dfree(t->d_data); dfree(t->d_stop); dfree(t->d_verts); dfree(t->d_edges);
     dfree(t->d_H); dfree(t->d_R); dfree(t->d_db); dfree(t->d_dq); dfree(t->d_smat);
-    dfree(t->d_b0); dfree(t->d_dwi); dfree(t->d_lut);
+    dfree(t->d_b0); dfree(t->d_dwi); dfree(t->d_lut); dfree(t->d_work); dfree(t->d_blocks);
     if (t->stream) (void)hipStreamDestroy(t->stream);
     delete t;
     return CUSLINES_OK;
@@ -1343,6 +1497,19 @@ int cuslines_set_data(cuslines_tracker *t, const float *dataf) {
         if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
         dfree(tmp);
         if (e != hipSuccess) return fail(CUSLINES_EHIP, std::string("PTT data upload: ") + hipGetErrorString(e));
+        dfree(t->d_blocks);
+        t->d_blocks = nullptr;
+        const long long ncells = (long long)d.nverts * (d.dimx + 1) * (d.dimy + 1) * (d.dimz + 1);
+        if (hipMalloc(&t->d_blocks, (size_t)ncells * 8 * sizeof(float)) == hipSuccess) {
+            hipLaunchKernelGGL(build_corner_blocks, dim3((unsigned)((ncells + 255) / 256)), dim3(256), 0, t->stream, d,
+                               t->d_data, t->d_blocks, ncells);
+            e = hipGetLastError();
+            if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+            if (e != hipSuccess) return fail(CUSLINES_EHIP, std::string("PTT blocks: ") + hipGetErrorString(e));
+        } else {
+            (void)hipGetLastError();  // no room for the blocks: taps gather their 8 texels
+            t->d_blocks = nullptr;
+        }
     } else {
         const size_t pitch = (size_t)d.t32 * sizeof(float);
         HIPC(hipMalloc(&t->d_data, nvox * pitch));
@@ -1478,6 +1645,25 @@ int cuslines_synchronize(cuslines_tracker *t) {
     if (!t) return fail(CUSLINES_EINVAL, "null tracker");
     HIPC(hipSetDevice(t->device));
     HIPC(hipStreamSynchronize(t->stream));
+    return CUSLINES_OK;
+}
+
+int cuslines_set_launch(cuslines_tracker *t, int schedule, int threads, int blocks_per_cu) {
+    if (!t) return fail(CUSLINES_EINVAL, "null tracker");
+    if (schedule != CUSLINES_SCHED_QUEUE && schedule != CUSLINES_SCHED_STATIC)
+        return fail(CUSLINES_EINVAL, "unknown schedule");
+    if (threads != 0 && (threads < 64 || threads > 256 || threads % 64 != 0))
+        return fail(CUSLINES_EINVAL, "threads per block must be 0 or a multiple of 64 in [64, 256]");
+    if (blocks_per_cu < 0) return fail(CUSLINES_EINVAL, "blocks_per_cu must be >= 0");
+    t->schedule = schedule;
+    t->threads = threads;
+    t->blocks_per_cu = blocks_per_cu;
+    return CUSLINES_OK;
+}
+
+int cuslines_set_ptt_layout(cuslines_tracker *t, int corner_blocks) {
+    if (!t) return fail(CUSLINES_EINVAL, "null tracker");
+    t->use_blocks = corner_blocks ? 1 : 0;
     return CUSLINES_OK;
 }
 

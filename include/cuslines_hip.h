@@ -130,6 +130,25 @@ int cuslines_set_boot_matrices(cuslines_tracker *t, const float *H, const float 
  * cu_direction_getters.py:601-623, uploaded as a point-sampled texture). */
 int cuslines_set_ptt_lut(cuslines_tracker *t, const int32_t *lut);
 
+/* Phase-2 schedules. */
+enum {
+    CUSLINES_SCHED_QUEUE = 0,  /* persistent grid, slots pull seeds from a counter */
+    CUSLINES_SCHED_STATIC = 1  /* one seed per 32-lane slot, one launch-wide grid  */
+};
+
+/* Launch configuration of phase 2 (chosen by gpustreamlines_amd.autotune):
+ * schedule, threads per block (0 = getter default, else 64..256 in steps of
+ * 64; every 32 threads are one seed slot) and resident blocks per CU of the
+ * persistent grid (0 = the occupancy maximum).  The reference's launch shape
+ * is fixed at 2 seeds per 64-thread block (cu_propagate_seeds.py:45-54). */
+int cuslines_set_launch(cuslines_tracker *t, int schedule, int threads, int blocks_per_cu);
+
+/* PTT probe taps: 1 (default) reads each tap's 8 texels as one 32-byte
+ * corner block, an expanded copy of the ODF built by cuslines_set_data when
+ * HBM allows (nverts x (X+1)(Y+1)(Z+1) x 32 B); 0 gathers the 8 texels from
+ * the vertex-major ODF.  Results are identical either way. */
+int cuslines_set_ptt_layout(cuslines_tracker *t, int corner_blocks);
+
 /* Device memory helpers for callers that keep seeds and outputs resident. */
 int cuslines_device_alloc(cuslines_tracker *t, size_t bytes, void **dptr);
 int cuslines_device_free(cuslines_tracker *t, void *dptr);

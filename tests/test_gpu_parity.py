@@ -14,13 +14,15 @@ pytestmark = pytest.mark.gpu
 NSEEDS = {"prob": 256, "ptt": 48, "csa": 128, "opdt": 128}
 
 
-def _run_both(w, seeds, rng_chunk=0):
+def _run_both(w, seeds, rng_chunk=0, launch=(0, 0, 0), blocks=True):
     from gpustreamlines_amd._device import DeviceTracker
     from oracle.oracle import Oracle
 
-    print(f"[parity] {w.kind} seeds={len(seeds)} chunk={rng_chunk}", flush=True)
+    print(f"[parity] {w.kind} seeds={len(seeds)} chunk={rng_chunk} launch={launch}", flush=True)
     dt = DeviceTracker(w.params, device=0)
     try:
+        dt.set_launch(*launch)
+        dt.set_ptt_layout(blocks)
         dt.upload(w.dataf, w.stop_map, w.vertices, w.edges, boot=w.boot, ptt_lut=w.ptt_lut)
         g = dt.track(seeds, rng_chunk=rng_chunk, keep_dirs=True)
     finally:
@@ -51,6 +53,28 @@ def test_track_bit_exact(small_workloads, kind):
     seeds = W.seeds_for(w, NSEEDS[kind], placement="mixed")
     g, o = _run_both(w, seeds)
     assert len(o["sline_len"]) > 0
+    _assert_same(g, o)
+
+
+@pytest.mark.parametrize("kind,launch", [("prob", (1, 0, 0)), ("prob", (0, 64, 1)), ("ptt", (1, 128, 0)),
+                                         ("csa", (0, 64, 2)), ("opdt", (1, 0, 0))])
+def test_launch_configs_bit_exact(small_workloads, kind, launch):
+    """Static and persistent-queue schedules, any block size: same output."""
+    from gpustreamlines_amd import workloads as W
+
+    w = small_workloads[kind]
+    seeds = W.seeds_for(w, NSEEDS[kind] // 2, placement="mixed", seed=11)
+    g, o = _run_both(w, seeds, launch=launch)
+    _assert_same(g, o)
+
+
+def test_ptt_texel_gather_path_bit_exact(small_workloads):
+    """PTT taps gathered texel by texel (no corner blocks): same output."""
+    from gpustreamlines_amd import workloads as W
+
+    w = small_workloads["ptt"]
+    seeds = W.seeds_for(w, 32, placement="mixed", seed=12)
+    g, o = _run_both(w, seeds, blocks=False)
     _assert_same(g, o)
 
 
