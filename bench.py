@@ -84,6 +84,21 @@ def cpu_baseline(w, seeds, n_sample):
                        f"streamlines) in {dt:.1f} s")
 
 
+def copy_calibration(nbytes):
+    """One hipMemcpy device-to-device of nbytes: a plain copy kernel of known
+    size, whose FETCH_SIZE/WRITE_SIZE scale the tracking kernels' counters."""
+    hip = C.CDLL("libamdhip64.so.7")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes": nbytes}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +109,8 @@ def main():
     ap.add_argument("--placement", default="mixed", choices=["mixed", "mask"])
     ap.add_argument("--cpu-sample", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run one device-to-device copy of BYTES (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs no multi-GPU configuration "
@@ -103,6 +120,9 @@ def main():
     from gpustreamlines_amd import workloads as W
     from gpustreamlines_amd._device import DeviceTracker
 
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     t_setup = time.perf_counter()
     w = W.build(args.getter)
     seeds = W.seeds_for(w, args.seeds, placement=args.placement)
