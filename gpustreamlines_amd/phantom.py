@@ -16,6 +16,11 @@ import numpy as np
 
 from .sphere import GradientTable, multi_shell_gtab, single_shell_gtab
 
+# Spatial frequencies (cycles per field of view) of the random fields.
+DIR_FREQ = 3.0
+DENS_FREQ = 4.0
+DENS_OFF = -0.5
+
 LAMBDA_PAR = 1.7e-3
 LAMBDA_PERP = 0.3e-3
 D_TISSUE = 0.8e-3
@@ -87,11 +92,11 @@ def make_phantom(shape=(81, 106, 76), gtab: GradientTable | None = None, S0=1000
     c = [np.linspace(-1, 1, s, dtype=np.float32) for s in shape]
     xx, yy, zz = np.meshgrid(*c, indexing="ij")
     brain = (xx ** 2 + yy ** 2 + zz ** 2) < 0.92 ** 2
-    dens = _smooth_field(rng, shape, 1, nmodes=5, max_freq=1.5)[..., 0]
-    dens = 1.0 / (1.0 + np.exp(-2.5 * (dens + 0.6)))
+    dens = _smooth_field(rng, shape, 1, nmodes=5, max_freq=DENS_FREQ)[..., 0]
+    dens = 1.0 / (1.0 + np.exp(-2.5 * (dens + DENS_OFF)))
     wm = np.where(brain, dens, 0.0).astype(np.float32)
     # Fibre orientations: smooth fields, decorrelated populations.
-    dirs = [_unit(_smooth_field(rng, shape, 3, nmodes=4, max_freq=1.2)) for _ in range(max_fibers)]
+    dirs = [_unit(_smooth_field(rng, shape, 3, nmodes=4, max_freq=DIR_FREQ)) for _ in range(max_fibers)]
     nf_field = _smooth_field(rng, shape, 2, nmodes=4, max_freq=1.5)
     nfib = 1 + (nf_field[..., 0] > 0.3).astype(np.int32)
     if max_fibers > 2:
