@@ -101,6 +101,10 @@ class DeviceTracker:
         """Phase-2 launch: schedule 0 = persistent work queue, 1 = static."""
         _lib.check(self.L.cuslines_set_launch(self.h, int(schedule), int(threads), int(blocks_per_cu)), "set_launch")
 
+    def set_lanes_per_seed(self, lanes: int = 0):
+        """PTT phase 2: lanes per seed (8, 16, 32; 0 = shared 32-lane kernel)."""
+        _lib.check(self.L.cuslines_set_lanes_per_seed(self.h, int(lanes)), "set_lanes_per_seed")
+
     def set_ptt_layout(self, corner_blocks: bool = True):
         """PTT taps from 32-byte corner blocks (True) or 8 texel gathers."""
         _lib.check(self.L.cuslines_set_ptt_layout(self.h, int(bool(corner_blocks))), "set_ptt_layout")

@@ -96,6 +96,7 @@ def load():
         "cuslines_get_stream": [vp, C.POINTER(vp)],
         "cuslines_set_launch": [vp, C.c_int, C.c_int, C.c_int],
         "cuslines_set_ptt_layout": [vp, C.c_int],
+        "cuslines_set_lanes_per_seed": [vp, C.c_int],
         "cuslines_phase1_async": [vp, vp, i32, i32, vp, vp],
         "cuslines_scan_async": [vp, vp, i32, vp],
         "cuslines_phase2_async": [vp, vp, i32, i32, vp, vp, vp, vp, vp],

@@ -1,0 +1,100 @@
+"""Launch-shape autotuner for phase 2.
+
+Every launch shape gives bit-identical streamlines (the random streams are
+keyed by seed and virtual lane, never by hardware slot), so the tuner only
+measures: it runs phase 2 of a seed sample under each candidate shape on the
+tracker's stream, times it with HIP events, and applies the fastest.
+
+Tuned knobs: schedule (persistent queue or static), threads per block (seed
+slots per workgroup), resident blocks per CU and, for PTT, lanes per seed
+(seeds per wavefront).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import itertools
+
+import numpy as np
+
+from ._globals import MAX_SLINE_LEN
+
+
+class _Events:
+    def __init__(self):
+        self.hip = C.CDLL("libamdhip64.so.7")
+        for f, a in (("hipEventCreate", [C.POINTER(C.c_void_p)]), ("hipEventRecord", [C.c_void_p, C.c_void_p]),
+                     ("hipEventSynchronize", [C.c_void_p]),
+                     ("hipEventElapsedTime", [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]),
+                     ("hipEventDestroy", [C.c_void_p])):
+            getattr(self.hip, f).argtypes = a
+
+    def pair(self):
+        a, b = C.c_void_p(), C.c_void_p()
+        self.hip.hipEventCreate(C.byref(a))
+        self.hip.hipEventCreate(C.byref(b))
+        return a, b
+
+    def ms(self, a, b):
+        self.hip.hipEventSynchronize(b)
+        t = C.c_float()
+        self.hip.hipEventElapsedTime(C.byref(t), a, b)
+        return float(t.value)
+
+
+def candidates(model_type: int):
+    if model_type == 3:  # PTT
+        for lanes, threads in itertools.product((0, 16, 8), (256, 128)):
+            yield dict(schedule=0, threads=threads, blocks_per_cu=0, lanes=lanes)
+        yield dict(schedule=1, threads=256, blocks_per_cu=0, lanes=0)
+    else:
+        threads = (128, 64) if model_type in (0, 1) else (256, 128, 64)
+        for t in threads:
+            yield dict(schedule=0, threads=t, blocks_per_cu=0, lanes=0)
+        yield dict(schedule=1, threads=threads[0], blocks_per_cu=0, lanes=0)
+
+
+def apply(dt, cfg):
+    dt.set_launch(cfg["schedule"], cfg["threads"], cfg["blocks_per_cu"])
+    dt.set_lanes_per_seed(cfg["lanes"])
+
+
+def tune(dt, seeds: np.ndarray, rng_chunk: int = 0, verbose: bool = False):
+    """Time phase 2 of ``seeds`` under each candidate shape on device tracker
+    ``dt``; apply and return the fastest ``(config, {config: ms})``."""
+    seeds = np.ascontiguousarray(seeds, dtype=np.float32)
+    n = len(seeds)
+    nv = dt.params["nverts"]
+    bufs = []
+    try:
+        sd = dt.alloc(seeds.nbytes); bufs.append(sd)
+        cd = dt.alloc(4 * (n + 1)); bufs.append(cd)
+        dd = dt.alloc(4 * n * nv); bufs.append(dd)
+        od = dt.alloc(4 * (n + 1)); bufs.append(od)
+        sd.upload(seeds)
+        dt.phase1_async(sd.ptr, n, rng_chunk, cd.ptr, dd.ptr)
+        dt.scan_async(cd.ptr, n, od.ptr)
+        nsl = int(od.download(np.zeros(n + 1, dtype=np.int32))[n])
+        ssd = dt.alloc(4 * max(nsl, 1)); bufs.append(ssd)
+        sld = dt.alloc(4 * max(nsl, 1)); bufs.append(sld)
+        sl = dt.alloc(4 * 3 * 2 * MAX_SLINE_LEN * max(nsl, 1)); bufs.append(sl)
+        ev = _Events()
+        a, b = ev.pair()
+        stream = C.c_void_p(dt.stream())
+        times = {}
+        for cfg in candidates(dt.params["model_type"]):
+            apply(dt, cfg)
+            ev.hip.hipEventRecord(a, stream)
+            dt.phase2_async(sd.ptr, n, rng_chunk, od.ptr, dd.ptr, ssd.ptr, sld.ptr, sl.ptr)
+            ev.hip.hipEventRecord(b, stream)
+            dt.synchronize()
+            key = tuple(sorted(cfg.items()))
+            times[key] = ev.ms(a, b)
+            if verbose:
+                print(f"[autotune] {cfg} {times[key]:.1f} ms", flush=True)
+        best = dict(min(times, key=times.get))
+        apply(dt, best)
+        return best, times
+    finally:
+        for buf in bufs:
+            buf.free()

@@ -880,16 +880,466 @@ __device__ int tracker(const DevParams &P, const DevBufs &B, const float *sv, Sl
 }
 
 // -------------------------------------------------------------------------
+// PTT with L lanes per seed (L in {8, 16, 32}; 64/L seeds per wave).
+//
+// The reference gives every seed 32 lanes, each with its own Philox stream
+// (subsequence 32*slot + lane + 1).  Here those 32 streams are virtual lanes
+// whose state -- the next draw index k and, for a cached Box-Muller value,
+// the index c of the draw pair it came from (-1: none) -- lives in LDS, and
+// draws are computed from (subsequence, k) directly (counter-based Philox).
+// A physical lane p evaluates virtual lanes p, p+L, p+2L, ... in batches.
+//  - vertex phase: all 24 candidates (their supports build the face CDF);
+//  - INIT rejection: all 32 candidates (the winner is the arg-max);
+//  - non-INIT rejection: batches in lane order, stopping at the first batch
+//    that holds a candidate with enough support.  The winner is the lowest
+//    virtual lane that passes, as in the reference, and every virtual lane's
+//    stream still advances by its 3 uniforms per round.
+// The vertex propagators depend only on the disc and MAX_CURVATURE and are
+// computed once per block.
+// -------------------------------------------------------------------------
+template <int L>
+struct Grp {
+    static __device__ __forceinline__ int lane() { return threadIdx.x & (L - 1); }
+    static __device__ __forceinline__ uint32_t ballot(bool p) {
+        const unsigned long long b = __ballot(p);
+        const int base = (threadIdx.x & 63) & ~(L - 1);
+        return (uint32_t)((b >> base) & (L == 32 ? 0xffffffffull : ((1ull << L) - 1)));
+    }
+    static __device__ __forceinline__ float shfl(float v, int src) { return __shfl(v, src, L); }
+    static __device__ __forceinline__ int shfl(int v, int src) { return __shfl(v, src, L); }
+    static __device__ __forceinline__ float shfl_xor(float v, int m) { return __shfl_xor(v, m, L); }
+    static __device__ __forceinline__ int shfl_xor(int v, int m) { return __shfl_xor(v, m, L); }
+};
+
+// Philox draw k of subsequence `sub`, with a one-block cache per lane.
+struct DrawCache {
+    uint32_t sub, blk, r0, r1, r2, r3;
+    int valid;
+};
+
+__device__ __forceinline__ uint32_t philox_draw(uint32_t key0, uint32_t key1, uint32_t sub, uint32_t k,
+                                                DrawCache &dc) {
+    const uint32_t blk = k >> 2;
+    if (!(dc.valid && dc.sub == sub && dc.blk == blk)) {
+        uint32_t x0 = blk, x1 = 0, x2 = sub, x3 = 0;
+        uint32_t a = key0, b = key1;
+#pragma unroll
+        for (int r = 0; r < 10; ++r) {
+            const uint32_t lo0 = 0xD2511F53u * x0, hi0 = __umulhi(0xD2511F53u, x0);
+            const uint32_t lo1 = 0xCD9E8D57u * x2, hi1 = __umulhi(0xCD9E8D57u, x2);
+            const uint32_t n0 = hi1 ^ x1 ^ a, n2 = hi0 ^ x3 ^ b;
+            x0 = n0; x1 = lo1; x2 = n2; x3 = lo0;
+            a += 0x9E3779B9u;
+            b += 0xBB67AE85u;
+        }
+        dc.r0 = x0; dc.r1 = x1; dc.r2 = x2; dc.r3 = x3;
+        dc.sub = sub; dc.blk = blk; dc.valid = 1;
+    }
+    const int w = k & 3;
+    return (w == 0) ? dc.r0 : (w == 1) ? dc.r1 : (w == 2) ? dc.r2 : dc.r3;
+}
+
+__device__ __forceinline__ float uniform_of(uint32_t x) { return 2.32830644e-10f + (float)x * 2.32830644e-10f; }
+
+// Box-Muller pair of draws (x, y): the sine term is returned first and the
+// cosine term cached, as Philox::normal does.
+__device__ __forceinline__ void box_muller(uint32_t x, uint32_t y, float *sn_out, float *cs_out) {
+    const float u = 2.32830644e-10f + (float)x * 2.32830644e-10f;
+    const float v = 1.46291807e-09f + (float)y * 1.46291807e-09f;
+    const float s = __builtin_sqrtf(-2.0f * ulog(u));
+    float sn, cs;
+    usincos(v, &sn, &cs);
+    *sn_out = sn * s;
+    *cs_out = cs * s;
+}
+
+struct VLane {
+    uint32_t key0, key1, sub;  // stream of this virtual lane
+    uint32_t k;                // next draw index
+    int c;                     // draw pair of the cached normal, -1 if none
+    __device__ __forceinline__ float uniform(DrawCache &dc) { return uniform_of(philox_draw(key0, key1, sub, k++, dc)); }
+    __device__ __forceinline__ float normal(DrawCache &dc) {
+        float sn, cs;
+        if (c >= 0) {
+            box_muller(philox_draw(key0, key1, sub, (uint32_t)c, dc), philox_draw(key0, key1, sub, (uint32_t)c + 1, dc),
+                       &sn, &cs);
+            c = -1;
+            return cs;
+        }
+        box_muller(philox_draw(key0, key1, sub, k, dc), philox_draw(key0, key1, sub, k + 1, dc), &sn, &cs);
+        c = (int)k;
+        k += 2;
+        return sn;
+    }
+};
+
+// LDS of one PTT seed group.
+struct PttG {
+    float *frame;     // 18
+    float *vert_pdf;  // 24
+    float *face_cdf;  // 32
+    uint32_t *rk;     // 32 virtual-lane draw indices
+    int *rc;          // 32 cached-normal pair indices
+};
+constexpr int PTTG_FLOATS = 160;
+
+// Block-shared constants: 24 vertex propagators (9 each) and the scaled
+// disc vertices (MAX_CURVATURE * DISC_VERT, 48).
+struct PttConst {
+    float *vprop;  // 24 * 9
+    float *vk;     // 48
+};
+constexpr int PTTC_FLOATS = 24 * 9 + 48;
+
+__device__ void ptt_const_init(const DevParams &P, PttConst &C) {
+    const double mc = (double)P.max_curv;
+    for (int i = threadIdx.x; i < 48; i += blockDim.x) C.vk[i] = (float)(mc * (double)c_disc_vert[i]);
+    for (int i = threadIdx.x; i < 24; i += blockDim.x) {
+        const float k1 = (float)((double)c_disc_vert[2 * i] * mc);
+        const float k2 = (float)((double)c_disc_vert[2 * i + 1] * mc);
+        prepare_propagator(k1, k2, P.probe_step, C.vprop + 9 * i);
+    }
+}
+
+__device__ __forceinline__ void probe_frame_init(const float *frame, VLane &vl, DrawCache &dc, float *pf) {
+    pf[0] = frame[0]; pf[1] = frame[1]; pf[2] = frame[2];
+    norm3<0>(pf[0], pf[1], pf[2]);
+    pf[3] = vl.normal(dc);
+    pf[4] = vl.normal(dc);
+    pf[5] = vl.normal(dc);
+    const float dt = __builtin_fmaf(pf[5], pf[2], __builtin_fmaf(pf[4], pf[1], pf[3] * pf[0]));
+    pf[3] = __builtin_fmaf(-dt, pf[0], pf[3]);
+    pf[4] = __builtin_fmaf(-dt, pf[1], pf[4]);
+    pf[5] = __builtin_fmaf(-dt, pf[2], pf[5]);
+    const float n2 = __builtin_fmaf(pf[5], pf[5], __builtin_fmaf(pf[4], pf[4], pf[3] * pf[3]));
+    if (n2 < PTT_NORM_EPS) {
+        const float ax = fabsf(pf[0]), ay = fabsf(pf[1]), az = fabsf(pf[2]);
+        if (ax <= ay && ax <= az) { pf[3] = 0.0f; pf[4] = pf[2]; pf[5] = -pf[1]; }
+        else if (ay <= az) { pf[3] = -pf[2]; pf[4] = 0.0f; pf[5] = pf[0]; }
+        else { pf[3] = pf[1]; pf[4] = -pf[0]; pf[5] = 0.0f; }
+    }
+    norm3<1>(pf[3], pf[4], pf[5]);
+    crossnorm3<2>(pf + 6, pf, pf + 3);
+}
+
+// Support of one candidate with a precomputed propagator.
+__device__ __forceinline__ float support_prop(const DevParams &P, const DevBufs &B, float support, float px,
+                                              float py, float pz, const float *prop, float *pf) {
+    for (int i = 0; i < P.probe_quality; ++i) {
+        float direc[3];
+        propagate_frame(prop, pf, direc);
+        px += direc[0]; py += direc[1]; pz += direc[2];
+        support += interp4(P, B, px, py, pz, pf[0], pf[1], pf[2]);
+    }
+    return support;
+}
+
+template <int L, bool INIT>
+__device__ int get_direction_ptt_g(const DevParams &P, const DevBufs &B, const PttConst &K, PttG &S, float *frame,
+                                   uint32_t key0, uint32_t key1, uint32_t sub0, DrawCache &dc, float dx, float dy,
+                                   float dz, float px, float py, float pz, float *ndx, float *ndy, float *ndz) {
+    using G = Grp<L>;
+    const int p = G::lane();
+    if (INIT) {
+        if (p == 0) { frame[0] = dx; frame[1] = dy; frame[2] = dz; }
+        WAVE_SYNC();
+    }
+    float fr[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) fr[i] = frame[i];
+    const float first_val = interp4(P, B, px, py, pz, fr[0], fr[1], fr[2]);
+    // vertex phase: virtual lanes 0..23
+    bool found = false;
+    for (int v = p; v < DISC_VERT_CNT; v += L) {
+        float pf[9];
+        if (INIT) {
+            VLane vl{key0, key1, sub0 + (uint32_t)v, S.rk[v], S.rc[v]};
+            probe_frame_init(fr, vl, dc, pf);
+            S.rk[v] = vl.k;
+            S.rc[v] = vl.c;
+        } else {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) pf[i] = fr[i];
+        }
+        const float sup = support_prop(P, B, first_val, px, py, pz, K.vprop + 9 * v, pf);
+        const bool ok = !(sup < P.min_support_f);
+        S.vert_pdf[v] = ok ? sup : 0.0f;
+        found |= ok;
+    }
+    if (G::ballot(found) == 0u) return 0;
+    WAVE_SYNC();
+    // faces and the 32-wide Hillis-Steele scan of prefix_sum_sh_d, per lane
+    float cdf[DISC_FACE_CNT];
+#pragma unroll
+    for (int f = 0; f < DISC_FACE_CNT; ++f) {
+        float face = 0.0f;
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj) {
+            const float vv = S.vert_pdf[c_disc_face[f * 3 + jj]];
+            if (vv == 0.0f) { face = 0.0f; break; }
+            face = face + vv;
+        }
+        cdf[f] = face;
+    }
+#pragma unroll
+    for (int d = 1; d < 32; d <<= 1) {
+#pragma unroll
+        for (int f = DISC_FACE_CNT - 1; f >= d; --f) cdf[f] = cdf[f] + cdf[f - d];
+    }
+    const float last = cdf[DISC_FACE_CNT - 1];
+    if (last == 0.0f) return 0;
+    for (int round = 0; round < 1024 / 32; ++round) {
+        float best = -__builtin_inff();
+        int bestv = -1;
+        int win = -1;
+        float wk1 = 0.0f, wk2 = 0.0f, wpf[9];
+        for (int b = 0; b < 32 / L; ++b) {
+            const int v = b * L + p;
+            VLane vl{key0, key1, sub0 + (uint32_t)v, S.rk[v], INIT ? S.rc[v] : -1};
+            float r1 = vl.uniform(dc), r2 = vl.uniform(dc);
+            if (r1 + r2 > 1.0f) { r1 = 1.0f - r1; r2 = 1.0f - r2; }
+            const float tmp = vl.uniform(dc) * last;
+            int jj = DISC_FACE_CNT - 1;
+#pragma unroll
+            for (int f = DISC_FACE_CNT - 2; f >= 0; --f)
+                if (cdf[f] >= tmp) jj = f;
+            const int f0 = c_disc_face[jj * 3], f1 = c_disc_face[jj * 3 + 1], f2 = c_disc_face[jj * 3 + 2];
+            const float vx0 = K.vk[f0 * 2], vx1 = K.vk[f1 * 2], vx2 = K.vk[f2 * 2];
+            const float vy0 = K.vk[f0 * 2 + 1], vy1 = K.vk[f1 * 2 + 1], vy2 = K.vk[f2 * 2 + 1];
+            const float k1 = __builtin_fmaf(r2, vx2 - vx0, __builtin_fmaf(r1, vx1 - vx0, vx0));
+            const float k2 = __builtin_fmaf(r2, vy2 - vy0, __builtin_fmaf(r1, vy1 - vy0, vy0));
+            float pf[9];
+            if (INIT) {
+                probe_frame_init(fr, vl, dc, pf);
+                S.rk[v] = vl.k;
+                S.rc[v] = vl.c;
+            } else {
+#pragma unroll
+                for (int i = 0; i < 9; ++i) pf[i] = fr[i];
+            }
+            float prop[9];
+            prepare_propagator(k1, k2, P.probe_step, prop);
+            const float sup = support_prop(P, B, first_val, px, py, pz, prop, pf);
+            if (INIT) {
+                // batches visit this lane's virtual lanes in increasing order, so
+                // keeping the first of equal supports keeps the lowest one
+                if (bestv < 0 || sup > best) {
+                    best = sup;
+                    bestv = v;
+#pragma unroll
+                    for (int i = 0; i < 9; ++i) wpf[i] = pf[i];
+                }
+            } else {
+                const uint32_t m = G::ballot(!(sup < P.min_support_f));
+                if (m != 0u) {
+                    win = b * L + (__ffs(m) - 1);
+                    if (v == win) {
+                        wk1 = k1; wk2 = k2;
+                    }
+                    break;
+                }
+            }
+        }
+        if (INIT) {
+            // arg-max over the 32 virtual lanes, ties to the lowest virtual lane
+            float m = best;
+            int mv = bestv;
+            for (int o = L / 2; o; o >>= 1) {
+                const float om = G::shfl_xor(m, o);
+                const int ov = G::shfl_xor(mv, o);
+                if (om > m || (om == m && ov < mv)) { m = om; mv = ov; }
+            }
+            WAVE_SYNC();
+            if (!(m < P.min_support_f)) {
+                if (bestv == mv) {
+#pragma unroll
+                    for (int i = 0; i < 9; ++i) frame[i] = wpf[i];
+                }
+                *ndx = dx; *ndy = dy; *ndz = dz;
+                WAVE_SYNC();
+                return 1;
+            }
+        } else {
+            // every virtual lane drew its 3 uniforms this round
+            for (int v = p; v < 32; v += L) S.rk[v] += 3u;
+            WAVE_SYNC();
+            if (win >= 0) {
+                float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+                if ((win & (L - 1)) == p) {
+                    float prop[9], d[3], pf[9];
+                    prepare_propagator(wk1, wk2, P.ptt_sub_step, prop);
+#pragma unroll
+                    for (int i = 0; i < 9; ++i) pf[i] = fr[i];
+                    propagate_frame(prop, pf, d);
+                    norm3<0>(d[0], d[1], d[2]);
+                    ox = d[0]; oy = d[1]; oz = d[2];
+#pragma unroll
+                    for (int i = 0; i < 9; ++i) frame[i] = pf[i];
+                }
+                const int wl = win & (L - 1);
+                *ndx = G::shfl(ox, wl);
+                *ndy = G::shfl(oy, wl);
+                *ndz = G::shfl(oz, wl);
+                WAVE_SYNC();
+                return 1;
+            }
+        }
+    }
+    return 0;
+}
+
+template <int L>
+__device__ bool init_frame_ptt_g(const DevParams &P, const DevBufs &B, const PttConst &K, PttG &S, uint32_t key0,
+                                 uint32_t key1, uint32_t sub0, DrawCache &dc, float fx, float fy, float fz, float sx,
+                                 float sy, float sz) {
+    const int p = Grp<L>::lane();
+    float tx, ty, tz;
+    int att = 0;
+    bool ok = false;
+    for (; att < 2; ++att) {
+        const float sg = att ? -1.0f : 1.0f;
+        ok = get_direction_ptt_g<L, true>(P, B, K, S, S.frame, key0, key1, sub0, dc, sg * fx, sg * fy, sg * fz, sx, sy,
+                                          sz, &tx, &ty, &tz);
+        if (ok) break;
+    }
+    if (!ok) return false;
+    if (p == 0) {
+        if (att == 0)
+            for (int i = 0; i < 6; ++i) S.frame[i] = -S.frame[i];
+        for (int i = 0; i < 6; ++i) S.frame[9 + i] = -S.frame[i];
+        for (int i = 0; i < 3; ++i) S.frame[15 + i] = S.frame[6 + i];
+    }
+    WAVE_SYNC();
+    return true;
+}
+
+template <int L>
+__device__ int tracker_ptt_g(const DevParams &P, const DevBufs &B, const PttConst &K, PttG &S, float *frame,
+                             uint32_t key0, uint32_t key1, uint32_t sub0, DrawCache &dc, float sx, float sy, float sz,
+                             float dx, float dy, float dz, float *__restrict__ sl) {
+    const int p = Grp<L>::lane();
+    const int step_frac = P.step_frac;
+    const int MAXL = P.max_sline_len;
+    const double sub = P.step_d / (double)step_frac;
+    float px = sx, py = sy, pz = sz;
+    if (p == 0) { sl[0] = px; sl[1] = py; sl[2] = pz; }
+    int i;
+    for (i = 1; i < MAXL * step_frac; ++i) {
+        float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+        const int ndir = get_direction_ptt_g<L, false>(P, B, K, S, frame, key0, key1, sub0, dc, dx, dy, dz, px, py, pz,
+                                                       &nx, &ny, &nz);
+        if (ndir == 0) break;
+        dx = nx; dy = ny; dz = nz;
+        px = (float)__builtin_fma((double)(dx / 1.0f), sub, (double)px);
+        py = (float)__builtin_fma((double)(dy / 1.0f), sub, (double)py);
+        pz = (float)__builtin_fma((double)(dz / 1.0f), sub, (double)pz);
+        if (p == 0 && (i % step_frac) == 0) {
+            float *o = sl + 3 * (size_t)(i / step_frac);
+            o[0] = px; o[1] = py; o[2] = pz;
+        }
+        const int tissue = check_point(P, B.stop, px, py, pz);
+        if (tissue == CUSLINES_ENDPOINT || tissue == CUSLINES_INVALIDPOINT || tissue == CUSLINES_OUTSIDEIMAGE) break;
+    }
+    int nsteps = i / step_frac;
+    if ((i % step_frac) != 0 && i < step_frac * (MAXL - 1)) {
+        ++nsteps;
+        if (p == 0) {
+            float *o = sl + 3 * (size_t)nsteps;
+            o[0] = px; o[1] = py; o[2] = pz;
+        }
+    }
+    return nsteps;
+}
+
+template <int L>
+__device__ void track_seed_ptt_g(const DevParams &P, const DevBufs &B, const PttConst &K, PttG &S, int s,
+                                 const float *__restrict__ seeds, int rng_chunk, const int *__restrict__ offsets,
+                                 const int *__restrict__ dirs, int *__restrict__ sline_seed, int *__restrict__ sline_len,
+                                 float *__restrict__ sline) {
+    const int p = Grp<L>::lane();
+    const float sx = seeds[3 * s], sy = seeds[3 * s + 1], sz = seeds[3 * s + 2];
+    const uint32_t key0 = P.seed_lo, key1 = P.seed_hi;
+    const uint32_t sub0 = 32u * (uint32_t)(s % rng_chunk) + 1u;
+    for (int v = p; v < 32; v += L) { S.rk[v] = 0u; S.rc[v] = -1; }
+    WAVE_SYNC();
+    DrawCache dc;
+    dc.valid = 0;
+    const int ndir = offsets[s + 1] - offsets[s];
+    int off = offsets[s];
+    const size_t slot_f = (size_t)2 * P.max_sline_len * 3;
+    for (int i = 0; i < ndir; ++i, ++off) {
+        const int vi = dirs[(size_t)s * P.nverts + i];
+        const float fx = B.verts[3 * vi], fy = B.verts[3 * vi + 1], fz = B.verts[3 * vi + 2];
+        float *cur = sline + (size_t)off * slot_f;
+        if (p == 0) sline_seed[off] = s;
+        if (!init_frame_ptt_g<L>(P, B, K, S, key0, key1, sub0, dc, fx, fy, fz, sx, sy, sz)) {
+            if (p == 0) {
+                sline_len[off] = 1;
+                cur[0] = sx; cur[1] = sy; cur[2] = sz;
+            }
+            continue;
+        }
+        int nb = 0, nf = 0;
+        for (int pass = 0; pass < 2; ++pass) {
+            const float sg = pass ? 1.0f : -1.0f;
+            float *out = pass ? cur + 3 * (size_t)(nb - 1) : cur;
+            const int ns = tracker_ptt_g<L>(P, B, K, S, S.frame + 9 * pass, key0, key1, sub0, dc, sx, sy, sz, sg * fx,
+                                            sg * fy, sg * fz, out);
+            if (pass == 0) {
+                nb = ns;
+                MEM_SYNC();
+                for (int j = p; j < nb / 2; j += L) {
+                    float *a = cur + 3 * (size_t)j, *b = cur + 3 * (size_t)(nb - 1 - j);
+                    const float a0 = a[0], a1 = a[1], a2 = a[2];
+                    a[0] = b[0]; a[1] = b[1]; a[2] = b[2];
+                    b[0] = a0; b[1] = a1; b[2] = a2;
+                }
+                MEM_SYNC();
+            } else {
+                nf = ns;
+            }
+        }
+        if (p == 0) sline_len[off] = nb - 1 + nf;
+    }
+}
+
+// Persistent PTT phase 2 with L lanes per seed.
+template <int L>
+__global__ void __launch_bounds__(256) phase2_ptt_queue_kernel(DevParams P, DevBufs B, const float *__restrict__ seeds,
+                                                               int nseeds, int rng_chunk,
+                                                               const int *__restrict__ offsets,
+                                                               const int *__restrict__ dirs,
+                                                               int *__restrict__ sline_seed,
+                                                               int *__restrict__ sline_len, float *__restrict__ sline,
+                                                               int *__restrict__ work) {
+    extern __shared__ float smem[];
+    PttConst K;
+    K.vprop = smem;
+    K.vk = smem + 24 * 9;
+    ptt_const_init(P, K);
+    __syncthreads();
+    const int g = threadIdx.x / L;
+    float *base = smem + PTTC_FLOATS + g * PTTG_FLOATS;
+    PttG S;
+    S.frame = base;
+    S.vert_pdf = base + 18;
+    S.face_cdf = base + 48;
+    S.rk = reinterpret_cast<uint32_t *>(base + 80);
+    S.rc = reinterpret_cast<int *>(base + 112);
+    const int p = Grp<L>::lane();
+    for (;;) {
+        int s = 0;
+        if (p == 0) s = atomicAdd(work, 1);
+        s = Grp<L>::shfl(s, 0);
+        if (s >= nseeds) break;
+        track_seed_ptt_g<L>(P, B, K, S, s, seeds, rng_chunk, offsets, dirs, sline_seed, sline_len, sline);
+    }
+}
+
+// -------------------------------------------------------------------------
 // Kernels
 // -------------------------------------------------------------------------
 
-// Orders memory traffic of one wave around cross-lane reuse of global data
-// (points written by lane 0 and read back by every lane of the slot).
-#define MEM_SYNC()                                             \
-    do {                                                       \
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); \
-        __builtin_amdgcn_wave_barrier();                       \
-    } while (0)
 
 __host__ __device__ inline int boot_region_a(int nv32, int t32, int hr32) {
     return (nv32 > t32 + hr32) ? nv32 : t32 + hr32;
@@ -1256,6 +1706,7 @@ struct cuslines_tracker {
     int threads = 0;        // 0: per-getter default
     int blocks_per_cu = 0;  // 0: occupancy maximum
     int num_cu = 1;
+    int ptt_lanes = 0;      // PTT: lanes per seed (8, 16, 32; 0 = shared 32-lane kernel)
     bool has_data = false, has_stop = false, has_sphere = false, has_boot = false, has_lut = false;
 };
 
@@ -1310,6 +1761,38 @@ static int launch_phase1(cuslines_tracker *t, const float *seeds, int n, int chu
     return CUSLINES_OK;
 }
 
+template <int L>
+static int launch_ptt_l(cuslines_tracker *t, int threads, const float *seeds, int n, int chunk, const int *offsets,
+                        const int *dirs, int *sseed, int *slen, float *sl) {
+    const DevParams &d = t->dp;
+    const size_t lds = sizeof(float) * (size_t)(PTTC_FLOATS + (threads / L) * PTTG_FLOATS);
+    const void *kfn = (const void *)phase2_ptt_queue_kernel<L>;
+    HIPC(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = 0;
+    HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, threads, lds));
+    if (per_cu < 1) per_cu = 1;
+    if (t->blocks_per_cu > 0 && t->blocks_per_cu < per_cu) per_cu = t->blocks_per_cu;
+    long long grid = (long long)per_cu * t->num_cu;
+    const long long need = ((long long)n * L + threads - 1) / threads;
+    if (grid > need) grid = need;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(zero_int, dim3(1), dim3(1), 0, t->stream, t->d_work);
+    hipLaunchKernelGGL(phase2_ptt_queue_kernel<L>, dim3((unsigned)grid), dim3(threads), lds, t->stream, d, bufs_of(t),
+                       seeds, n, chunk, offsets, dirs, sseed, slen, sl, t->d_work);
+    HIPC(hipGetLastError());
+    return CUSLINES_OK;
+}
+
+static int launch_ptt_groups(cuslines_tracker *t, const float *seeds, int n, int chunk, const int *offsets,
+                             const int *dirs, int *sseed, int *slen, float *sl) {
+    const int threads = t->threads > 0 ? t->threads : 256;
+    switch (t->ptt_lanes) {
+        case 8: return launch_ptt_l<8>(t, threads, seeds, n, chunk, offsets, dirs, sseed, slen, sl);
+        case 16: return launch_ptt_l<16>(t, threads, seeds, n, chunk, offsets, dirs, sseed, slen, sl);
+        default: return launch_ptt_l<32>(t, threads, seeds, n, chunk, offsets, dirs, sseed, slen, sl);
+    }
+}
+
 template <int MODEL>
 static int launch_phase2(cuslines_tracker *t, const float *seeds, int n, int chunk, const int *offsets,
                          const int *dirs, int *sseed, int *slen, float *sl) {
@@ -1329,6 +1812,7 @@ static int launch_phase2(cuslines_tracker *t, const float *seeds, int n, int chu
         HIPC(hipGetLastError());
         return CUSLINES_OK;
     }
+    if (MODEL == CUSLINES_PTT && t->ptt_lanes > 0) return launch_ptt_groups(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl);
     const void *kfn = (const void *)phase2_queue_kernel<MODEL>;
     HIPC(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int per_cu = 0;
@@ -1658,6 +2142,14 @@ int cuslines_set_launch(cuslines_tracker *t, int schedule, int threads, int bloc
     t->schedule = schedule;
     t->threads = threads;
     t->blocks_per_cu = blocks_per_cu;
+    return CUSLINES_OK;
+}
+
+int cuslines_set_lanes_per_seed(cuslines_tracker *t, int lanes) {
+    if (!t) return fail(CUSLINES_EINVAL, "null tracker");
+    if (lanes != 0 && lanes != 8 && lanes != 16 && lanes != 32)
+        return fail(CUSLINES_EINVAL, "lanes per seed must be 0, 8, 16 or 32");
+    t->ptt_lanes = lanes;
     return CUSLINES_OK;
 }
 

@@ -17,6 +17,14 @@
         __builtin_amdgcn_wave_barrier();                       \
     } while (0)
 
+// Orders memory traffic of one wave around cross-lane reuse of global data
+// (points written by one lane and read back by the other lanes of a slot).
+#define MEM_SYNC()                                             \
+    do {                                                       \
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); \
+        __builtin_amdgcn_wave_barrier();                       \
+    } while (0)
+
 namespace cusl {
 
 __device__ __forceinline__ int lane32() { return threadIdx.x & 31; }

@@ -143,6 +143,13 @@ enum {
  * is fixed at 2 seeds per 64-thread block (cu_propagate_seeds.py:45-54). */
 int cuslines_set_launch(cuslines_tracker *t, int schedule, int threads, int blocks_per_cu);
 
+/* PTT phase 2 with `lanes` (8, 16 or 32) lanes per seed, 64/lanes seeds per
+ * wavefront.  The reference's 32 per-lane random streams per seed are kept as
+ * counter-based virtual lanes, so the output does not depend on this choice.
+ * 0 (default) selects the one-seed-per-32-lane kernel shared with the other
+ * getters (which always use 32 lanes per seed). */
+int cuslines_set_lanes_per_seed(cuslines_tracker *t, int lanes);
+
 /* PTT probe taps: 1 (default) reads each tap's 8 texels as one 32-byte
  * corner block, an expanded copy of the ODF built by cuslines_set_data when
  * HBM allows (nverts x (X+1)(Y+1)(Z+1) x 32 B); 0 gathers the 8 texels from

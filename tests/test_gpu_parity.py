@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 NSEEDS = {"prob": 256, "ptt": 48, "csa": 128, "opdt": 128}
 
 
-def _run_both(w, seeds, rng_chunk=0, launch=(0, 0, 0), blocks=True):
+def _run_both(w, seeds, rng_chunk=0, launch=(0, 0, 0), blocks=True, lanes=0):
     from gpustreamlines_amd._device import DeviceTracker
     from oracle.oracle import Oracle
 
@@ -23,6 +23,7 @@ def _run_both(w, seeds, rng_chunk=0, launch=(0, 0, 0), blocks=True):
     try:
         dt.set_launch(*launch)
         dt.set_ptt_layout(blocks)
+        dt.set_lanes_per_seed(lanes)
         dt.upload(w.dataf, w.stop_map, w.vertices, w.edges, boot=w.boot, ptt_lut=w.ptt_lut)
         g = dt.track(seeds, rng_chunk=rng_chunk, keep_dirs=True)
     finally:
@@ -75,6 +76,18 @@ def test_ptt_texel_gather_path_bit_exact(small_workloads):
     w = small_workloads["ptt"]
     seeds = W.seeds_for(w, 32, placement="mixed", seed=12)
     g, o = _run_both(w, seeds, blocks=False)
+    _assert_same(g, o)
+
+
+@pytest.mark.parametrize("lanes", [8, 16, 32])
+def test_ptt_lanes_per_seed_bit_exact(small_workloads, lanes):
+    """PTT with 8, 16 or 32 lanes per seed (virtual random streams) equals the
+    shared 32-lane kernel and the oracle."""
+    from gpustreamlines_amd import workloads as W
+
+    w = small_workloads["ptt"]
+    seeds = W.seeds_for(w, 40, placement="mixed", seed=13)
+    g, o = _run_both(w, seeds, lanes=lanes)
     _assert_same(g, o)
 
 
