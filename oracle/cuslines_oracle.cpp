@@ -537,40 +537,51 @@ int closest_peak(const Problem &P, const float *dir, int npeaks, const int *peak
     return 0;
 }
 
-int get_direction_boot(const Problem &P, SeedScratch &S, int nattempts, const float *dir,
-                       const float *pt, float *new_dir, int *peaks_out) {
+// One bootstrap refit (boot.cu:447-545): trilinear DWI gather, residual
+// bootstrap h = H d + (R d)[perm], min_signal clip, b0 normalisation, model
+// fit (OPDT or CSA) and ODF on the sphere into odf[nverts] (all zero outside
+// the volume).  The normalised bootstrap signal is left in S.a[0..dimt).
+int boot_odf(const Problem &P, SeedScratch &S, const float *pt, float *odf) {
     const int hr = P.hr_side;
     const int dn = P.p.delta_nr;
     const int nv = P.p.nverts;
-    float *vox = S.a.data(), *msk = S.b.data(), *r = S.c.data(), *h = S.d.data();
+    float *vox = S.a.data(), *msk = S.b.data(), *r = S.c.data();
+    const int rv = trilinear(P, P.dataf.data(), P.p.dimt, pt, vox);
+    mask_get(P, vox, msk);
+    if (rv != 0) {
+        for (int j = 0; j < nv; ++j) odf[j] = 0.0f;
+        return rv;
+    }
+    ndotp(hr, hr, msk, P.R.data(), r);
+    ndotp(hr, hr, msk, P.H.data(), odf);
+    for (int j = 0; j < hr; ++j) {
+        const int src = (int)(S.rng[j % LANES].next() % (uint32_t)hr);
+        odf[j] = odf[j] + r[src];
+    }
+    mask_put(P, odf, vox);
+    for (int j = 0; j < P.p.dimt; ++j) vox[j] = (P.p.min_signal > vox[j]) ? P.p.min_signal : vox[j];
+    const float denom = avg_mask(P, vox);
+    for (int j = 0; j < P.p.dimt; ++j) vox[j] = vox[j] / denom;
+    mask_get(P, vox, msk);
+    if (P.p.model_type == CUSLINES_OPDT) {
+        ndotp_log_opdt(dn, hr, msk, P.delta_q.data(), r);
+        ndotp(dn, hr, msk, P.delta_b.data(), odf);
+        for (int j = 0; j < dn; ++j) r[j] = r[j] - odf[j];
+    } else {
+        ndotp_log_csa(dn, hr, msk, P.delta_q.data(), r);
+        r[0] = CSA_C0;
+    }
+    ndotp(nv, dn, r, P.smat.data(), odf);
+    return 0;
+}
+
+int get_direction_boot(const Problem &P, SeedScratch &S, int nattempts, const float *dir,
+                       const float *pt, float *new_dir, int *peaks_out) {
+    const int nv = P.p.nverts;
+    float *h = S.d.data();
     int *peaks = peaks_out ? peaks_out : S.peaks.data();
     for (int att = 0; att < nattempts; ++att) {
-        const int rv = trilinear(P, P.dataf.data(), P.p.dimt, pt, vox);
-        mask_get(P, vox, msk);
-        if (rv == 0) {
-            ndotp(hr, hr, msk, P.R.data(), r);
-            ndotp(hr, hr, msk, P.H.data(), h);
-            for (int j = 0; j < hr; ++j) {
-                const int src = (int)(S.rng[j % LANES].next() % (uint32_t)hr);
-                h[j] = h[j] + r[src];
-            }
-            mask_put(P, h, vox);
-            for (int j = 0; j < P.p.dimt; ++j) vox[j] = (P.p.min_signal > vox[j]) ? P.p.min_signal : vox[j];
-            const float denom = avg_mask(P, vox);
-            for (int j = 0; j < P.p.dimt; ++j) vox[j] = vox[j] / denom;
-            mask_get(P, vox, msk);
-            if (P.p.model_type == CUSLINES_OPDT) {
-                ndotp_log_opdt(dn, hr, msk, P.delta_q.data(), r);
-                ndotp(dn, hr, msk, P.delta_b.data(), h);
-                for (int j = 0; j < dn; ++j) r[j] = r[j] - h[j];
-            } else {
-                ndotp_log_csa(dn, hr, msk, P.delta_q.data(), r);
-                r[0] = CSA_C0;
-            }
-            ndotp(nv, dn, r, P.smat.data(), h);
-        } else {
-            for (int j = 0; j < nv; ++j) h[j] = 0.0f;
-        }
+        boot_odf(P, S, pt, h);
         float mx = REAL_MIN;
         for (int j = 0; j < nv; ++j) mx = (mx > h[j]) ? mx : h[j];
         const float thr = (float)(PMF_THRESHOLD_P * (double)mx);
@@ -1081,6 +1092,20 @@ int oracle_check_point(oracle_problem *h, const float *pt) {
 }
 float oracle_interp4(oracle_problem *h, const float *pos, const float *frame) {
     return interp4(*reinterpret_cast<Problem *>(h), pos, frame);
+}
+// One bootstrap refit at pt with the random streams of seed slot `slot` in
+// phase `phase` (0: getNumStreamlines, 1: genStreamlines): the ODF before
+// thresholding into odf[nverts] and the normalised bootstrap signal into
+// signal[dimt].  Returns the trilinear status (0 inside the volume).
+int oracle_boot_odf(oracle_problem *h, const float *pt, int slot, int phase, float *odf, float *signal) {
+    const Problem &P = *reinterpret_cast<Problem *>(h);
+    SeedScratch S;
+    S.size(P);
+    init_lane_rngs(P, S, slot, phase);
+    const int rv = boot_odf(P, S, pt, S.d.data());
+    for (int j = 0; j < P.p.nverts; ++j) odf[j] = S.d[j];
+    for (int j = 0; j < P.p.dimt; ++j) signal[j] = S.a[j];
+    return rv;
 }
 void oracle_prepare_propagator(float k1, float k2, float arclen, float *p) {
     prepare_propagator(k1, k2, arclen, p);

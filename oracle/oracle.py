@@ -47,6 +47,7 @@ def lib():
             ("oracle_peaks", [vp, fp, ip]),
             ("oracle_trilinear", [vp, fp, fp]),
             ("oracle_check_point", [vp, fp]),
+            ("oracle_boot_odf", [vp, fp, C.c_int, C.c_int, fp, fp]),
         ]:
             getattr(L, name).restype = C.c_int
             getattr(L, name).argtypes = args
@@ -158,6 +159,14 @@ class Oracle:
     def check_point(self, pt):
         pt = np.ascontiguousarray(pt, dtype=np.float32)
         return self.L.oracle_check_point(self.h, _f(pt))
+
+    def boot_odf(self, pt, slot=0, phase=0):
+        """One bootstrap refit at pt: (status, odf[nverts], signal[dimt])."""
+        pt = np.ascontiguousarray(pt, dtype=np.float32)
+        odf = np.zeros(self.params["nverts"], dtype=np.float32)
+        sig = np.zeros(self.params["dimt"], dtype=np.float32)
+        rv = self.L.oracle_boot_odf(self.h, _f(pt), int(slot), int(phase), _f(odf), _f(sig))
+        return rv, odf, sig
 
     def interp4(self, pos, frame):
         pos = np.ascontiguousarray(pos, dtype=np.float32)
