@@ -42,6 +42,25 @@ def alg_bytes_per_step(kind: str, p: dict) -> float:
     return 8 * p["dimt"] * 4 + stop_tap + 12
 
 
+MODEL_IDS = {"opdt": 0, "csa": 1, "prob": 2, "ptt": 3}
+CPU_SAMPLE = {"ptt": 2000, "prob": 150000, "csa": 4000, "opdt": 4000}
+
+
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summaries (profiles/*_traffic.json,
+    copy-calibrated), or None when no summary covers this workload."""
+    import glob
+
+    for f in sorted(glob.glob(os.path.join(HERE, "profiles", "*_traffic.json"))):
+        with open(f) as fh:
+            t = json.load(fh)
+        if t.get("workload") == workload and t.get("kernel") == kernel:
+            t["source"] = os.path.relpath(f, HERE)
+            return t
+    return None
+
+
 class HipEvents:
     """hipEvent timing on an explicit stream (the library's own)."""
 
@@ -217,15 +236,20 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": f"phase2_kernel<{args.getter}>",
+            "kernel": f"phase2_queue_kernel<{MODEL_IDS[args.getter]}>",
             "alg_bytes_per_step": abytes,
+            "alg_bytes_per_launch": abytes * steps_per_pass,
         },
     }
+    traffic = measured_traffic(res["config"]["workload"], res["roofline"]["kernel"])
+    if traffic is not None:
+        res["roofline"]["traffic"] = traffic["traffic_bytes"]
+        res["roofline"]["traffic_source"] = traffic["source"]
     for b in (sd, cd, dd, od, ssd, sld, sl):
         b.free()
     dt.close()
     if not args.no_cpu:
-        ns = args.cpu_sample or {"ptt": 400, "prob": 4000, "csa": 1000, "opdt": 1000}[args.getter]
+        ns = args.cpu_sample or CPU_SAMPLE[args.getter]
         print(f"[bench] cpu baseline on {ns} seeds", file=sys.stderr, flush=True)
         res["cpu_baseline"] = cpu_baseline(w, seeds, ns)
     else:

@@ -36,7 +36,7 @@ constexpr float REAL_MAX = 3.40282346638528859812e+38f;
 constexpr float REAL_MIN = -REAL_MAX;                  // globals.h:41-42
 constexpr float K_SMALL = 0.0001f;                     // ptt.cuh:10
 constexpr float PTT_NORM_EPS = 1e-8f;                  // ptt.cuh:12
-constexpr float PTT_INV_NORM_EPS = 1e8f;               // ptt.cuh:13
+constexpr float PTT_MIN_NORM2 = 1e-16f;                // (1/PTT_INV_NORM_EPS)^2, ptt.cuh:13
 constexpr float CSA_C0 = 0.28209479177387814f;         // boot.cu:363
 constexpr int DISC_VERT_CNT = 24;                      // disc.h:20
 constexpr int DISC_FACE_CNT = 31;                      // disc.h:21
@@ -600,10 +600,13 @@ __device__ int get_direction_boot(const DevParams &P, const DevBufs &B, const fl
 // -------------------------------------------------------------------------
 // PTT getter (ptt.cu)
 // -------------------------------------------------------------------------
+// norm3_d (ptt.cu:1-13): scale to unit length when 1/|v| is finite, positive
+// and below 1e8 (|v|^2 > 1e-16), else the FAIL axis.
 template <int FAIL>
 __device__ __forceinline__ void norm3(float &x, float &y, float &z) {
-    const float inv = 1.0f / __builtin_sqrtf(__builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x)));
-    if (__builtin_isfinite(inv) && inv > 0.0f && inv < PTT_INV_NORM_EPS) {
+    const float n2 = __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x));
+    if (n2 > PTT_MIN_NORM2 && n2 <= 3.40282347e+38f) {
+        const float inv = rsqrt_nr(n2);
         x *= inv; y *= inv; z *= inv;
     } else {
         x = (FAIL == 0) ? 1.0f : 0.0f;

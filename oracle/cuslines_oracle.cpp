@@ -54,7 +54,7 @@ constexpr float REAL_MAX = 3.40282346638528859812e+38f;
 constexpr float REAL_MIN = -REAL_MAX;     // globals.h:41-42
 constexpr float K_SMALL = 0.0001f;        // ptt.cuh:10
 constexpr float PTT_NORM_EPS = 1e-8f;     // ptt.cuh:12
-constexpr float PTT_INV_NORM_EPS = 1e8f;  // ptt.cuh:13
+constexpr float PTT_MIN_NORM2 = 1e-16f;   // (1/PTT_INV_NORM_EPS)^2, ptt.cuh:13
 constexpr float CSA_C0 = 0.28209479177387814f;  // boot.cu:363
 
 // ---------------------------------------------------------------------------
@@ -596,10 +596,30 @@ int get_direction_boot(const Problem &P, SeedScratch &S, int nattempts, const fl
 // ---------------------------------------------------------------------------
 // PTT getter (ptt.cu)
 // ---------------------------------------------------------------------------
+// 1/sqrt(n2) for finite n2 > 1e-16: Lomont's initial approximation
+// (0x5f375a86 - bits/2, "Fast Inverse Square Root", 2003) refined by three
+// Newton-Raphson steps y <- y (1.5 - (n2/2) y^2).  It stands in for
+// rnorm3df, whose implementation is not restated.
+float rsqrt_nr(float n2) {
+    uint32_t b;
+    std::memcpy(&b, &n2, 4);
+    b = 0x5f375a86u - (b >> 1);
+    float y;
+    std::memcpy(&y, &b, 4);
+    const float h = 0.5f * n2;
+    y = y * std::fmaf(-(h * y), y, 1.5f);
+    y = y * std::fmaf(-(h * y), y, 1.5f);
+    y = y * std::fmaf(-(h * y), y, 1.5f);
+    return y;
+}
+
+// norm3_d (ptt.cu:1-13): unit length when 1/|v| is finite, positive and
+// below PTT_INV_NORM_EPS = 1e8, i.e. |v|^2 > 1e-16; else the FAIL axis.
 template <int FAIL>
 void norm3(float *v) {
-    const float inv = 1.0f / std::sqrt(std::fmaf(v[2], v[2], std::fmaf(v[1], v[1], v[0] * v[0])));
-    if (std::isfinite(inv) && inv > 0.0f && inv < PTT_INV_NORM_EPS) {
+    const float n2 = std::fmaf(v[2], v[2], std::fmaf(v[1], v[1], v[0] * v[0]));
+    if (n2 > PTT_MIN_NORM2 && n2 <= REAL_MAX) {
+        const float inv = rsqrt_nr(n2);
         v[0] *= inv; v[1] *= inv; v[2] *= inv;
     } else {
         v[0] = v[1] = v[2] = 0.0f;
@@ -1070,6 +1090,7 @@ void oracle_philox_mixed(uint64_t seed, uint64_t subseq, int n, const int *kinds
 }
 float oracle_log(float x) { return ulog(x); }
 void oracle_sincos(float x, float *s, float *c) { usincos(x, s, c); }
+float oracle_rsqrt(float x) { return rsqrt_nr(x); }
 int oracle_sortnet_partner(int layer, int lane) {
     const SortNet &s = sortnet();
     if (layer < 0 || layer >= s.nlayers || lane < 0 || lane >= 32) return -1;

@@ -56,6 +56,8 @@ def lib():
         L.oracle_log.restype = C.c_float
         L.oracle_log.argtypes = [C.c_float]
         L.oracle_sincos.argtypes = [C.c_float, fp, fp]
+        L.oracle_rsqrt.restype = C.c_float
+        L.oracle_rsqrt.argtypes = [C.c_float]
         L.oracle_sortnet_partner.restype = C.c_int
         L.oracle_sortnet_partner.argtypes = [C.c_int, C.c_int]
         L.oracle_sortnet_layers.restype = C.c_int
@@ -195,6 +197,10 @@ def usincos(x):
     s, c = C.c_float(), C.c_float()
     lib().oracle_sincos(float(x), C.byref(s), C.byref(c))
     return s.value, c.value
+
+
+def rsqrt(x):
+    return float(lib().oracle_rsqrt(float(x)))
 
 
 def sortnet_table():
