@@ -109,6 +109,45 @@ class DeviceTracker:
         """PTT taps from 32-byte corner blocks (True) or 8 texel gathers."""
         _lib.check(self.L.cuslines_set_ptt_layout(self.h, int(bool(corner_blocks))), "set_ptt_layout")
 
+    # -- learned PTT proposal -------------------------------------------------
+    def set_sh_coeffs(self, sh):
+        sh = np.ascontiguousarray(sh, dtype=np.float32)
+        _lib.check(self.L.cuslines_set_sh_coeffs(self.h, sh.ctypes.data, int(sh.shape[-1])), "set_sh_coeffs")
+
+    def set_proposal(self, mlp_bf16):
+        """bf16 parameters (uint16 array) of the proposal network, or None."""
+        if mlp_bf16 is None:
+            _lib.check(self.L.cuslines_set_proposal(self.h, None), "set_proposal")
+            return
+        w = np.ascontiguousarray(mlp_bf16, dtype=np.uint16)
+        if w.size != _lib.MLP_NPARAM:
+            raise ValueError(f"expected {_lib.MLP_NPARAM} parameters, got {w.size}")
+        _lib.check(self.L.cuslines_set_proposal(self.h, w.ctypes.data), "set_proposal")
+
+    def set_recording(self, pairs_d=None, capacity=0, count_d=None, rate=1.0):
+        _lib.check(self.L.cuslines_set_recording(self.h, C.c_void_p(pairs_d), int(capacity), C.c_void_p(count_d),
+                                                 float(rate)), "set_recording")
+
+    def proposal_stats(self, reset: bool = True) -> np.ndarray:
+        out = (C.c_int64 * 3)()
+        _lib.check(self.L.cuslines_proposal_stats(self.h, out, int(reset)), "proposal_stats")
+        return np.array(list(out), dtype=np.int64)
+
+    def mlp_workspace_bytes(self, n: int) -> int:
+        b = C.c_size_t(0)
+        _lib.check(self.L.cuslines_mlp_workspace_bytes(int(n), C.byref(b)), "mlp_workspace_bytes")
+        return b.value
+
+    def mlp_loss_grad_async(self, params_d, pairs_d, n, loss_d, grad_d, work_d):
+        _lib.check(self.L.cuslines_mlp_loss_grad_async(self.h, C.c_void_p(params_d), C.c_void_p(pairs_d), int(n),
+                                                       C.c_void_p(loss_d), C.c_void_p(grad_d), C.c_void_p(work_d)),
+                   "mlp_loss_grad")
+
+    def mlp_adam_async(self, params_d, grad_d, m_d, v_d, step, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        _lib.check(self.L.cuslines_mlp_adam_async(self.h, C.c_void_p(params_d), C.c_void_p(grad_d), C.c_void_p(m_d),
+                                                  C.c_void_p(v_d), int(step), float(lr), float(beta1), float(beta2),
+                                                  float(eps)), "mlp_adam")
+
     def stream(self) -> int:
         s = C.c_void_p()
         _lib.check(self.L.cuslines_get_stream(self.h, C.byref(s)), "get_stream")

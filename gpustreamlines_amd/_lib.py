@@ -15,6 +15,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("CUSLINES_HIP_LIB") or os.path.join(HERE, "libcuslines_hip.so")
 
 ABI_VERSION = 1
+MLP_NPARAM = 26920  # CUSLINES_MLP_NPARAM
+PAIR_FLOATS = 44    # CUSLINES_PAIR_FLOATS
+SH_COEFFS = 28      # CUSLINES_SH_COEFFS
 
 
 class cuslines_params(C.Structure):
@@ -97,6 +100,13 @@ def load():
         "cuslines_set_launch": [vp, C.c_int, C.c_int, C.c_int],
         "cuslines_set_ptt_layout": [vp, C.c_int],
         "cuslines_set_lanes_per_seed": [vp, C.c_int],
+        "cuslines_set_sh_coeffs": [vp, vp, i32],
+        "cuslines_set_proposal": [vp, vp],
+        "cuslines_set_recording": [vp, vp, i32, vp, C.c_float],
+        "cuslines_proposal_stats": [vp, C.POINTER(C.c_int64), C.c_int],
+        "cuslines_mlp_workspace_bytes": [i32, C.POINTER(sz)],
+        "cuslines_mlp_loss_grad_async": [vp, vp, vp, i32, vp, vp, vp],
+        "cuslines_mlp_adam_async": [vp, vp, vp, vp, vp, i32, C.c_float, C.c_float, C.c_float, C.c_float],
         "cuslines_phase1_async": [vp, vp, i32, i32, vp, vp],
         "cuslines_scan_async": [vp, vp, i32, vp],
         "cuslines_phase2_async": [vp, vp, i32, i32, vp, vp, vp, vp, vp],

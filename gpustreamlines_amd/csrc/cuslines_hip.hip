@@ -101,6 +101,8 @@ struct DevParams {
     int step_frac, probe_quality, lut_res, nx, ny, nz, log2x, log2y;
     float probe_step, max_curv, min_support_f, ptt_seed_thr_f;
     float ptt_sub_step;  // (float)(STEP_SIZE / STEP_FRAC)
+    uint32_t rec_thresh;  // record a sub-step when its hash is below this
+    int rec_cap;          // capacity of the pair buffer
 };
 
 struct DevBufs {
@@ -113,6 +115,11 @@ struct DevBufs {
     const int *dwi_idx;
     const uint16_t *lut;
     const float *blocks;  // PTT corner blocks (nullptr: gather 8 texels)
+    const float *sh;       // SH coefficients of the ODF, [X][Y][Z][32] (28 used)
+    const uint16_t *mlp;   // proposal network, bf16 (MLP_NPARAM)
+    float *rec;            // training pairs [rec_cap][PAIR_FLOATS]
+    int *rec_count;
+    unsigned long long *stats;  // learned sub-steps, fallbacks, plain sub-steps
 };
 
 // -------------------------------------------------------------------------
@@ -704,13 +711,15 @@ struct PttLds {
     float *frame;     // 18 floats: [0..8] backward frame, [9..17] forward frame
     float *vert_pdf;  // 32
     float *face_cdf;  // 32
+    float *mix;       // learned proposal: 8 components x MIX_STRIDE
+    float *act;       // learned proposal: inputs [64], hidden 1 [128], hidden 2 [128], outputs [64]
 };
 
 // get_direction_ptt_d (ptt.cu:194-412); `frame` is the 9-float frame in use.
 template <bool INIT>
 __device__ int get_direction_ptt(const DevParams &P, const DevBufs &B, PttLds &S, float *frame,
                                  Philox &rng, float dx, float dy, float dz, float px, float py, float pz,
-                                 float *ndx, float *ndy, float *ndz) {
+                                 float *ndx, float *ndy, float *ndz, float *wk = nullptr) {
     const int l = lane32();
     if (INIT) {
         if (l == 0) { frame[0] = dx; frame[1] = dy; frame[2] = dz; }
@@ -799,6 +808,7 @@ __device__ int get_direction_ptt(const DevParams &P, const DevBufs &B, PttLds &S
             *ndx = shfl32(ox, win);
             *ndy = shfl32(oy, win);
             *ndz = shfl32(oz, win);
+            if (wk) { wk[0] = shfl32(k1, win); wk[1] = shfl32(k2, win); }
             WAVE_SYNC();
             return 1;
         }
@@ -830,6 +840,219 @@ __device__ bool init_frame_ptt(const DevParams &P, const DevBufs &B, PttLds &S, 
 }
 
 // -------------------------------------------------------------------------
+// Learned PTT proposal (no counterpart in the reference).
+//
+// Inputs x[40]: SH coefficients of the ODF at the point (trilinear, 0 outside
+// the volume), the incoming direction, the 9-float frame.  The MLP runs on the
+// 32 lanes of the seed slot: lane l owns hidden neurons l, l+32, l+64, l+96
+// and outputs l, l+32; each neuron is an fma chain over its inputs in order,
+// starting from the bias.  Parameters are bf16, arithmetic fp32.
+// -------------------------------------------------------------------------
+constexpr int MLP_SH = CUSLINES_SH_COEFFS, MLP_IN = CUSLINES_MLP_IN, MLP_H = CUSLINES_MLP_HIDDEN,
+              MLP_OUT = CUSLINES_MLP_OUT, MLP_K = CUSLINES_MLP_COMPONENTS;
+constexpr int MLP_W1 = 0, MLP_B1 = MLP_W1 + MLP_H * MLP_IN, MLP_W2 = MLP_B1 + MLP_H,
+              MLP_B2 = MLP_W2 + MLP_H * MLP_H, MLP_W3 = MLP_B2 + MLP_H, MLP_B3 = MLP_W3 + MLP_OUT * MLP_H,
+              MLP_NPARAM = MLP_B3 + MLP_OUT;
+static_assert(MLP_NPARAM == CUSLINES_MLP_NPARAM, "MLP layout");
+constexpr int MIX_STRIDE = 12;  // mean[3], e1[3], e2[3], kappa, exp(-2 kappa), cumulative weight
+constexpr int ACT_X = 0, ACT_H1 = 64, ACT_H2 = 192, ACT_O = 320, ACT_FLOATS = 384;
+constexpr int PTT_SLOT_FLOATS = 96 + MLP_K * MIX_STRIDE + ACT_FLOATS;
+
+__device__ __forceinline__ float bf16f(uint32_t h) { return __uint_as_float(h << 16); }
+
+// x[0..39] in LDS
+__device__ __forceinline__ void mlp_features(const DevParams &P, const DevBufs &B, float px, float py, float pz,
+                                             float dx, float dy, float dz, const float *fr, float *x) {
+    const int l = lane32();
+    if (trilinear(P, B.sh, 32, MLP_SH, px, py, pz, x) != 0 && l < MLP_SH) x[l] = 0.0f;
+    if (l < 3) x[MLP_SH + l] = (l == 0) ? dx : (l == 1) ? dy : dz;
+    if (l < 9) x[MLP_SH + 3 + l] = fr[l];
+    WAVE_SYNC();
+}
+
+// One neuron: bias + sum_k w[k] in[k] as an fma chain, K a multiple of 8.
+template <int K>
+__device__ __forceinline__ float mlp_neuron(const uint16_t *__restrict__ w, uint16_t bias, const float *in) {
+    float acc = bf16f(bias);
+    const uint4 *w8 = reinterpret_cast<const uint4 *>(w);
+#pragma unroll 2
+    for (int k8 = 0; k8 < K / 8; ++k8) {
+        const uint4 q = w8[k8];
+        const float4 a = *reinterpret_cast<const float4 *>(in + 8 * k8);
+        const float4 b = *reinterpret_cast<const float4 *>(in + 8 * k8 + 4);
+        acc = __builtin_fmaf(bf16f(q.x & 0xffffu), a.x, acc);
+        acc = __builtin_fmaf(bf16f(q.x >> 16), a.y, acc);
+        acc = __builtin_fmaf(bf16f(q.y & 0xffffu), a.z, acc);
+        acc = __builtin_fmaf(bf16f(q.y >> 16), a.w, acc);
+        acc = __builtin_fmaf(bf16f(q.z & 0xffffu), b.x, acc);
+        acc = __builtin_fmaf(bf16f(q.z >> 16), b.y, acc);
+        acc = __builtin_fmaf(bf16f(q.w & 0xffffu), b.z, acc);
+        acc = __builtin_fmaf(bf16f(q.w >> 16), b.w, acc);
+    }
+    return acc;
+}
+
+__device__ __noinline__ void mlp_forward(const uint16_t *__restrict__ W, float *act) {
+    const int l = lane32();
+#pragma unroll 1
+    for (int r = 0; r < 4; ++r) {
+        const int j = l + 32 * r;
+        const float v = mlp_neuron<MLP_IN>(W + MLP_W1 + j * MLP_IN, W[MLP_B1 + j], act + ACT_X);
+        act[ACT_H1 + j] = (v > 0.0f) ? v : 0.0f;
+    }
+    WAVE_SYNC();
+#pragma unroll 1
+    for (int r = 0; r < 4; ++r) {
+        const int j = l + 32 * r;
+        const float v = mlp_neuron<MLP_H>(W + MLP_W2 + j * MLP_H, W[MLP_B2 + j], act + ACT_H1);
+        act[ACT_H2 + j] = (v > 0.0f) ? v : 0.0f;
+    }
+    WAVE_SYNC();
+    for (int j = l; j < MLP_OUT; j += 32)
+        act[ACT_O + j] = mlp_neuron<MLP_H>(W + MLP_W3 + j * MLP_H, W[MLP_B3 + j], act + ACT_H2);
+    WAVE_SYNC();
+}
+
+// Mixture parameters from the outputs (once per output step; kept out of
+// line, as is mlp_forward, so the tracking loop keeps its register budget):
+// unit mean m (fallback +tangent), an
+// orthonormal pair (e1, e2) spanning the plane normal to m, kappa =
+// exp(clamp(.., -4, 6)), exp(-2 kappa), and cumulative softmax weights
+// (summed in component order).
+__device__ __noinline__ void mixture_setup(const float *o, float *mix) {
+    const int l = lane32();
+    if (l < MLP_K) {
+        const float *oc = o + 5 * l;
+        float mx = oc[0], my = oc[1], mz = oc[2];
+        const float n2 = __builtin_fmaf(mz, mz, __builtin_fmaf(my, my, mx * mx));
+        if (n2 > 1e-30f && n2 <= REAL_MAX) {
+            const float inv = rsqrt_nr(n2);
+            mx *= inv; my *= inv; mz *= inv;
+        } else {
+            mx = 1.0f; my = 0.0f; mz = 0.0f;
+        }
+        const float ax = (fabsf(mx) < 0.9f) ? 1.0f : 0.0f, ay = 1.0f - ax;
+        const float am = __builtin_fmaf(ay, my, ax * mx);
+        float e1x = __builtin_fmaf(-am, mx, ax), e1y = __builtin_fmaf(-am, my, ay), e1z = -am * mz;
+        const float inv1 = rsqrt_nr(__builtin_fmaf(e1z, e1z, __builtin_fmaf(e1y, e1y, e1x * e1x)));
+        e1x *= inv1; e1y *= inv1; e1z *= inv1;
+        const float e2x = __builtin_fmaf(my, e1z, -(mz * e1y));
+        const float e2y = __builtin_fmaf(mz, e1x, -(mx * e1z));
+        const float e2z = __builtin_fmaf(mx, e1y, -(my * e1x));
+        const float kap = uexp(fminf(fmaxf(oc[3], -4.0f), 6.0f));
+        float amax = o[4];
+        for (int c = 1; c < MLP_K; ++c) amax = fmaxf(amax, o[5 * c + 4]);
+        float *mc = mix + MIX_STRIDE * l;
+        mc[0] = mx; mc[1] = my; mc[2] = mz;
+        mc[3] = e1x; mc[4] = e1y; mc[5] = e1z;
+        mc[6] = e2x; mc[7] = e2y; mc[8] = e2z;
+        mc[9] = kap;
+        mc[10] = uexp(-2.0f * kap);
+        mc[11] = uexp(oc[4] - amax);
+    }
+    WAVE_SYNC();
+    if (l == 0) {
+        float c = 0.0f;
+        for (int k = 0; k < MLP_K; ++k) {
+            c = c + mix[MIX_STRIDE * k + 11];
+            mix[MIX_STRIDE * k + 11] = c;
+        }
+    }
+    WAVE_SYNC();
+}
+
+// One candidate per lane from the mixture: component by inverse CDF, then a
+// vMF direction (Wood 1994: w = 1 + log(u + (1-u) e^{-2 kappa}) / kappa,
+// azimuth 2 pi v) in frame coordinates; (k1, k2) = MAX_CURVATURE times its
+// normal and binormal parts.  Returns false for the far hemisphere.
+__device__ __forceinline__ bool mixture_draw(const DevParams &P, const float *mix, Philox &rng, float *k1, float *k2) {
+    const float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
+    const float tgt = u0 * mix[MIX_STRIDE * (MLP_K - 1) + 11];
+    int c = 0;
+    for (; c < MLP_K - 1; ++c)
+        if (mix[MIX_STRIDE * c + 11] >= tgt) break;
+    const float *mc = mix + MIX_STRIDE * c;
+    float w = 1.0f + ulog(__builtin_fmaf(1.0f - u1, mc[10], u1)) / mc[9];
+    w = fminf(fmaxf(w, -1.0f), 1.0f);
+    const float sw = __builtin_sqrtf(fmaxf(__builtin_fmaf(-w, w, 1.0f), 0.0f));
+    float sn, cs;
+    usincos(6.28318531f * u2, &sn, &cs);
+    const float a1 = sw * cs, a2 = sw * sn;
+    const float ut = __builtin_fmaf(a2, mc[6], __builtin_fmaf(a1, mc[3], w * mc[0]));
+    const float un = __builtin_fmaf(a2, mc[7], __builtin_fmaf(a1, mc[4], w * mc[1]));
+    const float ub = __builtin_fmaf(a2, mc[8], __builtin_fmaf(a1, mc[5], w * mc[2]));
+    *k1 = P.max_curv * un;
+    *k2 = P.max_curv * ub;
+    return ut > 0.0f;
+}
+
+// A non-initial PTT sub-step from the learned mixture: one round of 32
+// candidates under the reference's acceptance rule (support >= 0.4, lowest
+// lane wins).  Returns 1 with the new direction, or -1 when no candidate
+// passes (the caller then runs the plain getter on the same streams).
+__device__ int get_direction_ptt_learned(const DevParams &P, const DevBufs &B, PttLds &S, float *frame, Philox &rng,
+                                         float px, float py, float pz, float *ndx, float *ndy, float *ndz) {
+    const int l = lane32();
+    float fr[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) fr[i] = frame[i];
+    const float first_val = interp4(P, B, px, py, pz, fr[0], fr[1], fr[2]);
+    float k1, k2, pf[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) pf[i] = fr[i];
+    float sup = -1.0f;
+    if (mixture_draw(P, S.mix, rng, &k1, &k2)) sup = data_support(P, B, first_val, px, py, pz, k1, k2, pf);
+    const uint32_t msk = half_ballot(!(sup < P.min_support_f));
+    if (msk == 0u) return -1;
+    const int win = __ffs(msk) - 1;
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+    if (l == win) {
+        float prop[9], d[3];
+        prepare_propagator(k1, k2, P.ptt_sub_step, prop);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) pf[i] = fr[i];
+        propagate_frame(prop, pf, d);
+        norm3<0>(d[0], d[1], d[2]);
+        ox = d[0]; oy = d[1]; oz = d[2];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) frame[i] = pf[i];
+    }
+    *ndx = shfl32(ox, win);
+    *ndy = shfl32(oy, win);
+    *ndz = shfl32(oz, win);
+    WAVE_SYNC();
+    return 1;
+}
+
+// Training pair of an accepted plain sub-step: inputs at the sub-step's
+// start and the winner as a unit vector (tangent part, k1/mc, k2/mc).
+__device__ void record_pair(const DevParams &P, const DevBufs &B, PttLds &S, float px, float py, float pz, float dx,
+                            float dy, float dz, const float *fr, const float *wk) {
+    const int l = lane32();
+    mlp_features(P, B, px, py, pz, dx, dy, dz, fr, S.act + ACT_X);
+    int idx = 0;
+    if (l == 0) idx = atomicAdd(B.rec_count, 1);
+    idx = shfl32(idx, 0);
+    if (idx < P.rec_cap) {
+        float *o = B.rec + (size_t)idx * CUSLINES_PAIR_FLOATS;
+        for (int j = l; j < MLP_IN; j += 32) o[j] = S.act[ACT_X + j];
+        if (l == 0) {
+            const float a = wk[0] / P.max_curv, b = wk[1] / P.max_curv;
+            o[MLP_IN] = __builtin_sqrtf(fmaxf(1.0f - __builtin_fmaf(a, a, b * b), 0.0f));
+            o[MLP_IN + 1] = a;
+            o[MLP_IN + 2] = b;
+            o[MLP_IN + 3] = 0.0f;
+        }
+    }
+    WAVE_SYNC();
+}
+
+__device__ __forceinline__ uint32_t mix32(uint32_t h) {  // murmur3 finaliser
+    h ^= h >> 16; h *= 0x85ebca6bu; h ^= h >> 13; h *= 0xc2b2ae35u; h ^= h >> 16;
+    return h;
+}
+
+// -------------------------------------------------------------------------
 // Tracker (generate_streamlines_cuda.cu:277-421, boot.cu:720-824)
 // -------------------------------------------------------------------------
 struct SlotLds {
@@ -838,23 +1061,51 @@ struct SlotLds {
     PttLds ptt;
 };
 
-template <int MODEL>
+// V (PTT only): 0 plain getter, 1 learned proposal, 2 plain getter recording
+// training pairs (tag identifies seed, direction and pass for the hash).
+template <int MODEL, int V = 0>
 __device__ int tracker(const DevParams &P, const DevBufs &B, const float *sv, SlotLds &S, Philox &rng,
                        float sx, float sy, float sz, float dx, float dy, float dz, float *frame,
-                       float *__restrict__ sl) {
+                       float *__restrict__ sl, uint32_t tag = 0) {
     const int l = lane32();
     const int step_frac = (MODEL == CUSLINES_PTT) ? P.step_frac : 1;
     const int MAXL = P.max_sline_len;
     const double sub = P.step_d / (double)step_frac;
     float px = sx, py = sy, pz = sz;
     if (l == 0) { sl[0] = px; sl[1] = py; sl[2] = pz; }
+    int ltries = 0, lfalls = 0, nplain = 0;
+    bool learned = (V == 1);
     int i;
     for (i = 1; i < MAXL * step_frac; ++i) {
         float nx = 0.0f, ny = 0.0f, nz = 0.0f;
         int ndir;
         if (MODEL == CUSLINES_PROB)
             ndir = get_direction_prob<false>(P, B, sv, S.prob, rng, dx, dy, dz, px, py, pz, &nx, &ny, &nz);
-        else if (MODEL == CUSLINES_PTT)
+        else if (MODEL == CUSLINES_PTT && V == 1) {
+            ndir = -1;
+            if (learned) {
+                if ((i - 1) % step_frac == 0) {
+                    mlp_features(P, B, px, py, pz, dx, dy, dz, frame, S.ptt.act + ACT_X);
+                    mlp_forward(B.mlp, S.ptt.act);
+                    mixture_setup(S.ptt.act + ACT_O, S.ptt.mix);
+                }
+                ndir = get_direction_ptt_learned(P, B, S.ptt, frame, rng, px, py, pz, &nx, &ny, &nz);
+                ++ltries;
+                if (ndir < 0) ++lfalls;
+                if (ltries >= 16 && 4 * lfalls > ltries) learned = false;
+            } else {
+                ++nplain;
+            }
+            if (ndir < 0)
+                ndir = get_direction_ptt<false>(P, B, S.ptt, frame, rng, dx, dy, dz, px, py, pz, &nx, &ny, &nz);
+        } else if (MODEL == CUSLINES_PTT && V == 2) {
+            float fr0[9], wk[2];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) fr0[q] = frame[q];
+            ndir = get_direction_ptt<false>(P, B, S.ptt, frame, rng, dx, dy, dz, px, py, pz, &nx, &ny, &nz, wk);
+            if (ndir && mix32(tag * 0x9E3779B1u + (uint32_t)i) < P.rec_thresh)
+                record_pair(P, B, S.ptt, px, py, pz, dx, dy, dz, fr0, wk);
+        } else if (MODEL == CUSLINES_PTT)
             ndir = get_direction_ptt<false>(P, B, S.ptt, frame, rng, dx, dy, dz, px, py, pz, &nx, &ny, &nz);
         else
             ndir = get_direction_boot<MODEL>(P, B, sv, S.boot, rng, 5, dx, dy, dz, px, py, pz, &nx, &ny, &nz);
@@ -870,6 +1121,11 @@ __device__ int tracker(const DevParams &P, const DevBufs &B, const float *sv, Sl
         const int tissue = check_point(P, B.stop, px, py, pz);
         if (tissue == CUSLINES_ENDPOINT || tissue == CUSLINES_INVALIDPOINT || tissue == CUSLINES_OUTSIDEIMAGE)
             break;
+    }
+    if (V == 1 && B.stats && l == 0) {
+        atomicAdd(B.stats, (unsigned long long)(ltries - lfalls));
+        atomicAdd(B.stats + 1, (unsigned long long)lfalls);
+        atomicAdd(B.stats + 2, (unsigned long long)nplain);
     }
     int nsteps = i / step_frac;
     if ((i % step_frac) != 0 && i < step_frac * (MAXL - 1)) {
@@ -1350,7 +1606,7 @@ __host__ __device__ inline int boot_region_a(int nv32, int t32, int hr32) {
 
 // Floats of LDS per seed slot.
 __host__ __device__ inline int slot_floats(int model, bool phase1, int nv32, int t32, int hr32) {
-    if (model == CUSLINES_PTT) return phase1 ? 3 * nv32 : 96;
+    if (model == CUSLINES_PTT) return phase1 ? 3 * nv32 : PTT_SLOT_FLOATS;
     if (model == CUSLINES_PROB) return phase1 ? 3 * nv32 : nv32;
     const int bc = (nv32 > hr32) ? nv32 : hr32;
     return boot_region_a(nv32, t32, hr32) + 2 * bc;
@@ -1362,6 +1618,8 @@ __device__ void bind_slot(const DevParams &P, bool phase1, float *base, SlotLds 
         S.ptt.frame = base;
         S.ptt.vert_pdf = base + 32;
         S.ptt.face_cdf = base + 64;
+        S.ptt.mix = base + 96;
+        S.ptt.act = base + 96 + MLP_K * MIX_STRIDE;
     } else if (MODEL == CUSLINES_PROB || MODEL == CUSLINES_PTT) {
         S.prob.pmf = base;
         S.prob.marks = reinterpret_cast<int *>(base + P.nv32);
@@ -1430,7 +1688,7 @@ __global__ void __launch_bounds__(256) phase1_kernel(DevParams P, DevBufs B, con
 
 // genStreamlinesMerge{Prob,Boot}_k body for one seed (generate_streamlines_cuda.cu:484-621,
 // boot.cu:855-979): per direction, backward track, reverse, forward track.
-template <int MODEL>
+template <int MODEL, int V = 0>
 __device__ void track_seed(const DevParams &P, const DevBufs &B, const float *sv, SlotLds &S, int s,
                            const float *__restrict__ seeds, int rng_chunk, const int *__restrict__ offsets,
                            const int *__restrict__ dirs, int *__restrict__ sline_seed, int *__restrict__ sline_len,
@@ -1461,7 +1719,8 @@ __device__ void track_seed(const DevParams &P, const DevBufs &B, const float *sv
             const float sg = pass ? 1.0f : -1.0f;
             float *frame = (MODEL == CUSLINES_PTT) ? S.ptt.frame + 9 * pass : nullptr;
             float *out = pass ? cur + 3 * (size_t)(nb - 1) : cur;
-            const int ns = tracker<MODEL>(P, B, sv, S, rng, sx, sy, sz, sg * fx, sg * fy, sg * fz, frame, out);
+            const int ns = tracker<MODEL, V>(P, B, sv, S, rng, sx, sy, sz, sg * fx, sg * fy, sg * fz, frame, out,
+                                             ((uint32_t)s * 64u + (uint32_t)i) * 2u + (uint32_t)pass);
             if (pass == 0) {
                 nb = ns;
                 MEM_SYNC();
@@ -1481,7 +1740,7 @@ __device__ void track_seed(const DevParams &P, const DevBufs &B, const float *sv
 }
 
 // One seed per 32-lane slot, static assignment.
-template <int MODEL>
+template <int MODEL, int V = 0>
 __global__ void __launch_bounds__(256) phase2_kernel(DevParams P, DevBufs B, const float *__restrict__ seeds,
                                                      int nseeds, int rng_chunk, const int *__restrict__ offsets,
                                                      const int *__restrict__ dirs, int *__restrict__ sline_seed,
@@ -1500,14 +1759,16 @@ __global__ void __launch_bounds__(256) phase2_kernel(DevParams P, DevBufs B, con
     if (s >= nseeds) return;
     SlotLds S;
     bind_slot<MODEL>(P, false, smem + svf + slot * slot_floats(MODEL, false, P.nv32, P.t32, P.hr32), S);
-    track_seed<MODEL>(P, B, sv, S, s, seeds, rng_chunk, offsets, dirs, sline_seed, sline_len, sline);
+    track_seed<MODEL, V>(P, B, sv, S, s, seeds, rng_chunk, offsets, dirs, sline_seed, sline_len, sline);
 }
 
 // Persistent work queue: a fixed grid of slots; each 32-lane slot takes the
 // next seed from a global counter as soon as its previous seed is done, so
 // seeds of very uneven length keep every slot busy.  `work` must hold 0.
-template <int MODEL>
-__global__ void __launch_bounds__(256) phase2_queue_kernel(DevParams P, DevBufs B, const float *__restrict__ seeds,
+// Plain PTT is held to 128 VGPRs (4 waves per SIMD): the few spills cost less
+// than the latency hiding gained (+8% at 200k seeds).
+template <int MODEL, int V = 0>
+__global__ void __launch_bounds__(256, (MODEL == CUSLINES_PTT && V == 0) ? 4 : 1) phase2_queue_kernel(DevParams P, DevBufs B, const float *__restrict__ seeds,
                                                            int nseeds, int rng_chunk, const int *__restrict__ offsets,
                                                            const int *__restrict__ dirs, int *__restrict__ sline_seed,
                                                            int *__restrict__ sline_len, float *__restrict__ sline,
@@ -1529,11 +1790,148 @@ __global__ void __launch_bounds__(256) phase2_queue_kernel(DevParams P, DevBufs 
         if (l == 0) s = atomicAdd(work, 1);
         s = shfl32(s, 0);
         if (s >= nseeds) break;
-        track_seed<MODEL>(P, B, sv, S, s, seeds, rng_chunk, offsets, dirs, sline_seed, sline_len, sline);
+        track_seed<MODEL, V>(P, B, sv, S, s, seeds, rng_chunk, offsets, dirs, sline_seed, sline_len, sline);
     }
 }
 
 __global__ void zero_int(int *p) { *p = 0; }
+
+// -------------------------------------------------------------------------
+// Training of the proposal network (no counterpart in the reference).  Plain
+// per-element kernels: the network is 26,920 parameters and a batch of a few
+// thousand pairs, so a training step is microseconds of arithmetic.
+// -------------------------------------------------------------------------
+__device__ __forceinline__ float bf16_round(float x) {  // to the nearest bf16, ties to even
+    uint32_t b = __float_as_uint(x);
+    if ((b & 0x7f800000u) == 0x7f800000u) return x;
+    b += 0x7fffu + ((b >> 16) & 1u);
+    return __uint_as_float(b & 0xffff0000u);
+}
+
+// out[s][j] = act(b[j] + sum_k W[j][k] in[s][k]), parameters rounded to bf16.
+template <int K, int N, bool RELU>
+__global__ void mlp_fwd_layer(const float *__restrict__ prm, int w_off, int b_off, const float *__restrict__ in,
+                              int in_ld, int n, float *__restrict__ out) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)n * N) return;
+    const int smp = (int)(idx / N), j = (int)(idx % N);
+    const float *w = prm + w_off + (size_t)j * K;
+    const float *x = in + (size_t)smp * in_ld;
+    float acc = bf16_round(prm[b_off + j]);
+    for (int k = 0; k < K; ++k) acc = __builtin_fmaf(bf16_round(w[k]), x[k], acc);
+    out[idx] = RELU ? ((acc > 0.0f) ? acc : 0.0f) : acc;
+}
+
+// Per pair: negative log-likelihood of the target under the mixture and its
+// gradient with respect to the 40 outputs, divided by n.
+__global__ void mlp_loss_out(const float *__restrict__ O, const float *__restrict__ pairs, int n,
+                             float *__restrict__ dO, float *__restrict__ loss) {
+    const int smp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (smp >= n) return;
+    const float *o = O + (size_t)smp * MLP_OUT;
+    const float *u = pairs + (size_t)smp * CUSLINES_PAIR_FLOATS + MLP_IN;
+    float mh[MLP_K][3], nrm[MLP_K], kap[MLP_K], dd[MLP_K], ell[MLP_K], lpi[MLP_K], e2[MLP_K];
+    float amax = o[4];
+    for (int c = 1; c < MLP_K; ++c) amax = fmaxf(amax, o[5 * c + 4]);
+    float sa = 0.0f;
+    for (int c = 0; c < MLP_K; ++c) sa += uexp(o[5 * c + 4] - amax);
+    const float lsa = amax + ulog(sa);
+    float lmax = -REAL_MAX;
+    for (int c = 0; c < MLP_K; ++c) {
+        const float *oc = o + 5 * c;
+        const float n2 = __builtin_fmaf(oc[2], oc[2], __builtin_fmaf(oc[1], oc[1], oc[0] * oc[0]));
+        nrm[c] = __builtin_sqrtf(n2);
+        const float inv = (nrm[c] > 1e-12f) ? 1.0f / nrm[c] : 0.0f;
+        mh[c][0] = oc[0] * inv; mh[c][1] = oc[1] * inv; mh[c][2] = oc[2] * inv;
+        kap[c] = uexp(fminf(fmaxf(oc[3], -4.0f), 6.0f));
+        e2[c] = uexp(-2.0f * kap[c]);
+        dd[c] = __builtin_fmaf(mh[c][2], u[2], __builtin_fmaf(mh[c][1], u[1], mh[c][0] * u[0]));
+        lpi[c] = o[5 * c + 4] - lsa;
+        const float logc = ulog(kap[c]) - 1.83787707f - kap[c] - ulog(1.0f - e2[c]);
+        ell[c] = lpi[c] + logc + kap[c] * dd[c];
+        lmax = fmaxf(lmax, ell[c]);
+    }
+    float se = 0.0f;
+    for (int c = 0; c < MLP_K; ++c) se += uexp(ell[c] - lmax);
+    const float L = lmax + ulog(se);
+    loss[smp] = -L;
+    const float sc = 1.0f / (float)n;
+    float *g = dO + (size_t)smp * MLP_OUT;
+    for (int c = 0; c < MLP_K; ++c) {
+        const float r = uexp(ell[c] - L);
+        const float pi = uexp(lpi[c]);
+        g[5 * c + 4] = (pi - r) * sc;
+        const float z = o[5 * c + 3];
+        const float dl = 1.0f - 2.0f * kap[c] * e2[c] / (1.0f - e2[c]) - kap[c] + kap[c] * dd[c];
+        g[5 * c + 3] = (z < -4.0f || z > 6.0f) ? 0.0f : -r * dl * sc;
+        const float f = (nrm[c] > 1e-12f) ? -r * kap[c] / nrm[c] * sc : 0.0f;
+        for (int a = 0; a < 3; ++a) g[5 * c + a] = f * (u[a] - dd[c] * mh[c][a]);
+    }
+}
+
+// din[s][k] = (sum_j W[j][k] dout[s][j]) where hin[s][k] > 0, else 0.
+template <int N, int K>
+__global__ void mlp_bwd_input(const float *__restrict__ prm, int w_off, const float *__restrict__ dout,
+                              const float *__restrict__ hin, int n, float *__restrict__ din) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)n * K) return;
+    const int smp = (int)(idx / K), k = (int)(idx % K);
+    const float *g = dout + (size_t)smp * N;
+    float acc = 0.0f;
+    for (int j = 0; j < N; ++j) acc = __builtin_fmaf(bf16_round(prm[w_off + (size_t)j * K + k]), g[j], acc);
+    din[idx] = (hin[idx] > 0.0f) ? acc : 0.0f;
+}
+
+// grad W[j][k] = sum_s dout[s][j] in[s][k]; column k == K is the bias.
+template <int N, int K>
+__global__ void mlp_wgrad(const float *__restrict__ dout, const float *__restrict__ in, int in_ld, int n,
+                          float *__restrict__ grad, int w_off, int b_off) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= N * (K + 1)) return;
+    const int j = idx / (K + 1), k = idx % (K + 1);
+    float acc = 0.0f;
+    if (k == K) {
+        for (int smp = 0; smp < n; ++smp) acc += dout[(size_t)smp * N + j];
+        grad[b_off + j] = acc;
+    } else {
+        for (int smp = 0; smp < n; ++smp)
+            acc = __builtin_fmaf(dout[(size_t)smp * N + j], in[(size_t)smp * in_ld + k], acc);
+        grad[w_off + (size_t)j * K + k] = acc;
+    }
+}
+
+__global__ void __launch_bounds__(256) mean_kernel(const float *__restrict__ x, int n, float *__restrict__ out) {
+    __shared__ float part[256];
+    float acc = 0.0f;
+    for (int i = threadIdx.x; i < n; i += 256) acc += x[i];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = part[0] / (float)n;
+}
+
+// Adam (Kingma and Ba, 2015); c1 = 1 - beta1^step, c2 = 1 - beta2^step.
+__global__ void adam_kernel(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                            float *__restrict__ v, int n, float lr, float b1, float b2, float eps, float c1, float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (__builtin_sqrtf(vi / c2) + eps);
+}
+
+__global__ void pad_sh(const float *__restrict__ src, float *__restrict__ dst, long long nvox) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nvox * 32) return;
+    const int c = (int)(i & 31);
+    dst[i] = (c < MLP_SH) ? src[(i >> 5) * MLP_SH + c] : 0.0f;
+}
 
 // PTT corner blocks: for vertex v and texel cell (cx,cy,cz) in [-1, D-1]^3 of
 // its tile, the 8 texels one linear-filtered tap reads (x fastest), with the
@@ -1710,6 +2108,11 @@ struct cuslines_tracker {
     int blocks_per_cu = 0;  // 0: occupancy maximum
     int num_cu = 1;
     int ptt_lanes = 0;      // PTT: lanes per seed (8, 16, 32; 0 = shared 32-lane kernel)
+    float *d_sh = nullptr;               // SH coefficients [X][Y][Z][32]
+    uint16_t *d_mlp = nullptr;           // learned proposal (nullptr: plain getter)
+    unsigned long long *d_stats = nullptr;
+    float *rec = nullptr;                // caller's training-pair buffer (nullptr: no recording)
+    int *rec_count = nullptr;
     bool has_data = false, has_stop = false, has_sphere = false, has_boot = false, has_lut = false;
 };
 
@@ -1732,6 +2135,11 @@ static DevBufs bufs_of(const cuslines_tracker *t) {
     b.dwi_idx = t->d_dwi;
     b.lut = t->d_lut;
     b.blocks = t->use_blocks ? t->d_blocks : nullptr;
+    b.sh = t->d_sh;
+    b.mlp = t->d_mlp;
+    b.rec = t->rec;
+    b.rec_count = t->rec_count;
+    b.stats = t->d_stats;
     return b;
 }
 
@@ -1796,9 +2204,9 @@ static int launch_ptt_groups(cuslines_tracker *t, const float *seeds, int n, int
     }
 }
 
-template <int MODEL>
-static int launch_phase2(cuslines_tracker *t, const float *seeds, int n, int chunk, const int *offsets,
-                         const int *dirs, int *sseed, int *slen, float *sl) {
+template <int MODEL, int V>
+static int launch_phase2_v(cuslines_tracker *t, const float *seeds, int n, int chunk, const int *offsets,
+                           const int *dirs, int *sseed, int *slen, float *sl) {
     const DevParams &d = t->dp;
     int threads = t->threads;
     if (threads <= 0) threads = (MODEL == CUSLINES_OPDT || MODEL == CUSLINES_CSA) ? 128 : 256;
@@ -1807,16 +2215,15 @@ static int launch_phase2(cuslines_tracker *t, const float *seeds, int n, int chu
     const size_t lds = sizeof(float) * (vf + (size_t)nslots * slot_floats(MODEL, false, d.nv32, d.t32, d.hr32));
     if (lds > 160 * 1024) return fail(CUSLINES_EINVAL, "sphere too large for LDS");
     if (t->schedule == CUSLINES_SCHED_STATIC) {
-        HIPC(hipFuncSetAttribute((const void *)phase2_kernel<MODEL>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        HIPC(hipFuncSetAttribute((const void *)phase2_kernel<MODEL, V>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)lds));
         const unsigned grid = (unsigned)((n + nslots - 1) / nslots);
-        hipLaunchKernelGGL(phase2_kernel<MODEL>, dim3(grid), dim3(threads), lds, t->stream, d, bufs_of(t), seeds, n,
-                           chunk, offsets, dirs, sseed, slen, sl);
+        hipLaunchKernelGGL((phase2_kernel<MODEL, V>), dim3(grid), dim3(threads), lds, t->stream, d, bufs_of(t), seeds,
+                           n, chunk, offsets, dirs, sseed, slen, sl);
         HIPC(hipGetLastError());
         return CUSLINES_OK;
     }
-    if (MODEL == CUSLINES_PTT && t->ptt_lanes > 0) return launch_ptt_groups(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl);
-    const void *kfn = (const void *)phase2_queue_kernel<MODEL>;
+    const void *kfn = (const void *)phase2_queue_kernel<MODEL, V>;
     HIPC(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int per_cu = 0;
     HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, threads, lds));
@@ -1827,10 +2234,24 @@ static int launch_phase2(cuslines_tracker *t, const float *seeds, int n, int chu
     if (grid > need) grid = need;
     if (grid < 1) grid = 1;
     hipLaunchKernelGGL(zero_int, dim3(1), dim3(1), 0, t->stream, t->d_work);
-    hipLaunchKernelGGL(phase2_queue_kernel<MODEL>, dim3((unsigned)grid), dim3(threads), lds, t->stream, d,
+    hipLaunchKernelGGL((phase2_queue_kernel<MODEL, V>), dim3((unsigned)grid), dim3(threads), lds, t->stream, d,
                        bufs_of(t), seeds, n, chunk, offsets, dirs, sseed, slen, sl, t->d_work);
     HIPC(hipGetLastError());
     return CUSLINES_OK;
+}
+
+template <int MODEL>
+static int launch_phase2(cuslines_tracker *t, const float *seeds, int n, int chunk, const int *offsets,
+                         const int *dirs, int *sseed, int *slen, float *sl) {
+    if constexpr (MODEL == CUSLINES_PTT) {
+        if (t->d_mlp)
+            return launch_phase2_v<MODEL, 1>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl);
+        if (t->rec)
+            return launch_phase2_v<MODEL, 2>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl);
+        if (t->ptt_lanes > 0 && t->schedule != CUSLINES_SCHED_STATIC)
+            return launch_ptt_groups(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl);
+    }
+    return launch_phase2_v<MODEL, 0>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl);
 }
 
 template <typename TO>
@@ -1956,6 +2377,7 @@ int cuslines_tracker_destroy(cuslines_tracker *t) {
     dfree(t->d_data); dfree(t->d_stop); dfree(t->d_verts); dfree(t->d_edges);
     dfree(t->d_H); dfree(t->d_R); dfree(t->d_db); dfree(t->d_dq); dfree(t->d_smat);
     dfree(t->d_b0); dfree(t->d_dwi); dfree(t->d_lut); dfree(t->d_work); dfree(t->d_blocks);
+    dfree(t->d_sh); dfree(t->d_mlp); dfree(t->d_stats);
     if (t->stream) (void)hipStreamDestroy(t->stream);
     delete t;
     return CUSLINES_OK;
@@ -2159,6 +2581,137 @@ int cuslines_set_lanes_per_seed(cuslines_tracker *t, int lanes) {
 int cuslines_set_ptt_layout(cuslines_tracker *t, int corner_blocks) {
     if (!t) return fail(CUSLINES_EINVAL, "null tracker");
     t->use_blocks = corner_blocks ? 1 : 0;
+    return CUSLINES_OK;
+}
+
+int cuslines_set_sh_coeffs(cuslines_tracker *t, const float *sh, int32_t ncoef) {
+    if (!t || !sh) return fail(CUSLINES_EINVAL, "null argument");
+    if (ncoef != CUSLINES_SH_COEFFS) return fail(CUSLINES_EINVAL, "ncoef must be CUSLINES_SH_COEFFS (28)");
+    HIPC(hipSetDevice(t->device));
+    const long long nvox = (long long)t->dp.dimx * t->dp.dimy * t->dp.dimz;
+    float *tmp = nullptr;
+    HIPC(hipMalloc(&tmp, (size_t)nvox * MLP_SH * sizeof(float)));
+    dfree(t->d_sh);
+    t->d_sh = nullptr;
+    hipError_t e = hipMalloc(&t->d_sh, (size_t)nvox * 32 * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpy(tmp, sh, (size_t)nvox * MLP_SH * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(pad_sh, dim3((unsigned)((nvox * 32 + 255) / 256)), dim3(256), 0, t->stream, tmp, t->d_sh,
+                           nvox);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+    dfree(tmp);
+    if (e != hipSuccess) return fail(CUSLINES_EHIP, std::string("SH upload: ") + hipGetErrorString(e));
+    return CUSLINES_OK;
+}
+
+int cuslines_set_proposal(cuslines_tracker *t, const uint16_t *mlp_bf16) {
+    if (!t) return fail(CUSLINES_EINVAL, "null tracker");
+    HIPC(hipSetDevice(t->device));
+    if (!mlp_bf16) {
+        dfree(t->d_mlp);
+        t->d_mlp = nullptr;
+        return CUSLINES_OK;
+    }
+    if (t->p.model_type != CUSLINES_PTT) return fail(CUSLINES_EINVAL, "the learned proposal is for PTT");
+    if (!t->d_sh) return fail(CUSLINES_ESTATE, "cuslines_set_sh_coeffs first");
+    if (!t->d_mlp) HIPC(hipMalloc(&t->d_mlp, MLP_NPARAM * sizeof(uint16_t)));
+    HIPC(hipMemcpy(t->d_mlp, mlp_bf16, MLP_NPARAM * sizeof(uint16_t), hipMemcpyHostToDevice));
+    if (!t->d_stats) {
+        HIPC(hipMalloc(&t->d_stats, 4 * sizeof(unsigned long long)));
+        HIPC(hipMemset(t->d_stats, 0, 4 * sizeof(unsigned long long)));
+    }
+    return CUSLINES_OK;
+}
+
+int cuslines_set_recording(cuslines_tracker *t, float *pairs_d, int32_t capacity, int32_t *count_d, float rate) {
+    if (!t) return fail(CUSLINES_EINVAL, "null tracker");
+    if (!pairs_d) {
+        t->rec = nullptr;
+        t->rec_count = nullptr;
+        return CUSLINES_OK;
+    }
+    if (!count_d || capacity <= 0 || !(rate > 0.0f && rate <= 1.0f))
+        return fail(CUSLINES_EINVAL, "recording needs count_d, capacity > 0 and 0 < rate <= 1");
+    if (t->p.model_type != CUSLINES_PTT) return fail(CUSLINES_EINVAL, "recording is for PTT");
+    if (!t->d_sh) return fail(CUSLINES_ESTATE, "cuslines_set_sh_coeffs first");
+    t->rec = pairs_d;
+    t->rec_count = count_d;
+    t->dp.rec_cap = capacity;
+    const double th = (double)rate * 4294967296.0;
+    t->dp.rec_thresh = (th >= 4294967295.0) ? 0xffffffffu : (uint32_t)th;
+    return CUSLINES_OK;
+}
+
+int cuslines_proposal_stats(cuslines_tracker *t, int64_t *out3, int reset) {
+    if (!t || !out3) return fail(CUSLINES_EINVAL, "null argument");
+    HIPC(hipSetDevice(t->device));
+    unsigned long long h[4] = {0, 0, 0, 0};
+    if (t->d_stats) {
+        HIPC(hipStreamSynchronize(t->stream));
+        HIPC(hipMemcpy(h, t->d_stats, sizeof(h), hipMemcpyDeviceToHost));
+        if (reset) HIPC(hipMemset(t->d_stats, 0, sizeof(h)));
+    }
+    for (int i = 0; i < 3; ++i) out3[i] = (int64_t)h[i];
+    return CUSLINES_OK;
+}
+
+static size_t mlp_ws_floats(int n) { return (size_t)n * (4 * MLP_H + 2 * MLP_OUT + 1); }
+
+int cuslines_mlp_workspace_bytes(int32_t n, size_t *bytes) {
+    if (!bytes || n <= 0) return fail(CUSLINES_EINVAL, "n > 0 and bytes required");
+    *bytes = mlp_ws_floats(n) * sizeof(float);
+    return CUSLINES_OK;
+}
+
+int cuslines_mlp_loss_grad_async(cuslines_tracker *t, const float *params_d, const float *pairs_d, int32_t n,
+                                 float *loss_d, float *grad_d, void *work_d) {
+    if (!t || !params_d || !pairs_d || !loss_d || !grad_d || !work_d || n <= 0)
+        return fail(CUSLINES_EINVAL, "null argument or n <= 0");
+    HIPC(hipSetDevice(t->device));
+    float *H1 = static_cast<float *>(work_d);
+    float *H2 = H1 + (size_t)n * MLP_H;
+    float *O = H2 + (size_t)n * MLP_H;
+    float *dO = O + (size_t)n * MLP_OUT;
+    float *dH2 = dO + (size_t)n * MLP_OUT;
+    float *dH1 = dH2 + (size_t)n * MLP_H;
+    float *ls = dH1 + (size_t)n * MLP_H;
+    const float *X = pairs_d;
+    const int LD = CUSLINES_PAIR_FLOATS;
+    auto g1 = [](long long cnt) { return dim3((unsigned)((cnt + 255) / 256)); };
+    hipStream_t st = t->stream;
+    hipLaunchKernelGGL((mlp_fwd_layer<MLP_IN, MLP_H, true>), g1((long long)n * MLP_H), dim3(256), 0, st, params_d,
+                       MLP_W1, MLP_B1, X, LD, n, H1);
+    hipLaunchKernelGGL((mlp_fwd_layer<MLP_H, MLP_H, true>), g1((long long)n * MLP_H), dim3(256), 0, st, params_d,
+                       MLP_W2, MLP_B2, H1, MLP_H, n, H2);
+    hipLaunchKernelGGL((mlp_fwd_layer<MLP_H, MLP_OUT, false>), g1((long long)n * MLP_OUT), dim3(256), 0, st,
+                       params_d, MLP_W3, MLP_B3, H2, MLP_H, n, O);
+    hipLaunchKernelGGL(mlp_loss_out, g1(n), dim3(256), 0, st, O, X, n, dO, ls);
+    hipLaunchKernelGGL((mlp_wgrad<MLP_OUT, MLP_H>), g1(MLP_OUT * (MLP_H + 1)), dim3(256), 0, st, dO, H2, MLP_H, n,
+                       grad_d, MLP_W3, MLP_B3);
+    hipLaunchKernelGGL((mlp_bwd_input<MLP_OUT, MLP_H>), g1((long long)n * MLP_H), dim3(256), 0, st, params_d, MLP_W3,
+                       dO, H2, n, dH2);
+    hipLaunchKernelGGL((mlp_wgrad<MLP_H, MLP_H>), g1(MLP_H * (MLP_H + 1)), dim3(256), 0, st, dH2, H1, MLP_H, n, grad_d,
+                       MLP_W2, MLP_B2);
+    hipLaunchKernelGGL((mlp_bwd_input<MLP_H, MLP_H>), g1((long long)n * MLP_H), dim3(256), 0, st, params_d, MLP_W2,
+                       dH2, H1, n, dH1);
+    hipLaunchKernelGGL((mlp_wgrad<MLP_H, MLP_IN>), g1(MLP_H * (MLP_IN + 1)), dim3(256), 0, st, dH1, X, LD, n, grad_d,
+                       MLP_W1, MLP_B1);
+    hipLaunchKernelGGL(mean_kernel, dim3(1), dim3(256), 0, st, ls, n, loss_d);
+    HIPC(hipGetLastError());
+    return CUSLINES_OK;
+}
+
+int cuslines_mlp_adam_async(cuslines_tracker *t, float *params_d, const float *grad_d, float *m_d, float *v_d,
+                            int32_t step, float lr, float beta1, float beta2, float eps) {
+    if (!t || !params_d || !grad_d || !m_d || !v_d || step < 1) return fail(CUSLINES_EINVAL, "bad Adam arguments");
+    HIPC(hipSetDevice(t->device));
+    const float c1 = (float)(1.0 - std::pow((double)beta1, (double)step));
+    const float c2 = (float)(1.0 - std::pow((double)beta2, (double)step));
+    hipLaunchKernelGGL(adam_kernel, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, t->stream, params_d, grad_d, m_d,
+                       v_d, MLP_NPARAM, lr, beta1, beta2, eps, c1, c2);
+    HIPC(hipGetLastError());
     return CUSLINES_OK;
 }
 

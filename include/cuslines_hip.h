@@ -83,6 +83,31 @@ typedef struct cuslines_params {
     float max_curvature;             /* MAX_CURVATURE                            */
 } cuslines_params;
 
+/*
+ * Learned PTT proposal (no counterpart in the reference).  A BF16 MLP
+ * 40 -> 128 -> 128 -> 40 (ReLU) maps (28 SH coefficients of the ODF at the
+ * point, the incoming direction, the 9-float PTT frame) to an 8-component
+ * von Mises-Fisher mixture on the hemisphere around the frame tangent, whose
+ * orthographic projection onto the (k1, k2) curvature disc proposes PTT
+ * candidates.  The reference's acceptance rule (probe support >= 0.4, first
+ * lane wins) decides every candidate; when no candidate of a round passes,
+ * the plain getter runs the sub-step.
+ * Parameter layout (row-major, CUSLINES_MLP_NPARAM values): W1[128][40],
+ * b1[128], W2[128][128], b2[128], W3[40][128], b3[40].  Output j of component
+ * c: 5c..5c+2 mean direction in frame coordinates (tangent, normal,
+ * binormal), 5c+3 log concentration (clamped to [-4, 6]), 5c+4 mixture logit.
+ * A training pair is CUSLINES_PAIR_FLOATS floats: the 40 inputs, the accepted
+ * candidate as a unit vector (tangent, k1/MAX_CURVATURE, k2/MAX_CURVATURE)
+ * and one float of padding.
+ */
+#define CUSLINES_SH_COEFFS 28
+#define CUSLINES_MLP_IN 40
+#define CUSLINES_MLP_HIDDEN 128
+#define CUSLINES_MLP_OUT 40
+#define CUSLINES_MLP_COMPONENTS 8
+#define CUSLINES_MLP_NPARAM 26920
+#define CUSLINES_PAIR_FLOATS 44
+
 typedef struct cuslines_tracker cuslines_tracker;
 
 /* ABI version of the loaded library (CUSLINES_ABI_VERSION). */
@@ -155,6 +180,52 @@ int cuslines_set_lanes_per_seed(cuslines_tracker *t, int lanes);
  * HBM allows (nverts x (X+1)(Y+1)(Z+1) x 32 B); 0 gathers the 8 texels from
  * the vertex-major ODF.  Results are identical either way. */
 int cuslines_set_ptt_layout(cuslines_tracker *t, int corner_blocks);
+
+/* SH coefficients of the (PTT-prepared) ODF, host float32
+ * [dimx][dimy][dimz][ncoef] with ncoef == CUSLINES_SH_COEFFS (order 6):
+ * the first 28 inputs of the learned proposal, trilinearly interpolated.
+ * No counterpart in the reference. */
+int cuslines_set_sh_coeffs(cuslines_tracker *t, const float *sh, int32_t ncoef);
+
+/* Learned proposal for PTT phase 2: host bf16 parameters (uint16 bit
+ * patterns, CUSLINES_MLP_NPARAM of them, layout above); NULL returns to the
+ * plain getter.  Needs cuslines_set_sh_coeffs.  The network is evaluated once
+ * per output step (every STEP_FRAC sub-steps) and its mixture proposes the
+ * candidates of each sub-step; a streamline whose learned rounds fail more
+ * than one time in four (after 16 sub-steps) continues with the plain
+ * getter.  No counterpart in the reference: it is for callers that trade the
+ * reference's face-CDF proposal for fewer probe taps. */
+int cuslines_set_proposal(cuslines_tracker *t, const uint16_t *mlp_bf16);
+
+/* Record training pairs during later plain PTT phase-2 calls: each accepted
+ * non-initial sub-step is kept with probability `rate` (a hash of seed, pass
+ * and sub-step), appended at atomicAdd(count_d) into pairs_d
+ * [capacity][CUSLINES_PAIR_FLOATS] while there is room (count_d keeps
+ * counting past capacity).  pairs_d == NULL stops recording.  Needs
+ * cuslines_set_sh_coeffs.  No counterpart in the reference. */
+int cuslines_set_recording(cuslines_tracker *t, float *pairs_d, int32_t capacity, int32_t *count_d,
+                           float rate);
+
+/* Proposal counters since the last reset: out[0] learned sub-steps accepted
+ * in their learned round, out[1] learned rounds that fell back to the plain
+ * getter, out[2] plain sub-steps of streamlines that left the learned
+ * proposal.  reset != 0 zeroes them after reading. */
+int cuslines_proposal_stats(cuslines_tracker *t, int64_t *out3, int reset);
+
+/* Training of the proposal network on device buffers (no counterpart in the
+ * reference).  params_d: float32 master parameters [CUSLINES_MLP_NPARAM];
+ * the forward pass rounds them to bf16 as inference does.
+ * cuslines_mlp_loss_grad_async: mean negative log-likelihood of the targets
+ * of n pairs under the predicted mixture into loss_d[0], and its gradient
+ * into grad_d [CUSLINES_MLP_NPARAM]; work_d is a caller buffer of
+ * cuslines_mlp_workspace_bytes(n) bytes.
+ * cuslines_mlp_adam_async: one Adam step (Kingma and Ba, 2015) with step
+ * count `step` (>= 1); m_d and v_d start at zero. */
+int cuslines_mlp_workspace_bytes(int32_t n, size_t *bytes);
+int cuslines_mlp_loss_grad_async(cuslines_tracker *t, const float *params_d, const float *pairs_d, int32_t n,
+                                 float *loss_d, float *grad_d, void *work_d);
+int cuslines_mlp_adam_async(cuslines_tracker *t, float *params_d, const float *grad_d, float *m_d, float *v_d,
+                            int32_t step, float lr, float beta1, float beta2, float eps);
 
 /* Device memory helpers for callers that keep seeds and outputs resident. */
 int cuslines_device_alloc(cuslines_tracker *t, size_t bytes, void **dptr);

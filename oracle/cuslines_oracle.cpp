@@ -91,6 +91,32 @@ float ulog(float x) {
     return std::fmaf(fe, 0.693145751953125f, std::fmaf(fe, 1.4286068203094172e-06f, l1p));
 }
 
+// e^x: n = rint(x log2 e), r = x - n ln2 (two-part split), Taylor
+// polynomial of e^r to r^7 on |r| <= ln2/2, times 2^n built from exponent
+// bits.  0 below -87.3, +inf above 88.7.
+inline float bits_float(uint32_t b) {
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f;
+}
+float uexp(float x) {
+    if (x < -87.3f) return 0.0f;
+    if (x > 88.7f) return INFINITY;
+    const float n = std::rint(x * 1.44269504f);
+    float r = std::fmaf(-n, 0.693145751953125f, x);
+    r = std::fmaf(-n, 1.4286068203094172e-06f, r);
+    float p = std::fmaf(r, 1.98412698e-04f, 1.38888889e-03f);
+    p = std::fmaf(r, p, 8.33333333e-03f);
+    p = std::fmaf(r, p, 4.16666667e-02f);
+    p = std::fmaf(r, p, 1.66666667e-01f);
+    p = std::fmaf(r, p, 0.5f);
+    p = std::fmaf(r, p, 1.0f);
+    p = std::fmaf(r, p, 1.0f);
+    const int e = (int)n + 127;
+    if (e <= 0) return p * 1.17549435e-38f * bits_float((uint32_t)(e + 126) << 23);
+    return p * bits_float((uint32_t)e << 23);
+}
+
 // sin and cos of |x| < ~1e4: q = rint(x*2/pi), r = x - q*pi/2 with a
 // three-part Cody-Waite split of pi/2, Taylor polynomials on |r| <= pi/4
 // (sin to r^11, cos to r^12), quadrant rotation.
@@ -251,6 +277,10 @@ struct Problem {
     float cos_max_angle = 0, cos_min_sep = 0;
     double step_d = 0;  // STEP_SIZE as the double literal the macro holds
     int nx = 1, ny = 1, nz = 1;
+    // learned PTT proposal (no counterpart in the reference)
+    std::vector<float> sh;         // [X][Y][Z][28]
+    std::vector<uint16_t> mlp;     // bf16 parameters; empty: plain getter
+    mutable long long stats[3] = {0, 0, 0};
 };
 
 // Lane-strided FMA partial sums reduced by the shuffle tree that lane 0 sees
@@ -439,6 +469,8 @@ struct SeedScratch {
     std::vector<int> iscr, peaks;
     Philox rng[LANES];
     float frame[18];
+    float mix[8 * 12];
+    long long stats[3] = {0, 0, 0};
     void size(const Problem &P) {
         const size_t n = (size_t)std::max(P.p.dimt, P.p.nverts) + 64;
         a.assign(n, 0.f); b.assign(n, 0.f); c.assign(n, 0.f); d.assign(n, 0.f);
@@ -866,6 +898,133 @@ bool init_frame_ptt(const Problem &P, SeedScratch &S, const float *first_step, c
 }
 
 // ---------------------------------------------------------------------------
+// Learned PTT proposal (no counterpart in the reference; the layout and
+// semantics are those of include/cuslines_hip.h).  Same operation order as
+// the kernels: neurons are fma chains over their inputs in order starting
+// from the bias; the mixture draw is the vMF sampler of Wood (1994).
+// ---------------------------------------------------------------------------
+constexpr int MLP_SH = CUSLINES_SH_COEFFS, MLP_IN = CUSLINES_MLP_IN, MLP_H = CUSLINES_MLP_HIDDEN,
+              MLP_OUT = CUSLINES_MLP_OUT, MLP_K = CUSLINES_MLP_COMPONENTS;
+constexpr int MLP_W1 = 0, MLP_B1 = MLP_W1 + MLP_H * MLP_IN, MLP_W2 = MLP_B1 + MLP_H,
+              MLP_B2 = MLP_W2 + MLP_H * MLP_H, MLP_W3 = MLP_B2 + MLP_H, MLP_B3 = MLP_W3 + MLP_OUT * MLP_H;
+constexpr int MIX_STRIDE = 12;
+
+inline float bf16f(uint16_t h) { return bits_float((uint32_t)h << 16); }
+
+void mlp_features(const Problem &P, const float *pt, const float *dir, const float *frame, float *x) {
+    if (trilinear(P, P.sh.data(), MLP_SH, pt, x) != 0)
+        for (int j = 0; j < MLP_SH; ++j) x[j] = 0.0f;
+    for (int j = 0; j < 3; ++j) x[MLP_SH + j] = dir[j];
+    for (int j = 0; j < 9; ++j) x[MLP_SH + 3 + j] = frame[j];
+}
+
+float mlp_neuron(const uint16_t *w, uint16_t bias, const float *in, int K) {
+    float acc = bf16f(bias);
+    for (int k = 0; k < K; ++k) acc = std::fmaf(bf16f(w[k]), in[k], acc);
+    return acc;
+}
+
+void mlp_forward(const uint16_t *W, const float *x, float *o) {
+    float h1[MLP_H], h2[MLP_H];
+    for (int j = 0; j < MLP_H; ++j) {
+        const float v = mlp_neuron(W + MLP_W1 + j * MLP_IN, W[MLP_B1 + j], x, MLP_IN);
+        h1[j] = (v > 0.0f) ? v : 0.0f;
+    }
+    for (int j = 0; j < MLP_H; ++j) {
+        const float v = mlp_neuron(W + MLP_W2 + j * MLP_H, W[MLP_B2 + j], h1, MLP_H);
+        h2[j] = (v > 0.0f) ? v : 0.0f;
+    }
+    for (int j = 0; j < MLP_OUT; ++j) o[j] = mlp_neuron(W + MLP_W3 + j * MLP_H, W[MLP_B3 + j], h2, MLP_H);
+}
+
+// Per component: unit mean (fallback +tangent), orthonormal (e1, e2) normal
+// to it, kappa = exp(clamp(., -4, 6)), exp(-2 kappa), cumulative softmax
+// weight summed in component order.
+void mixture_setup(const float *o, float *mix) {
+    float amax = o[4];
+    for (int c = 1; c < MLP_K; ++c) amax = std::fmax(amax, o[5 * c + 4]);
+    for (int c = 0; c < MLP_K; ++c) {
+        const float *oc = o + 5 * c;
+        float mx = oc[0], my = oc[1], mz = oc[2];
+        const float n2 = std::fmaf(mz, mz, std::fmaf(my, my, mx * mx));
+        if (n2 > 1e-30f && n2 <= REAL_MAX) {
+            const float inv = rsqrt_nr(n2);
+            mx *= inv; my *= inv; mz *= inv;
+        } else {
+            mx = 1.0f; my = 0.0f; mz = 0.0f;
+        }
+        const float ax = (std::fabs(mx) < 0.9f) ? 1.0f : 0.0f, ay = 1.0f - ax;
+        const float am = std::fmaf(ay, my, ax * mx);
+        float e1x = std::fmaf(-am, mx, ax), e1y = std::fmaf(-am, my, ay), e1z = -am * mz;
+        const float inv1 = rsqrt_nr(std::fmaf(e1z, e1z, std::fmaf(e1y, e1y, e1x * e1x)));
+        e1x *= inv1; e1y *= inv1; e1z *= inv1;
+        float *mc = mix + MIX_STRIDE * c;
+        mc[0] = mx; mc[1] = my; mc[2] = mz;
+        mc[3] = e1x; mc[4] = e1y; mc[5] = e1z;
+        mc[6] = std::fmaf(my, e1z, -(mz * e1y));
+        mc[7] = std::fmaf(mz, e1x, -(mx * e1z));
+        mc[8] = std::fmaf(mx, e1y, -(my * e1x));
+        const float kap = uexp(std::fmin(std::fmax(oc[3], -4.0f), 6.0f));
+        mc[9] = kap;
+        mc[10] = uexp(-2.0f * kap);
+        mc[11] = uexp(oc[4] - amax);
+    }
+    float c = 0.0f;
+    for (int k = 0; k < MLP_K; ++k) {
+        c = c + mix[MIX_STRIDE * k + 11];
+        mix[MIX_STRIDE * k + 11] = c;
+    }
+}
+
+bool mixture_draw(const Problem &P, const float *mix, Philox &rng, float *k1, float *k2) {
+    const float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
+    const float tgt = u0 * mix[MIX_STRIDE * (MLP_K - 1) + 11];
+    int c = 0;
+    for (; c < MLP_K - 1; ++c)
+        if (mix[MIX_STRIDE * c + 11] >= tgt) break;
+    const float *mc = mix + MIX_STRIDE * c;
+    float w = 1.0f + ulog(std::fmaf(1.0f - u1, mc[10], u1)) / mc[9];
+    w = std::fmin(std::fmax(w, -1.0f), 1.0f);
+    const float sw = std::sqrt(std::fmax(std::fmaf(-w, w, 1.0f), 0.0f));
+    float sn, cs;
+    usincos(6.28318531f * u2, &sn, &cs);
+    const float a1 = sw * cs, a2 = sw * sn;
+    const float ut = std::fmaf(a2, mc[6], std::fmaf(a1, mc[3], w * mc[0]));
+    const float un = std::fmaf(a2, mc[7], std::fmaf(a1, mc[4], w * mc[1]));
+    const float ub = std::fmaf(a2, mc[8], std::fmaf(a1, mc[5], w * mc[2]));
+    *k1 = P.p.max_curvature * un;
+    *k2 = P.p.max_curvature * ub;
+    return ut > 0.0f;
+}
+
+// One learned round of 32 candidates (lane order, reference acceptance
+// rule).  1 with new_dir and the frame advanced, or -1 (no winner).
+int get_direction_ptt_learned(const Problem &P, SeedScratch &S, float *frame, const float *pos, float *new_dir) {
+    const double min_support = (double)P.p.probe_quality * PMF_THRESHOLD_P;
+    const float first_val = interp4(P, pos, frame);
+    float k1s[LANES], k2s[LANES];
+    int win = -1;
+    for (int t = 0; t < LANES; ++t) {
+        float sup = -1.0f;
+        if (mixture_draw(P, S.mix, S.rng[t], &k1s[t], &k2s[t])) {
+            float pf[9];
+            for (int i = 0; i < 9; ++i) pf[i] = frame[i];
+            sup = data_support(P, first_val, pos, k1s[t], k2s[t], pf);
+        }
+        if (win < 0 && (double)sup >= min_support) win = t;
+    }
+    if (win < 0) return -1;
+    float prop[9], d[3], pf[9];
+    prepare_propagator(k1s[win], k2s[win], (float)(P.step_d / P.p.step_frac), prop);
+    for (int i = 0; i < 9; ++i) pf[i] = frame[i];
+    propagate_frame(prop, pf, d);
+    norm3<0>(d);
+    new_dir[0] = d[0]; new_dir[1] = d[1]; new_dir[2] = d[2];
+    for (int i = 0; i < 9; ++i) frame[i] = pf[i];
+    return 1;
+}
+
+// ---------------------------------------------------------------------------
 // Tracker (generate_streamlines_cuda.cu:277-421, boot.cu:720-824)
 // ---------------------------------------------------------------------------
 int tracker(const Problem &P, SeedScratch &S, const float *seed, const float *first_step,
@@ -878,11 +1037,34 @@ int tracker(const Problem &P, SeedScratch &S, const float *seed, const float *fi
     float dir[3] = {first_step[0], first_step[1], first_step[2]};
     sl[0] = pt[0]; sl[1] = pt[1]; sl[2] = pt[2];
     int tissue = CUSLINES_TRACKPOINT;
+    const bool use_mlp = (model == CUSLINES_PTT) && !P.mlp.empty();
+    bool learned = use_mlp;
+    int ltries = 0, lfalls = 0, nplain = 0;
     int i;
     for (i = 1; i < MAXL * step_frac; ++i) {
         float nd[3];
         int ndir;
         if (model == CUSLINES_PROB) ndir = get_direction_prob(P, S, false, dir, pt, nd, nullptr);
+        else if (model == CUSLINES_PTT && use_mlp) {
+            // learned proposal: the network once per output step, plain
+            // getter when a round fails or the streamline left the proposal
+            ndir = -1;
+            if (learned) {
+                if ((i - 1) % step_frac == 0) {
+                    float x[MLP_IN], o[MLP_OUT];
+                    mlp_features(P, pt, dir, ptt_frame, x);
+                    mlp_forward(P.mlp.data(), x, o);
+                    mixture_setup(o, S.mix);
+                }
+                ndir = get_direction_ptt_learned(P, S, ptt_frame, pt, nd);
+                ++ltries;
+                if (ndir < 0) ++lfalls;
+                if (ltries >= 16 && 4 * lfalls > ltries) learned = false;
+            } else {
+                ++nplain;
+            }
+            if (ndir < 0) ndir = get_direction_ptt(P, S, false, dir, ptt_frame, pt, nd);
+        }
         else if (model == CUSLINES_PTT) ndir = get_direction_ptt(P, S, false, dir, ptt_frame, pt, nd);
         else ndir = get_direction_boot(P, S, 5, dir, pt, nd, nullptr);
         if (ndir == 0) break;
@@ -897,6 +1079,9 @@ int tracker(const Problem &P, SeedScratch &S, const float *seed, const float *fi
             tissue == CUSLINES_OUTSIDEIMAGE)
             break;
     }
+    S.stats[0] += ltries - lfalls;
+    S.stats[1] += lfalls;
+    S.stats[2] += nplain;
     int nsteps = i / step_frac;
     if ((i % step_frac) != 0 && i < step_frac * (MAXL - 1)) {
         ++nsteps;
@@ -974,6 +1159,38 @@ int oracle_set_boot_matrices(oracle_problem *h, const float *H, const float *R, 
     P->delta_q.assign(dq, dq + (size_t)P->p.delta_nr * hr);
     P->smat.assign(sm, sm + (size_t)P->p.nverts * P->p.delta_nr);
     return 0;
+}
+int oracle_set_sh(oracle_problem *h, const float *sh) {
+    Problem *P = reinterpret_cast<Problem *>(h);
+    const size_t n = (size_t)P->p.dimx * P->p.dimy * P->p.dimz * MLP_SH;
+    P->sh.assign(sh, sh + n);
+    return 0;
+}
+int oracle_set_proposal(oracle_problem *h, const uint16_t *w) {
+    Problem *P = reinterpret_cast<Problem *>(h);
+    if (!w) P->mlp.clear();
+    else P->mlp.assign(w, w + CUSLINES_MLP_NPARAM);
+    return 0;
+}
+int oracle_proposal_stats(oracle_problem *h, long long *out, int reset) {
+    Problem *P = reinterpret_cast<Problem *>(h);
+    for (int q = 0; q < 3; ++q) {
+        out[q] = P->stats[q];
+        if (reset) P->stats[q] = 0;
+    }
+    return 0;
+}
+// Unit entry points of the learned proposal for the oracle's own tests.
+void oracle_mlp_forward(const uint16_t *w, const float *x, float *o) { mlp_forward(w, x, o); }
+void oracle_mixture_setup(const float *o, float *mix) { mixture_setup(o, mix); }
+// n draws of lane stream (seed, subsequence): k1, k2 and validity.
+void oracle_mixture_draws(const float *mix, float max_curv, uint64_t seed, uint64_t sub, int n, float *k1,
+                          float *k2, int *valid) {
+    Problem P;
+    P.p.max_curvature = max_curv;
+    Philox rng;
+    rng.init(seed, sub);
+    for (int i = 0; i < n; ++i) valid[i] = mixture_draw(P, mix, rng, &k1[i], &k2[i]) ? 1 : 0;
 }
 int oracle_set_ptt_lut(oracle_problem *h, const int32_t *lut) {
     Problem *P = reinterpret_cast<Problem *>(h);
@@ -1072,6 +1289,8 @@ int oracle_phase2(oracle_problem *h, const float *seeds, int nseeds, int rng_chu
                 ++off;
             }
         }
+#pragma omp critical
+        for (int q = 0; q < 3; ++q) P.stats[q] += S.stats[q];
     }
     return 0;
 }
@@ -1091,6 +1310,7 @@ void oracle_philox_mixed(uint64_t seed, uint64_t subseq, int n, const int *kinds
 float oracle_log(float x) { return ulog(x); }
 void oracle_sincos(float x, float *s, float *c) { usincos(x, s, c); }
 float oracle_rsqrt(float x) { return rsqrt_nr(x); }
+float oracle_exp(float x) { return uexp(x); }
 int oracle_sortnet_partner(int layer, int lane) {
     const SortNet &s = sortnet();
     if (layer < 0 || layer >= s.nlayers || lane < 0 || lane >= 32) return -1;

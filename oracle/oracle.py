@@ -48,6 +48,9 @@ def lib():
             ("oracle_trilinear", [vp, fp, fp]),
             ("oracle_check_point", [vp, fp]),
             ("oracle_boot_odf", [vp, fp, C.c_int, C.c_int, fp, fp]),
+            ("oracle_set_sh", [vp, fp]),
+            ("oracle_set_proposal", [vp, C.POINTER(C.c_uint16)]),
+            ("oracle_proposal_stats", [vp, C.POINTER(C.c_longlong), C.c_int]),
         ]:
             getattr(L, name).restype = C.c_int
             getattr(L, name).argtypes = args
@@ -56,6 +59,8 @@ def lib():
         L.oracle_log.restype = C.c_float
         L.oracle_log.argtypes = [C.c_float]
         L.oracle_sincos.argtypes = [C.c_float, fp, fp]
+        L.oracle_exp.restype = C.c_float
+        L.oracle_exp.argtypes = [C.c_float]
         L.oracle_rsqrt.restype = C.c_float
         L.oracle_rsqrt.argtypes = [C.c_float]
         L.oracle_sortnet_partner.restype = C.c_int
@@ -68,6 +73,9 @@ def lib():
         L.oracle_prepare_propagator.argtypes = [C.c_float, C.c_float, C.c_float, fp]
         L.oracle_propagate_frame.argtypes = [fp, fp, fp]
         L.oracle_num_threads.restype = C.c_int
+        L.oracle_mlp_forward.argtypes = [C.POINTER(C.c_uint16), fp, fp]
+        L.oracle_mixture_setup.argtypes = [fp, fp]
+        L.oracle_mixture_draws.argtypes = [fp, C.c_float, C.c_uint64, C.c_uint64, C.c_int, fp, fp, ip]
         _lib = L
     return _lib
 
@@ -162,6 +170,22 @@ class Oracle:
         pt = np.ascontiguousarray(pt, dtype=np.float32)
         return self.L.oracle_check_point(self.h, _f(pt))
 
+    def set_learned(self, sh, mlp_bf16):
+        """Learned PTT proposal: SH coefficients [X,Y,Z,28] and bf16
+        parameters (uint16), or mlp_bf16=None for the plain getter."""
+        self._sh = np.ascontiguousarray(sh, dtype=np.float32)
+        self.L.oracle_set_sh(self.h, _f(self._sh))
+        if mlp_bf16 is None:
+            self.L.oracle_set_proposal(self.h, None)
+        else:
+            w = np.ascontiguousarray(mlp_bf16, dtype=np.uint16)
+            self.L.oracle_set_proposal(self.h, w.ctypes.data_as(C.POINTER(C.c_uint16)))
+
+    def proposal_stats(self, reset=True):
+        out = (C.c_longlong * 3)()
+        self.L.oracle_proposal_stats(self.h, out, int(reset))
+        return np.array(list(out), dtype=np.int64)
+
     def boot_odf(self, pt, slot=0, phase=0):
         """One bootstrap refit at pt: (status, odf[nverts], signal[dimt])."""
         pt = np.ascontiguousarray(pt, dtype=np.float32)
@@ -197,6 +221,34 @@ def usincos(x):
     s, c = C.c_float(), C.c_float()
     lib().oracle_sincos(float(x), C.byref(s), C.byref(c))
     return s.value, c.value
+
+
+def mlp_forward(mlp_bf16, x):
+    w = np.ascontiguousarray(mlp_bf16, dtype=np.uint16)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    o = np.zeros(40, dtype=np.float32)
+    lib().oracle_mlp_forward(w.ctypes.data_as(C.POINTER(C.c_uint16)), _f(x), _f(o))
+    return o
+
+
+def mixture_setup(o):
+    o = np.ascontiguousarray(o, dtype=np.float32)
+    mix = np.zeros(96, dtype=np.float32)
+    lib().oracle_mixture_setup(_f(o), _f(mix))
+    return mix.reshape(8, 12)
+
+
+def mixture_draws(mix, max_curv, seed, sub, n):
+    mix = np.ascontiguousarray(mix, dtype=np.float32).reshape(-1)
+    k1 = np.zeros(n, np.float32)
+    k2 = np.zeros(n, np.float32)
+    ok = np.zeros(n, np.int32)
+    lib().oracle_mixture_draws(_f(mix), max_curv, seed, sub, n, _f(k1), _f(k2), _i(ok))
+    return k1, k2, ok.astype(bool)
+
+
+def uexp(x):
+    return float(lib().oracle_exp(float(x)))
 
 
 def rsqrt(x):

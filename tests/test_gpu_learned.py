@@ -1,0 +1,197 @@
+"""Learned PTT proposal on the GPU: bit-exact against the oracle's restatement
+for fixed weights, the training gradient against the float64 objective, a
+short training run on pairs the plain getter records, what the trained
+network gains over the untrained one, and the tracking statistics of the
+learned sampler against the plain getter on the same seeds."""
+
+import numpy as np
+import pytest
+
+from conftest import packed_points
+
+pytestmark = pytest.mark.gpu
+
+
+def _sh(w):
+    from gpustreamlines_amd import learned as LP
+
+    key = "_sh"
+    if key not in w.extra:
+        w.extra[key] = LP.sh_coefficients(w.dataf, w.vertices)
+    return w.extra[key]
+
+
+def _tracker(w):
+    from gpustreamlines_amd._device import DeviceTracker
+
+    dt = DeviceTracker(w.params, device=0)
+    dt.upload(w.dataf, w.stop_map, w.vertices, w.edges, ptt_lut=w.ptt_lut)
+    dt.set_sh_coeffs(_sh(w))
+    return dt
+
+
+def test_learned_bit_exact_vs_oracle(small_workloads):
+    from gpustreamlines_amd import learned as LP
+    from gpustreamlines_amd import workloads as W
+    from oracle.oracle import Oracle
+
+    w = small_workloads["ptt"]
+    seeds = W.seeds_for(w, 48, placement="mixed")
+    wts = LP.to_bf16(LP.init_params(11))
+    print("[learned] bit-exact, 48 seeds", flush=True)
+    dt = _tracker(w)
+    try:
+        dt.set_proposal(wts)
+        g = dt.track(seeds, keep_dirs=True)
+        gstats = dt.proposal_stats()
+    finally:
+        dt.close()
+    o = Oracle(w.params, w.dataf, w.stop_map, w.vertices, w.edges, ptt_lut=w.ptt_lut)
+    o.set_learned(_sh(w), wts)
+    r = o.track(seeds)
+    ostats = o.proposal_stats()
+    assert gstats.sum() > 0 and gstats[0] > 0
+    assert np.array_equal(gstats, ostats), (gstats, ostats)
+    assert np.array_equal(g["sline_len"], r["sline_len"])
+    assert np.array_equal(g["points"], packed_points(r))
+
+
+def test_training_gradient_matches_float64(small_workloads):
+    from gpustreamlines_amd import learned as LP
+
+    w = small_workloads["ptt"]
+    rng = np.random.default_rng(0)
+    n = 512
+    X = rng.normal(size=(n, LP.N_IN)).astype(np.float32)
+    a = rng.uniform(-0.5, 0.5, size=(n, 2))
+    U = np.column_stack([np.sqrt(1 - (a ** 2).sum(1)), a]).astype(np.float32)
+    pairs = np.zeros((n, 44), np.float32)
+    pairs[:, :40] = X
+    pairs[:, 40:43] = U
+    p = LP.init_params(4)
+    p[LP.W3:LP.B3] *= 10
+    dt = _tracker(w)
+    try:
+        pd = dt.alloc(pairs.nbytes)
+        pd.upload(pairs)
+        tr = LP.Trainer(dt, p, batch=n)
+        loss, g = tr.loss_grad(pd.ptr, n)
+        tr.free()
+        pd.free()
+    finally:
+        dt.close()
+    pr = LP.from_bf16(LP.to_bf16(p)).astype(np.float64)
+    want_loss = LP.loss_np(pr, X.astype(np.float64), U.astype(np.float64))
+    want = LP.grad_np(pr, X.astype(np.float64), U.astype(np.float64))
+    assert abs(loss - want_loss) < 1e-4 * max(1.0, abs(want_loss))
+    for s, e in [(LP.W1, LP.B1), (LP.B1, LP.W2), (LP.W2, LP.B2), (LP.B2, LP.W3), (LP.W3, LP.B3), (LP.B3, LP.NPARAM)]:
+        scale = np.abs(want[s:e]).max()
+        np.testing.assert_allclose(g[s:e], want[s:e], atol=2e-3 * scale, rtol=0)
+
+
+@pytest.fixture(scope="module")
+def trained(small_workloads):
+    """Pairs recorded by the plain getter on 2,000 seeds, 3 epochs of Adam."""
+    from gpustreamlines_amd import learned as LP
+    from gpustreamlines_amd import workloads as W
+
+    w = small_workloads["ptt"]
+    seeds = W.seeds_for(w, 2000, placement="mask", seed=5)
+    print("[learned] recording + training", flush=True)
+    dt = _tracker(w)
+    try:
+        pairs, n = LP.record_pairs(dt, seeds, capacity=400_000, rate=0.25)
+        host = pairs.download(np.zeros((n, 44), np.float32))
+        tr = LP.Trainer(dt, batch=4096)
+        l0 = tr.loss_grad(pairs.ptr, 4096)[0]
+        hist = tr.fit(pairs, n, epochs=3, lr=2e-3)
+        params = tr.params()
+        tr.free()
+        pairs.free()
+    finally:
+        dt.close()
+    return dict(n=n, pairs=host, loss0=l0, hist=hist, params=params)
+
+
+def test_recorded_pairs_are_well_formed(trained):
+    P = trained["pairs"]
+    assert trained["n"] > 50_000
+    assert np.isfinite(P).all()
+    u = P[:, 40:43].astype(np.float64)
+    np.testing.assert_allclose((u ** 2).sum(1), 1.0, atol=1e-5)
+    assert (u[:, 0] >= 0).all()
+    f = P[:, 31:40].reshape(-1, 3, 3).astype(np.float64)  # frame rows are unit vectors
+    np.testing.assert_allclose(np.linalg.norm(f, axis=2), 1.0, atol=1e-4)
+
+
+def _ks_stat(a, b):
+    from scipy.stats import ks_2samp
+
+    return ks_2samp(a, b).statistic
+
+
+def test_training_lowers_loss_and_improves_fidelity(small_workloads, trained):
+    """What the trained network gains over the method without it: lower
+    loss, and streamline lengths closer (two-sample KS distance) to the plain
+    getter's on the same seeds than with the untrained network, at a high
+    first-round acceptance."""
+    from gpustreamlines_amd import learned as LP
+    from gpustreamlines_amd import workloads as W
+
+    assert trained["hist"][-1] < trained["loss0"] - 0.5
+    w = small_workloads["ptt"]
+    seeds = W.seeds_for(w, 1500, placement="mixed", seed=9)
+    lens, rates = {}, {}
+    dt = _tracker(w)
+    try:
+        for name, p in (("plain", None), ("untrained", LP.init_params()), ("trained", trained["params"])):
+            dt.set_proposal(None if p is None else LP.to_bf16(p))
+            lens[name] = dt.track(seeds)["sline_len"]
+            s = dt.proposal_stats()
+            rates[name] = s[0] / max(1, s[0] + s[1])
+            print(f"[learned] {name}: mean length {lens[name].mean():.1f} accepted {s[0]} fell back {s[1]} "
+                  f"plain {s[2]}", flush=True)
+    finally:
+        dt.close()
+    d_untrained = _ks_stat(lens["plain"], lens["untrained"])
+    d_trained = _ks_stat(lens["plain"], lens["trained"])
+    print(f"[learned] KS distance to plain: untrained {d_untrained:.4f} trained {d_trained:.4f}", flush=True)
+    assert d_trained < d_untrained
+    assert rates["trained"] > 0.8
+
+
+def test_learned_statistics_match_plain(small_workloads, trained):
+    """Same seeds, plain getter vs learned proposal: streamline count, the
+    length distribution (two-sample KS) and voxel visit density."""
+    from scipy.stats import ks_2samp
+
+    from gpustreamlines_amd import learned as LP
+    from gpustreamlines_amd import workloads as W
+
+    w = small_workloads["ptt"]
+    seeds = W.seeds_for(w, 3000, placement="mixed", seed=13)
+    res = {}
+    dt = _tracker(w)
+    try:
+        for name, p in (("plain", None), ("learned", trained["params"])):
+            dt.set_proposal(None if p is None else LP.to_bf16(p))
+            res[name] = dt.track(seeds)
+    finally:
+        dt.close()
+    a, b = res["plain"], res["learned"]
+    na, nb = len(a["sline_len"]), len(b["sline_len"])
+    assert abs(na - nb) <= 0.02 * na
+    ks = ks_2samp(a["sline_len"], b["sline_len"])
+    shape = tuple(w.params[k] for k in ("dimx", "dimy", "dimz"))
+
+    def density(r):
+        idx = np.clip(np.rint(r["points"]).astype(int), 0, np.array(shape) - 1)
+        d = np.zeros(shape)
+        np.add.at(d, tuple(idx.T), 1)
+        return d.ravel()
+
+    corr = np.corrcoef(density(a), density(b))[0, 1]
+    print(f"[learned] streamlines {na} vs {nb}, mean length {a['sline_len'].mean():.1f} vs "
+          f"{b['sline_len'].mean():.1f}, KS p={ks.pvalue:.3g}, density r={corr:.4f}", flush=True)
+    assert corr > 0.98
+    assert ks.pvalue > 0.05
