@@ -103,6 +103,85 @@ def cpu_baseline(w, seeds, n_sample):
                        f"streamlines) in {dt:.1f} s")
 
 
+DENSITY_SLINES = 20000
+
+
+def _density(slots, lens, shape):
+    """Voxel visit counts of the points of the given padded slots."""
+    pts = np.concatenate([slots[i, :n] for i, n in enumerate(lens)])
+    idx = np.clip(np.rint(pts).astype(np.int64), 0, np.array(shape) - 1)
+    d = np.zeros(shape)
+    np.add.at(d, tuple(idx.T), 1)
+    return d.ravel()
+
+
+def measure_learned(args, w, dt, step, plain_lens, sld, sl, nsl, L):
+    """Train the BF16 proposal network on the GPU from pairs the plain getter
+    records (20,000 seeds in the mask, 5% of accepted sub-steps, 4 epochs of
+    Adam), then time the same pass with the learned proposal.  Compares the
+    streamline lengths (two-sample KS) and, on the first 20,000 streamlines,
+    the voxel visit density with the plain pass."""
+    from scipy.stats import ks_2samp
+
+    from gpustreamlines_amd import learned as LP
+    from gpustreamlines_amd import workloads as W
+
+    shape = tuple(w.params[k] for k in ("dimx", "dimy", "dimz"))
+    k = min(DENSITY_SLINES, nsl)
+    plain_slots = sl.download(np.zeros((k, 2 * L, 3), np.float32))
+    d_plain = _density(plain_slots, plain_lens[:k], shape)
+    del plain_slots
+    t0 = time.perf_counter()
+    pairs, npairs = LP.record_pairs(dt, W.seeds_for(w, 20000, placement="mask", seed=7), capacity=3_000_000,
+                                    rate=0.05, rng_chunk=CHUNK)
+    tr = LP.Trainer(dt, batch=4096)
+    loss0 = tr.loss_grad(pairs.ptr, 4096)[0]
+    hist = tr.fit(pairs, npairs, epochs=4, lr=2e-3)
+    params = tr.params()
+    tr.free()
+    pairs.free()
+    t_train = time.perf_counter() - t0
+    print(f"[bench] learned proposal: {npairs} pairs, loss {loss0:.3f} -> {hist[-1]:.3f}, {t_train:.1f}s",
+          file=sys.stderr, flush=True)
+    dt.set_proposal(LP.to_bf16(params))
+    for i in range(args.warmup):
+        r = step()
+        print(f"[bench] learned warmup {i}: {r[3] / 1e3:.2f}s", file=sys.stderr, flush=True)
+    dt.proposal_stats(reset=True)
+    times = []
+    for i in range(args.steps):
+        r = step()
+        times.append(r)
+        print(f"[bench] learned step {i}: phase2 {r[2]:.1f} ms", file=sys.stderr, flush=True)
+    stats = dt.proposal_stats(reset=True) // max(1, args.steps)
+    dt.set_proposal(None)
+    lens = sld.download(np.zeros(nsl, dtype=np.int32)).astype(np.int64)
+    slots = sl.download(np.zeros((k, 2 * L, 3), np.float32))
+    d_learned = _density(slots, lens[:k], shape)
+    del slots
+    t_step = float(np.mean([t[3] for t in times])) / 1e3
+    steps = int((lens - 1).clip(min=0).sum())
+    ks = ks_2samp(plain_lens, lens)
+    return {
+        "steps_per_s": steps / t_step,
+        "ms_per_step": t_step * 1e3,
+        "phase2_ms": float(np.mean([t[2] for t in times])),
+        "points": int(lens.sum()),
+        "first_round_accepted_substeps": int(stats[0]),
+        "fallback_substeps": int(stats[1]),
+        "plain_substeps_after_switch": int(stats[2]),
+        "first_round_acceptance": float(stats[0] / max(1, stats[0] + stats[1])),
+        "mean_length_plain": float(plain_lens.mean()),
+        "mean_length_learned": float(lens.mean()),
+        "ks_length_statistic": float(ks.statistic),
+        "ks_length_p": float(ks.pvalue),
+        "density_corr_first_20k": float(np.corrcoef(d_plain, d_learned)[0, 1]),
+        "train_pairs": int(npairs),
+        "train_loss": [float(loss0)] + [float(h) for h in hist],
+        "train_s": t_train,
+    }
+
+
 def copy_calibration(nbytes):
     """One hipMemcpy device-to-device of nbytes: a plain copy kernel of known
     size, whose FETCH_SIZE/WRITE_SIZE scale the tracking kernels' counters."""
@@ -128,6 +207,8 @@ def main():
     ap.add_argument("--placement", default="mixed", choices=["mixed", "mask"])
     ap.add_argument("--cpu-sample", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--proposal", default="both", choices=["plain", "learned", "both"],
+                    help="PTT: time the plain getter, the learned proposal, or both (value = plain)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -195,6 +276,13 @@ def main():
               flush=True)
     wall = time.perf_counter() - t0
     lens = sld.download(np.zeros(nsl, dtype=np.int32)).astype(np.int64)
+    learned = None
+    if args.getter == "ptt" and args.proposal != "plain":
+        try:
+            learned = measure_learned(args, w, dt, step, lens, sld, sl, nsl, L)
+        except Exception as exc:  # the plain line is still reported
+            print(f"[bench] learned proposal failed: {exc!r}", file=sys.stderr, flush=True)
+            learned = {"error": repr(exc)}
     steps_per_pass = int((lens - 1).clip(min=0).sum())
     t_step = np.array([t[3] for t in times]) / 1e3
     t_p2 = np.array([t[2] for t in times]) / 1e3
@@ -241,6 +329,12 @@ def main():
             "alg_bytes_per_launch": abytes * steps_per_pass,
         },
     }
+    if learned is not None:
+        res["learned"] = learned
+        if args.proposal == "learned" and "steps_per_s" in learned:
+            res["value"] = learned["steps_per_s"]
+            res["ms_per_step"] = learned["ms_per_step"]
+            res["config"]["proposal"] = "learned"
     traffic = measured_traffic(res["config"]["workload"], res["roofline"]["kernel"])
     if traffic is not None:
         res["roofline"]["traffic"] = traffic["traffic_bytes"]

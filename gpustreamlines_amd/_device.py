@@ -110,10 +110,6 @@ class DeviceTracker:
         _lib.check(self.L.cuslines_set_ptt_layout(self.h, int(bool(corner_blocks))), "set_ptt_layout")
 
     # -- learned PTT proposal -------------------------------------------------
-    def set_sh_coeffs(self, sh):
-        sh = np.ascontiguousarray(sh, dtype=np.float32)
-        _lib.check(self.L.cuslines_set_sh_coeffs(self.h, sh.ctypes.data, int(sh.shape[-1])), "set_sh_coeffs")
-
     def set_proposal(self, mlp_bf16):
         """bf16 parameters (uint16 array) of the proposal network, or None."""
         if mlp_bf16 is None:

@@ -15,9 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("CUSLINES_HIP_LIB") or os.path.join(HERE, "libcuslines_hip.so")
 
 ABI_VERSION = 1
-MLP_NPARAM = 26920  # CUSLINES_MLP_NPARAM
-PAIR_FLOATS = 44    # CUSLINES_PAIR_FLOATS
-SH_COEFFS = 28      # CUSLINES_SH_COEFFS
+MLP_NPARAM = 24856  # CUSLINES_MLP_NPARAM
+PAIR_FLOATS = 64    # CUSLINES_PAIR_FLOATS
 
 
 class cuslines_params(C.Structure):
@@ -100,7 +99,6 @@ def load():
         "cuslines_set_launch": [vp, C.c_int, C.c_int, C.c_int],
         "cuslines_set_ptt_layout": [vp, C.c_int],
         "cuslines_set_lanes_per_seed": [vp, C.c_int],
-        "cuslines_set_sh_coeffs": [vp, vp, i32],
         "cuslines_set_proposal": [vp, vp],
         "cuslines_set_recording": [vp, vp, i32, vp, C.c_float],
         "cuslines_proposal_stats": [vp, C.POINTER(C.c_int64), C.c_int],

@@ -115,7 +115,7 @@ struct DevBufs {
     const int *dwi_idx;
     const uint16_t *lut;
     const float *blocks;  // PTT corner blocks (nullptr: gather 8 texels)
-    const float *sh;       // SH coefficients of the ODF, [X][Y][Z][32] (28 used)
+    const float *fprop;    // learned proposal: per disc vertex, propagator of 4 probe steps [24][9]
     const uint16_t *mlp;   // proposal network, bf16 (MLP_NPARAM)
     float *rec;            // training pairs [rec_cap][PAIR_FLOATS]
     int *rec_count;
@@ -711,37 +711,19 @@ struct PttLds {
     float *frame;     // 18 floats: [0..8] backward frame, [9..17] forward frame
     float *vert_pdf;  // 32
     float *face_cdf;  // 32
-    float *mix;       // learned proposal: 8 components x MIX_STRIDE
-    float *act;       // learned proposal: inputs [64], hidden 1 [128], hidden 2 [128], outputs [64]
+    float *act;       // learned proposal: inputs [64], hidden 1 [128], hidden 2 [128], outputs [32]
 };
 
-// get_direction_ptt_d (ptt.cu:194-412); `frame` is the 9-float frame in use.
+// Face CDF over the 31 disc faces from S.vert_pdf (prefix_sum_sh_d, one
+// block of 32) and up to `rounds` rejection rounds of 32 candidates
+// (ptt.cu:270-410).  Returns 1 with the new direction, 0 when the CDF is
+// empty or no round has a winner.  `wk` (optional) receives the winner's
+// curvature.
 template <bool INIT>
-__device__ int get_direction_ptt(const DevParams &P, const DevBufs &B, PttLds &S, float *frame,
-                                 Philox &rng, float dx, float dy, float dz, float px, float py, float pz,
-                                 float *ndx, float *ndy, float *ndz, float *wk = nullptr) {
+__device__ int ptt_sample(const DevParams &P, const DevBufs &B, PttLds &S, float *frame, const float *fr,
+                          Philox &rng, float first_val, int rounds, float dx, float dy, float dz, float px, float py,
+                          float pz, float *ndx, float *ndy, float *ndz, float *wk) {
     const int l = lane32();
-    if (INIT) {
-        if (l == 0) { frame[0] = dx; frame[1] = dy; frame[2] = dz; }
-        WAVE_SYNC();
-    }
-    float fr[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) fr[i] = frame[i];
-    const float first_val = interp4(P, B, px, py, pz, fr[0], fr[1], fr[2]);
-    bool found = false;
-    if (l < DISC_VERT_CNT) {
-        const float k1 = (float)((double)c_disc_vert[2 * l] * (double)P.max_curv);
-        const float k2 = (float)((double)c_disc_vert[2 * l + 1] * (double)P.max_curv);
-        float pf[9];
-        probing_frame<INIT>(fr, rng, pf);
-        const float sup = data_support(P, B, first_val, px, py, pz, k1, k2, pf);
-        const bool ok = !(sup < P.min_support_f);
-        S.vert_pdf[l] = ok ? sup : 0.0f;
-        found = ok;
-    }
-    if (half_ballot(found) == 0u) return 0;
-    WAVE_SYNC();
     float face = 0.0f;
     if (l < DISC_FACE_CNT) {
 #pragma unroll
@@ -751,7 +733,6 @@ __device__ int get_direction_ptt(const DevParams &P, const DevBufs &B, PttLds &S
             face = face + vv;
         }
     }
-    // prefix_sum_sh_d over 31 faces: a single block of 32
 #pragma unroll
     for (int i = 1; i < 32; i <<= 1) {
         const float o = shfl_up32(face, i);
@@ -762,7 +743,7 @@ __device__ int get_direction_ptt(const DevParams &P, const DevBufs &B, PttLds &S
     WAVE_SYNC();
     if (last == 0.0f) return 0;
     const double mc = (double)P.max_curv;
-    for (int round = 0; round < 1024 / 32; ++round) {
+    for (int round = 0; round < rounds; ++round) {
         float r1 = rng.uniform(), r2 = rng.uniform();
         if (r1 + r2 > 1.0f) { r1 = 1.0f - r1; r2 = 1.0f - r2; }
         const float tmp = rng.uniform() * last;
@@ -816,6 +797,38 @@ __device__ int get_direction_ptt(const DevParams &P, const DevBufs &B, PttLds &S
     return 0;
 }
 
+// get_direction_ptt_d (ptt.cu:194-412); `frame` is the 9-float frame in use.
+// `raw` (optional, LDS [24]) receives the vertex supports before the 0.4 cut.
+template <bool INIT>
+__device__ int get_direction_ptt(const DevParams &P, const DevBufs &B, PttLds &S, float *frame,
+                                 Philox &rng, float dx, float dy, float dz, float px, float py, float pz,
+                                 float *ndx, float *ndy, float *ndz, float *wk = nullptr, float *raw = nullptr) {
+    const int l = lane32();
+    if (INIT) {
+        if (l == 0) { frame[0] = dx; frame[1] = dy; frame[2] = dz; }
+        WAVE_SYNC();
+    }
+    float fr[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) fr[i] = frame[i];
+    const float first_val = interp4(P, B, px, py, pz, fr[0], fr[1], fr[2]);
+    bool found = false;
+    if (l < DISC_VERT_CNT) {
+        const float k1 = (float)((double)c_disc_vert[2 * l] * (double)P.max_curv);
+        const float k2 = (float)((double)c_disc_vert[2 * l + 1] * (double)P.max_curv);
+        float pf[9];
+        probing_frame<INIT>(fr, rng, pf);
+        const float sup = data_support(P, B, first_val, px, py, pz, k1, k2, pf);
+        const bool ok = !(sup < P.min_support_f);
+        S.vert_pdf[l] = ok ? sup : 0.0f;
+        if (raw) raw[l] = sup;
+        found = ok;
+    }
+    if (half_ballot(found) == 0u) return 0;
+    WAVE_SYNC();
+    return ptt_sample<INIT>(P, B, S, frame, fr, rng, first_val, 1024 / 32, dx, dy, dz, px, py, pz, ndx, ndy, ndz, wk);
+}
+
 // init_frame_ptt_d (ptt.cu:415-474); frame[15..17] is set to the binormal.
 __device__ bool init_frame_ptt(const DevParams &P, const DevBufs &B, PttLds &S, Philox &rng,
                                float fx, float fy, float fz, float sx, float sy, float sz) {
@@ -842,31 +855,54 @@ __device__ bool init_frame_ptt(const DevParams &P, const DevBufs &B, PttLds &S, 
 // -------------------------------------------------------------------------
 // Learned PTT proposal (no counterpart in the reference).
 //
-// Inputs x[40]: SH coefficients of the ODF at the point (trilinear, 0 outside
-// the volume), the incoming direction, the 9-float frame.  The MLP runs on the
-// 32 lanes of the seed slot: lane l owns hidden neurons l, l+32, l+64, l+96
-// and outputs l, l+32; each neuron is an fma chain over its inputs in order,
-// starting from the bias.  Parameters are bf16, arithmetic fp32.
+// The network predicts the 24 vertex supports that the plain getter measures
+// with 8 probe taps each, from two taps per vertex; the reference's face CDF
+// and acceptance rule then run on the prediction.  The MLP runs on the 32
+// lanes of the seed slot: lane l owns hidden neurons l, l+32, l+64, l+96 and
+// output l; each neuron is an fma chain over its inputs in order, starting
+// from the bias.  Parameters are bf16, arithmetic fp32.
 // -------------------------------------------------------------------------
-constexpr int MLP_SH = CUSLINES_SH_COEFFS, MLP_IN = CUSLINES_MLP_IN, MLP_H = CUSLINES_MLP_HIDDEN,
-              MLP_OUT = CUSLINES_MLP_OUT, MLP_K = CUSLINES_MLP_COMPONENTS;
+constexpr int MLP_IN = CUSLINES_MLP_IN, MLP_H = CUSLINES_MLP_HIDDEN, MLP_OUT = CUSLINES_MLP_OUT;
 constexpr int MLP_W1 = 0, MLP_B1 = MLP_W1 + MLP_H * MLP_IN, MLP_W2 = MLP_B1 + MLP_H,
               MLP_B2 = MLP_W2 + MLP_H * MLP_H, MLP_W3 = MLP_B2 + MLP_H, MLP_B3 = MLP_W3 + MLP_OUT * MLP_H,
               MLP_NPARAM = MLP_B3 + MLP_OUT;
 static_assert(MLP_NPARAM == CUSLINES_MLP_NPARAM, "MLP layout");
-constexpr int MIX_STRIDE = 12;  // mean[3], e1[3], e2[3], kappa, exp(-2 kappa), cumulative weight
-constexpr int ACT_X = 0, ACT_H1 = 64, ACT_H2 = 192, ACT_O = 320, ACT_FLOATS = 384;
-constexpr int PTT_SLOT_FLOATS = 96 + MLP_K * MIX_STRIDE + ACT_FLOATS;
+static_assert(MLP_OUT == DISC_VERT_CNT, "one output per disc vertex");
+constexpr int ACT_X = 0, ACT_H1 = 64, ACT_H2 = 192, ACT_O = 320, ACT_FLOATS = 352;
+constexpr int PTT_SLOT_FLOATS = 96 + ACT_FLOATS;
+constexpr int FEAT_STEPS = 4;  // probe steps between the two feature taps
 
 __device__ __forceinline__ float bf16f(uint32_t h) { return __uint_as_float(h << 16); }
 
-// x[0..39] in LDS
+// Inputs at the start of a sub-step into x[0..39] (LDS).  Lane v < 24: ODF
+// taps after 4 and 8 probe steps along vertex v's curve (one propagator of
+// arc length 4 probe steps, applied twice), summed.
 __device__ __forceinline__ void mlp_features(const DevParams &P, const DevBufs &B, float px, float py, float pz,
-                                             float dx, float dy, float dz, const float *fr, float *x) {
+                                             float dx, float dy, float dz, const float *fr, float first_val,
+                                             float *x) {
     const int l = lane32();
-    if (trilinear(P, B.sh, 32, MLP_SH, px, py, pz, x) != 0 && l < MLP_SH) x[l] = 0.0f;
-    if (l < 3) x[MLP_SH + l] = (l == 0) ? dx : (l == 1) ? dy : dz;
-    if (l < 9) x[MLP_SH + 3 + l] = fr[l];
+    if (l < DISC_VERT_CNT) {
+        float pf[9], prop[9], d[3];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) pf[i] = fr[i];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) prop[i] = B.fprop[9 * l + i];
+        float qx = px, qy = py, qz = pz;
+        propagate_frame(prop, pf, d);
+        qx += d[0]; qy += d[1]; qz += d[2];
+        const float t1 = interp4(P, B, qx, qy, qz, pf[0], pf[1], pf[2]);
+        propagate_frame(prop, pf, d);
+        qx += d[0]; qy += d[1]; qz += d[2];
+        const float t2 = interp4(P, B, qx, qy, qz, pf[0], pf[1], pf[2]);
+        x[l] = t1 + t2;
+    } else if (l == DISC_VERT_CNT) {
+        x[l] = first_val;
+    } else if (l < 28) {
+        x[l] = 0.0f;
+    } else if (l < 31) {
+        x[l] = (l == 28) ? dx : (l == 29) ? dy : dz;
+    }
+    if (l < 9) x[31 + l] = fr[l];
     WAVE_SYNC();
 }
 
@@ -892,6 +928,7 @@ __device__ __forceinline__ float mlp_neuron(const uint16_t *__restrict__ w, uint
     return acc;
 }
 
+// Kept out of line so the tracking loop keeps its register budget.
 __device__ __noinline__ void mlp_forward(const uint16_t *__restrict__ W, float *act) {
     const int l = lane32();
 #pragma unroll 1
@@ -908,141 +945,72 @@ __device__ __noinline__ void mlp_forward(const uint16_t *__restrict__ W, float *
         act[ACT_H2 + j] = (v > 0.0f) ? v : 0.0f;
     }
     WAVE_SYNC();
-    for (int j = l; j < MLP_OUT; j += 32)
-        act[ACT_O + j] = mlp_neuron<MLP_H>(W + MLP_W3 + j * MLP_H, W[MLP_B3 + j], act + ACT_H2);
+    if (l < MLP_OUT) act[ACT_O + l] = mlp_neuron<MLP_H>(W + MLP_W3 + l * MLP_H, W[MLP_B3 + l], act + ACT_H2);
     WAVE_SYNC();
 }
 
-// Mixture parameters from the outputs (once per output step; kept out of
-// line, as is mlp_forward, so the tracking loop keeps its register budget):
-// unit mean m (fallback +tangent), an
-// orthonormal pair (e1, e2) spanning the plane normal to m, kappa =
-// exp(clamp(.., -4, 6)), exp(-2 kappa), and cumulative softmax weights
-// (summed in component order).
-__device__ __noinline__ void mixture_setup(const float *o, float *mix) {
-    const int l = lane32();
-    if (l < MLP_K) {
-        const float *oc = o + 5 * l;
-        float mx = oc[0], my = oc[1], mz = oc[2];
-        const float n2 = __builtin_fmaf(mz, mz, __builtin_fmaf(my, my, mx * mx));
-        if (n2 > 1e-30f && n2 <= REAL_MAX) {
-            const float inv = rsqrt_nr(n2);
-            mx *= inv; my *= inv; mz *= inv;
-        } else {
-            mx = 1.0f; my = 0.0f; mz = 0.0f;
-        }
-        const float ax = (fabsf(mx) < 0.9f) ? 1.0f : 0.0f, ay = 1.0f - ax;
-        const float am = __builtin_fmaf(ay, my, ax * mx);
-        float e1x = __builtin_fmaf(-am, mx, ax), e1y = __builtin_fmaf(-am, my, ay), e1z = -am * mz;
-        const float inv1 = rsqrt_nr(__builtin_fmaf(e1z, e1z, __builtin_fmaf(e1y, e1y, e1x * e1x)));
-        e1x *= inv1; e1y *= inv1; e1z *= inv1;
-        const float e2x = __builtin_fmaf(my, e1z, -(mz * e1y));
-        const float e2y = __builtin_fmaf(mz, e1x, -(mx * e1z));
-        const float e2z = __builtin_fmaf(mx, e1y, -(my * e1x));
-        const float kap = uexp(fminf(fmaxf(oc[3], -4.0f), 6.0f));
-        float amax = o[4];
-        for (int c = 1; c < MLP_K; ++c) amax = fmaxf(amax, o[5 * c + 4]);
-        float *mc = mix + MIX_STRIDE * l;
-        mc[0] = mx; mc[1] = my; mc[2] = mz;
-        mc[3] = e1x; mc[4] = e1y; mc[5] = e1z;
-        mc[6] = e2x; mc[7] = e2y; mc[8] = e2z;
-        mc[9] = kap;
-        mc[10] = uexp(-2.0f * kap);
-        mc[11] = uexp(oc[4] - amax);
-    }
-    WAVE_SYNC();
-    if (l == 0) {
-        float c = 0.0f;
-        for (int k = 0; k < MLP_K; ++k) {
-            c = c + mix[MIX_STRIDE * k + 11];
-            mix[MIX_STRIDE * k + 11] = c;
-        }
-    }
-    WAVE_SYNC();
-}
-
-// One candidate per lane from the mixture: component by inverse CDF, then a
-// vMF direction (Wood 1994: w = 1 + log(u + (1-u) e^{-2 kappa}) / kappa,
-// azimuth 2 pi v) in frame coordinates; (k1, k2) = MAX_CURVATURE times its
-// normal and binormal parts.  Returns false for the far hemisphere.
-__device__ __forceinline__ bool mixture_draw(const DevParams &P, const float *mix, Philox &rng, float *k1, float *k2) {
-    const float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
-    const float tgt = u0 * mix[MIX_STRIDE * (MLP_K - 1) + 11];
-    int c = 0;
-    for (; c < MLP_K - 1; ++c)
-        if (mix[MIX_STRIDE * c + 11] >= tgt) break;
-    const float *mc = mix + MIX_STRIDE * c;
-    float w = 1.0f + ulog(__builtin_fmaf(1.0f - u1, mc[10], u1)) / mc[9];
-    w = fminf(fmaxf(w, -1.0f), 1.0f);
-    const float sw = __builtin_sqrtf(fmaxf(__builtin_fmaf(-w, w, 1.0f), 0.0f));
-    float sn, cs;
-    usincos(6.28318531f * u2, &sn, &cs);
-    const float a1 = sw * cs, a2 = sw * sn;
-    const float ut = __builtin_fmaf(a2, mc[6], __builtin_fmaf(a1, mc[3], w * mc[0]));
-    const float un = __builtin_fmaf(a2, mc[7], __builtin_fmaf(a1, mc[4], w * mc[1]));
-    const float ub = __builtin_fmaf(a2, mc[8], __builtin_fmaf(a1, mc[5], w * mc[2]));
-    *k1 = P.max_curv * un;
-    *k2 = P.max_curv * ub;
-    return ut > 0.0f;
-}
-
-// A non-initial PTT sub-step from the learned mixture: one round of 32
-// candidates under the reference's acceptance rule (support >= 0.4, lowest
-// lane wins).  Returns 1 with the new direction, or -1 when no candidate
-// passes (the caller then runs the plain getter on the same streams).
+// A non-initial PTT sub-step with the learned proposal.  First attempt
+// (when `learned`): the prediction, cut at 0.4 like a measured support,
+// builds the face CDF and one round of 32 candidates runs under the
+// reference's acceptance rule.  If the prediction leaves no face or no
+// candidate passes, *fell is set and the plain getter runs the sub-step on
+// the same streams (vertex supports measured, up to 32 rounds).  One call
+// site per phase keeps the kernel's code and registers small.
 __device__ int get_direction_ptt_learned(const DevParams &P, const DevBufs &B, PttLds &S, float *frame, Philox &rng,
-                                         float px, float py, float pz, float *ndx, float *ndy, float *ndz) {
+                                         bool learned, float dx, float dy, float dz, float px, float py, float pz,
+                                         float *ndx, float *ndy, float *ndz, int *fell) {
     const int l = lane32();
     float fr[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) fr[i] = frame[i];
     const float first_val = interp4(P, B, px, py, pz, fr[0], fr[1], fr[2]);
-    float k1, k2, pf[9];
+    *fell = 0;
+    for (int attempt = learned ? 0 : 1; attempt < 2; ++attempt) {
+        if (attempt == 0) {
+            mlp_features(P, B, px, py, pz, dx, dy, dz, fr, first_val, S.act + ACT_X);
+            mlp_forward(B.mlp, S.act);
+            if (l < DISC_VERT_CNT) {
+                const float sup = S.act[ACT_O + l];
+                S.vert_pdf[l] = (sup < P.min_support_f) ? 0.0f : sup;
+            }
+            WAVE_SYNC();
+        } else {
+            bool found = false;
+            if (l < DISC_VERT_CNT) {
+                const float k1 = (float)((double)c_disc_vert[2 * l] * (double)P.max_curv);
+                const float k2 = (float)((double)c_disc_vert[2 * l + 1] * (double)P.max_curv);
+                float pf[9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) pf[i] = fr[i];
-    float sup = -1.0f;
-    if (mixture_draw(P, S.mix, rng, &k1, &k2)) sup = data_support(P, B, first_val, px, py, pz, k1, k2, pf);
-    const uint32_t msk = half_ballot(!(sup < P.min_support_f));
-    if (msk == 0u) return -1;
-    const int win = __ffs(msk) - 1;
-    float ox = 0.0f, oy = 0.0f, oz = 0.0f;
-    if (l == win) {
-        float prop[9], d[3];
-        prepare_propagator(k1, k2, P.ptt_sub_step, prop);
-#pragma unroll
-        for (int i = 0; i < 9; ++i) pf[i] = fr[i];
-        propagate_frame(prop, pf, d);
-        norm3<0>(d[0], d[1], d[2]);
-        ox = d[0]; oy = d[1]; oz = d[2];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) frame[i] = pf[i];
+                for (int i = 0; i < 9; ++i) pf[i] = fr[i];
+                const float sup = data_support(P, B, first_val, px, py, pz, k1, k2, pf);
+                found = !(sup < P.min_support_f);
+                S.vert_pdf[l] = found ? sup : 0.0f;
+            }
+            if (half_ballot(found) == 0u) return 0;
+            WAVE_SYNC();
+        }
+        const int r = ptt_sample<false>(P, B, S, frame, fr, rng, first_val, attempt ? 1024 / 32 : 1, dx, dy, dz, px,
+                                        py, pz, ndx, ndy, ndz, nullptr);
+        if (r || attempt) return r;
+        *fell = 1;
     }
-    *ndx = shfl32(ox, win);
-    *ndy = shfl32(oy, win);
-    *ndz = shfl32(oz, win);
-    WAVE_SYNC();
-    return 1;
+    return 0;
 }
 
-// Training pair of an accepted plain sub-step: inputs at the sub-step's
-// start and the winner as a unit vector (tangent part, k1/mc, k2/mc).
+// Training pair of a plain sub-step: inputs at the sub-step's start and the
+// 24 measured vertex supports.
 __device__ void record_pair(const DevParams &P, const DevBufs &B, PttLds &S, float px, float py, float pz, float dx,
-                            float dy, float dz, const float *fr, const float *wk) {
+                            float dy, float dz, const float *fr0, const float *raw) {
     const int l = lane32();
-    mlp_features(P, B, px, py, pz, dx, dy, dz, fr, S.act + ACT_X);
+    const float first_val = interp4(P, B, px, py, pz, fr0[0], fr0[1], fr0[2]);
+    mlp_features(P, B, px, py, pz, dx, dy, dz, fr0, first_val, S.act + ACT_X);
     int idx = 0;
     if (l == 0) idx = atomicAdd(B.rec_count, 1);
     idx = shfl32(idx, 0);
     if (idx < P.rec_cap) {
         float *o = B.rec + (size_t)idx * CUSLINES_PAIR_FLOATS;
         for (int j = l; j < MLP_IN; j += 32) o[j] = S.act[ACT_X + j];
-        if (l == 0) {
-            const float a = wk[0] / P.max_curv, b = wk[1] / P.max_curv;
-            o[MLP_IN] = __builtin_sqrtf(fmaxf(1.0f - __builtin_fmaf(a, a, b * b), 0.0f));
-            o[MLP_IN + 1] = a;
-            o[MLP_IN + 2] = b;
-            o[MLP_IN + 3] = 0.0f;
-        }
+        if (l < DISC_VERT_CNT) o[MLP_IN + l] = raw[l];
     }
     WAVE_SYNC();
 }
@@ -1082,29 +1050,24 @@ __device__ int tracker(const DevParams &P, const DevBufs &B, const float *sv, Sl
         if (MODEL == CUSLINES_PROB)
             ndir = get_direction_prob<false>(P, B, sv, S.prob, rng, dx, dy, dz, px, py, pz, &nx, &ny, &nz);
         else if (MODEL == CUSLINES_PTT && V == 1) {
-            ndir = -1;
+            int fell = 0;
+            ndir = get_direction_ptt_learned(P, B, S.ptt, frame, rng, learned, dx, dy, dz, px, py, pz, &nx, &ny, &nz,
+                                             &fell);
             if (learned) {
-                if ((i - 1) % step_frac == 0) {
-                    mlp_features(P, B, px, py, pz, dx, dy, dz, frame, S.ptt.act + ACT_X);
-                    mlp_forward(B.mlp, S.ptt.act);
-                    mixture_setup(S.ptt.act + ACT_O, S.ptt.mix);
-                }
-                ndir = get_direction_ptt_learned(P, B, S.ptt, frame, rng, px, py, pz, &nx, &ny, &nz);
                 ++ltries;
-                if (ndir < 0) ++lfalls;
+                lfalls += fell;
                 if (ltries >= 16 && 4 * lfalls > ltries) learned = false;
             } else {
                 ++nplain;
             }
-            if (ndir < 0)
-                ndir = get_direction_ptt<false>(P, B, S.ptt, frame, rng, dx, dy, dz, px, py, pz, &nx, &ny, &nz);
         } else if (MODEL == CUSLINES_PTT && V == 2) {
-            float fr0[9], wk[2];
+            float fr0[9];
 #pragma unroll
             for (int q = 0; q < 9; ++q) fr0[q] = frame[q];
-            ndir = get_direction_ptt<false>(P, B, S.ptt, frame, rng, dx, dy, dz, px, py, pz, &nx, &ny, &nz, wk);
-            if (ndir && mix32(tag * 0x9E3779B1u + (uint32_t)i) < P.rec_thresh)
-                record_pair(P, B, S.ptt, px, py, pz, dx, dy, dz, fr0, wk);
+            const bool rec = mix32(tag * 0x9E3779B1u + (uint32_t)i) < P.rec_thresh;
+            ndir = get_direction_ptt<false>(P, B, S.ptt, frame, rng, dx, dy, dz, px, py, pz, &nx, &ny, &nz, nullptr,
+                                            rec ? S.ptt.act + ACT_O : nullptr);
+            if (rec) record_pair(P, B, S.ptt, px, py, pz, dx, dy, dz, fr0, S.ptt.act + ACT_O);
         } else if (MODEL == CUSLINES_PTT)
             ndir = get_direction_ptt<false>(P, B, S.ptt, frame, rng, dx, dy, dz, px, py, pz, &nx, &ny, &nz);
         else
@@ -1618,8 +1581,7 @@ __device__ void bind_slot(const DevParams &P, bool phase1, float *base, SlotLds 
         S.ptt.frame = base;
         S.ptt.vert_pdf = base + 32;
         S.ptt.face_cdf = base + 64;
-        S.ptt.mix = base + 96;
-        S.ptt.act = base + 96 + MLP_K * MIX_STRIDE;
+        S.ptt.act = base + 96;
     } else if (MODEL == CUSLINES_PROB || MODEL == CUSLINES_PTT) {
         S.prob.pmf = base;
         S.prob.marks = reinterpret_cast<int *>(base + P.nv32);
@@ -1768,7 +1730,7 @@ __global__ void __launch_bounds__(256) phase2_kernel(DevParams P, DevBufs B, con
 // Plain PTT is held to 128 VGPRs (4 waves per SIMD): the few spills cost less
 // than the latency hiding gained (+8% at 200k seeds).
 template <int MODEL, int V = 0>
-__global__ void __launch_bounds__(256, (MODEL == CUSLINES_PTT && V == 0) ? 4 : 1) phase2_queue_kernel(DevParams P, DevBufs B, const float *__restrict__ seeds,
+__global__ void __launch_bounds__(256, (MODEL == CUSLINES_PTT && V != 2) ? 4 : 1) phase2_queue_kernel(DevParams P, DevBufs B, const float *__restrict__ seeds,
                                                            int nseeds, int rng_chunk, const int *__restrict__ offsets,
                                                            const int *__restrict__ dirs, int *__restrict__ sline_seed,
                                                            int *__restrict__ sline_len, float *__restrict__ sline,
@@ -1822,51 +1784,23 @@ __global__ void mlp_fwd_layer(const float *__restrict__ prm, int w_off, int b_of
     out[idx] = RELU ? ((acc > 0.0f) ? acc : 0.0f) : acc;
 }
 
-// Per pair: negative log-likelihood of the target under the mixture and its
-// gradient with respect to the 40 outputs, divided by n.
+// Per pair: squared error of the 24 predicted supports, averaged over the
+// 24, and its gradient with respect to the outputs divided by n.
 __global__ void mlp_loss_out(const float *__restrict__ O, const float *__restrict__ pairs, int n,
                              float *__restrict__ dO, float *__restrict__ loss) {
     const int smp = blockIdx.x * blockDim.x + threadIdx.x;
     if (smp >= n) return;
     const float *o = O + (size_t)smp * MLP_OUT;
-    const float *u = pairs + (size_t)smp * CUSLINES_PAIR_FLOATS + MLP_IN;
-    float mh[MLP_K][3], nrm[MLP_K], kap[MLP_K], dd[MLP_K], ell[MLP_K], lpi[MLP_K], e2[MLP_K];
-    float amax = o[4];
-    for (int c = 1; c < MLP_K; ++c) amax = fmaxf(amax, o[5 * c + 4]);
-    float sa = 0.0f;
-    for (int c = 0; c < MLP_K; ++c) sa += uexp(o[5 * c + 4] - amax);
-    const float lsa = amax + ulog(sa);
-    float lmax = -REAL_MAX;
-    for (int c = 0; c < MLP_K; ++c) {
-        const float *oc = o + 5 * c;
-        const float n2 = __builtin_fmaf(oc[2], oc[2], __builtin_fmaf(oc[1], oc[1], oc[0] * oc[0]));
-        nrm[c] = __builtin_sqrtf(n2);
-        const float inv = (nrm[c] > 1e-12f) ? 1.0f / nrm[c] : 0.0f;
-        mh[c][0] = oc[0] * inv; mh[c][1] = oc[1] * inv; mh[c][2] = oc[2] * inv;
-        kap[c] = uexp(fminf(fmaxf(oc[3], -4.0f), 6.0f));
-        e2[c] = uexp(-2.0f * kap[c]);
-        dd[c] = __builtin_fmaf(mh[c][2], u[2], __builtin_fmaf(mh[c][1], u[1], mh[c][0] * u[0]));
-        lpi[c] = o[5 * c + 4] - lsa;
-        const float logc = ulog(kap[c]) - 1.83787707f - kap[c] - ulog(1.0f - e2[c]);
-        ell[c] = lpi[c] + logc + kap[c] * dd[c];
-        lmax = fmaxf(lmax, ell[c]);
-    }
-    float se = 0.0f;
-    for (int c = 0; c < MLP_K; ++c) se += uexp(ell[c] - lmax);
-    const float L = lmax + ulog(se);
-    loss[smp] = -L;
-    const float sc = 1.0f / (float)n;
+    const float *t = pairs + (size_t)smp * CUSLINES_PAIR_FLOATS + MLP_IN;
     float *g = dO + (size_t)smp * MLP_OUT;
-    for (int c = 0; c < MLP_K; ++c) {
-        const float r = uexp(ell[c] - L);
-        const float pi = uexp(lpi[c]);
-        g[5 * c + 4] = (pi - r) * sc;
-        const float z = o[5 * c + 3];
-        const float dl = 1.0f - 2.0f * kap[c] * e2[c] / (1.0f - e2[c]) - kap[c] + kap[c] * dd[c];
-        g[5 * c + 3] = (z < -4.0f || z > 6.0f) ? 0.0f : -r * dl * sc;
-        const float f = (nrm[c] > 1e-12f) ? -r * kap[c] / nrm[c] * sc : 0.0f;
-        for (int a = 0; a < 3; ++a) g[5 * c + a] = f * (u[a] - dd[c] * mh[c][a]);
+    const float sc = 2.0f / ((float)MLP_OUT * (float)n);
+    float acc = 0.0f;
+    for (int j = 0; j < MLP_OUT; ++j) {
+        const float e = o[j] - t[j];
+        acc = __builtin_fmaf(e, e, acc);
+        g[j] = e * sc;
     }
+    loss[smp] = acc / (float)MLP_OUT;
 }
 
 // din[s][k] = (sum_j W[j][k] dout[s][j]) where hin[s][k] > 0, else 0.
@@ -1926,11 +1860,17 @@ __global__ void adam_kernel(float *__restrict__ p, const float *__restrict__ g, 
     p[i] -= lr * (mi / c1) / (__builtin_sqrtf(vi / c2) + eps);
 }
 
-__global__ void pad_sh(const float *__restrict__ src, float *__restrict__ dst, long long nvox) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nvox * 32) return;
-    const int c = (int)(i & 31);
-    dst[i] = (c < MLP_SH) ? src[(i >> 5) * MLP_SH + c] : 0.0f;
+// Propagators of arc length FEAT_STEPS probe steps for the 24 disc vertices
+// (the curves along which the learned proposal's inputs are tapped).
+__global__ void feature_propagators(DevParams P, float *__restrict__ out) {
+    const int l = threadIdx.x;
+    if (l < DISC_VERT_CNT) {
+        const float k1 = (float)((double)c_disc_vert[2 * l] * (double)P.max_curv);
+        const float k2 = (float)((double)c_disc_vert[2 * l + 1] * (double)P.max_curv);
+        float p[9];
+        prepare_propagator(k1, k2, (float)FEAT_STEPS * P.probe_step, p);
+        for (int i = 0; i < 9; ++i) out[9 * l + i] = p[i];
+    }
 }
 
 // PTT corner blocks: for vertex v and texel cell (cx,cy,cz) in [-1, D-1]^3 of
@@ -2108,7 +2048,7 @@ struct cuslines_tracker {
     int blocks_per_cu = 0;  // 0: occupancy maximum
     int num_cu = 1;
     int ptt_lanes = 0;      // PTT: lanes per seed (8, 16, 32; 0 = shared 32-lane kernel)
-    float *d_sh = nullptr;               // SH coefficients [X][Y][Z][32]
+    float *d_fprop = nullptr;            // learned proposal: feature propagators [24][9]
     uint16_t *d_mlp = nullptr;           // learned proposal (nullptr: plain getter)
     unsigned long long *d_stats = nullptr;
     float *rec = nullptr;                // caller's training-pair buffer (nullptr: no recording)
@@ -2135,7 +2075,7 @@ static DevBufs bufs_of(const cuslines_tracker *t) {
     b.dwi_idx = t->d_dwi;
     b.lut = t->d_lut;
     b.blocks = t->use_blocks ? t->d_blocks : nullptr;
-    b.sh = t->d_sh;
+    b.fprop = t->d_fprop;
     b.mlp = t->d_mlp;
     b.rec = t->rec;
     b.rec_count = t->rec_count;
@@ -2377,7 +2317,7 @@ int cuslines_tracker_destroy(cuslines_tracker *t) {
     dfree(t->d_data); dfree(t->d_stop); dfree(t->d_verts); dfree(t->d_edges);
     dfree(t->d_H); dfree(t->d_R); dfree(t->d_db); dfree(t->d_dq); dfree(t->d_smat);
     dfree(t->d_b0); dfree(t->d_dwi); dfree(t->d_lut); dfree(t->d_work); dfree(t->d_blocks);
-    dfree(t->d_sh); dfree(t->d_mlp); dfree(t->d_stats);
+    dfree(t->d_fprop); dfree(t->d_mlp); dfree(t->d_stats);
     if (t->stream) (void)hipStreamDestroy(t->stream);
     delete t;
     return CUSLINES_OK;
@@ -2584,25 +2524,12 @@ int cuslines_set_ptt_layout(cuslines_tracker *t, int corner_blocks) {
     return CUSLINES_OK;
 }
 
-int cuslines_set_sh_coeffs(cuslines_tracker *t, const float *sh, int32_t ncoef) {
-    if (!t || !sh) return fail(CUSLINES_EINVAL, "null argument");
-    if (ncoef != CUSLINES_SH_COEFFS) return fail(CUSLINES_EINVAL, "ncoef must be CUSLINES_SH_COEFFS (28)");
-    HIPC(hipSetDevice(t->device));
-    const long long nvox = (long long)t->dp.dimx * t->dp.dimy * t->dp.dimz;
-    float *tmp = nullptr;
-    HIPC(hipMalloc(&tmp, (size_t)nvox * MLP_SH * sizeof(float)));
-    dfree(t->d_sh);
-    t->d_sh = nullptr;
-    hipError_t e = hipMalloc(&t->d_sh, (size_t)nvox * 32 * sizeof(float));
-    if (e == hipSuccess) e = hipMemcpy(tmp, sh, (size_t)nvox * MLP_SH * sizeof(float), hipMemcpyHostToDevice);
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(pad_sh, dim3((unsigned)((nvox * 32 + 255) / 256)), dim3(256), 0, t->stream, tmp, t->d_sh,
-                           nvox);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
-    dfree(tmp);
-    if (e != hipSuccess) return fail(CUSLINES_EHIP, std::string("SH upload: ") + hipGetErrorString(e));
+static int ensure_feature_propagators(cuslines_tracker *t) {
+    if (t->d_fprop) return CUSLINES_OK;
+    HIPC(hipMalloc(&t->d_fprop, DISC_VERT_CNT * 9 * sizeof(float)));
+    hipLaunchKernelGGL(feature_propagators, dim3(1), dim3(32), 0, t->stream, t->dp, t->d_fprop);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(t->stream));
     return CUSLINES_OK;
 }
 
@@ -2615,7 +2542,7 @@ int cuslines_set_proposal(cuslines_tracker *t, const uint16_t *mlp_bf16) {
         return CUSLINES_OK;
     }
     if (t->p.model_type != CUSLINES_PTT) return fail(CUSLINES_EINVAL, "the learned proposal is for PTT");
-    if (!t->d_sh) return fail(CUSLINES_ESTATE, "cuslines_set_sh_coeffs first");
+    if (int rc = ensure_feature_propagators(t)) return rc;
     if (!t->d_mlp) HIPC(hipMalloc(&t->d_mlp, MLP_NPARAM * sizeof(uint16_t)));
     HIPC(hipMemcpy(t->d_mlp, mlp_bf16, MLP_NPARAM * sizeof(uint16_t), hipMemcpyHostToDevice));
     if (!t->d_stats) {
@@ -2635,7 +2562,7 @@ int cuslines_set_recording(cuslines_tracker *t, float *pairs_d, int32_t capacity
     if (!count_d || capacity <= 0 || !(rate > 0.0f && rate <= 1.0f))
         return fail(CUSLINES_EINVAL, "recording needs count_d, capacity > 0 and 0 < rate <= 1");
     if (t->p.model_type != CUSLINES_PTT) return fail(CUSLINES_EINVAL, "recording is for PTT");
-    if (!t->d_sh) return fail(CUSLINES_ESTATE, "cuslines_set_sh_coeffs first");
+    if (int rc = ensure_feature_propagators(t)) return rc;
     t->rec = pairs_d;
     t->rec_count = count_d;
     t->dp.rec_cap = capacity;
@@ -2657,7 +2584,7 @@ int cuslines_proposal_stats(cuslines_tracker *t, int64_t *out3, int reset) {
     return CUSLINES_OK;
 }
 
-static size_t mlp_ws_floats(int n) { return (size_t)n * (4 * MLP_H + 2 * MLP_OUT + 1); }
+static size_t mlp_ws_floats(int n) { return (size_t)n * (4 * MLP_H + 2 * MLP_OUT + 1); }  // H1 H2 dH2 dH1, O dO, loss
 
 int cuslines_mlp_workspace_bytes(int32_t n, size_t *bytes) {
     if (!bytes || n <= 0) return fail(CUSLINES_EINVAL, "n > 0 and bytes required");

@@ -183,12 +183,17 @@ class PttDirectionGetter(ProbDirectionGetter):
     model_type = MODEL_TYPES["PTT"]
 
     def __init__(self, odf_lut_res: int = 128, probe_length: float = 0.25,
-                 target_short_step: float = 0.025, probe_quality: int = 8):
+                 target_short_step: float = 0.025, probe_quality: int = 8, proposal=None):
+        """``proposal`` (not in the reference): parameters of a trained
+        learned-proposal network (float32, or bf16 bit patterns as uint16;
+        see gpustreamlines_amd.learned).  None keeps the reference's face-CDF
+        proposal."""
         self.odf_lut_res = int(odf_lut_res)
         self.probe_length = probe_length
         self.probe_quality = int(probe_quality)
         self.target_short_step = target_short_step
         self.sphere_vertices_lut_h = None
+        self.proposal = proposal
 
     def set_macros(self, gpu_tracker):
         step = np.float32(gpu_tracker.step_size)
@@ -217,4 +222,10 @@ class PttDirectionGetter(ProbDirectionGetter):
         return np.ascontiguousarray(dataf, dtype=np.float32)
 
     def allocate_on_gpu(self, n):
-        self.gpu_tracker.device_trackers[n].set_lut(self.sphere_vertices_lut_h)
+        dt = self.gpu_tracker.device_trackers[n]
+        dt.set_lut(self.sphere_vertices_lut_h)
+        if self.proposal is not None:
+            from .learned import to_bf16
+
+            w = np.asarray(self.proposal)
+            dt.set_proposal(w if w.dtype == np.uint16 else to_bf16(w))

@@ -1,58 +1,42 @@
-"""Learned PTT proposal: features, training pairs, training and export.
+"""Learned PTT proposal: training pairs, training on the GPU and export.
 
-No counterpart in the reference.  A BF16 MLP (40 -> 128 -> 128 -> 40, ReLU)
-maps the 28 order-6 SH coefficients of the ODF at the point, the incoming
-direction and the 9-float PTT frame to an 8-component von Mises-Fisher
-mixture on the hemisphere around the frame tangent; its orthographic
-projection onto the (k1, k2) curvature disc proposes PTT candidates, and the
-reference's acceptance rule (probe support >= 0.4, first candidate wins)
-still decides each one.  Parameter layout: ``include/cuslines_hip.h``.
+No counterpart in the reference.  At every non-initial sub-step the
+reference's PTT getter measures the probe support of 24 disc vertices (8 ODF
+taps each) to build its face-CDF proposal (cuslines/cuda_c/ptt.cu:232-300).
+The learned proposal predicts those 24 supports with a BF16 MLP
+(40 -> 128 -> 128 -> 24, ReLU) from two taps per vertex; the reference's face
+CDF, candidate draw and acceptance rule (support >= 0.4, first lane wins)
+then run unchanged on the prediction, and a round without a winner falls back
+to the plain getter.  Inputs and parameter layout: ``include/cuslines_hip.h``.
 
-Training runs on the GPU through the library's own kernels
-(``cuslines_mlp_loss_grad_async``, ``cuslines_mlp_adam_async``) on pairs the
-plain getter records while it tracks (``cuslines_set_recording``).
+The plain getter records (inputs, measured supports) pairs while it tracks
+(``cuslines_set_recording``); the network trains on them with the library's
+own kernels (``cuslines_mlp_loss_grad_async``: squared error,
+``cuslines_mlp_adam_async``).
 """
 
 from __future__ import annotations
 
 import numpy as np
 
-from . import _lib, shm
+from . import _lib
 
-N_IN, N_H, N_OUT, N_K = 40, 128, 40, 8
+N_IN, N_H, N_OUT = 40, 128, 24
 W1, B1 = 0, N_H * N_IN
 W2, B2 = B1 + N_H, B1 + N_H + N_H * N_H
 W3, B3 = B2 + N_H, B2 + N_H + N_OUT * N_H
 NPARAM = B3 + N_OUT
 assert NPARAM == _lib.MLP_NPARAM
-
-
-def sh_coefficients(odf: np.ndarray, vertices: np.ndarray, order: int = 6, chunk: int = 65536) -> np.ndarray:
-    """Least-squares real SH coefficients (order 6: 28) of the ODF sampled on
-    the sphere's vertices: [X, Y, Z, 28] float32."""
-    _, th, ph = shm.cart2sphere(*np.asarray(vertices, dtype=np.float64).T)
-    B, _, _ = shm.real_sh_descoteaux(order, th, ph)
-    P = np.linalg.pinv(B).T  # [V, 28]
-    flat = odf.reshape(-1, odf.shape[-1])
-    out = np.empty((flat.shape[0], P.shape[1]), dtype=np.float32)
-    for i in range(0, flat.shape[0], chunk):
-        out[i:i + chunk] = flat[i:i + chunk].astype(np.float64) @ P
-    return out.reshape(odf.shape[:-1] + (P.shape[1],))
+PAIR = _lib.PAIR_FLOATS
 
 
 def init_params(seed: int = 0) -> np.ndarray:
-    """He-initialised hidden layers; the output layer starts near a single
-    broad lobe around the tangent (mean bias +1 on the tangent axis, log
-    concentration bias 2)."""
+    """He-initialised layers, zero biases."""
     rng = np.random.default_rng(seed)
     p = np.zeros(NPARAM, dtype=np.float32)
     p[W1:B1] = rng.normal(0, np.sqrt(2.0 / N_IN), N_H * N_IN)
     p[W2:B2] = rng.normal(0, np.sqrt(2.0 / N_H), N_H * N_H)
-    p[W3:B3] = rng.normal(0, 0.1 * np.sqrt(1.0 / N_H), N_OUT * N_H)
-    b3 = p[B3:B3 + N_OUT].reshape(N_K, 5)
-    b3[:, 0] = 1.0
-    b3[:, 1:3] = rng.normal(0, 0.3, (N_K, 2))
-    b3[:, 3] = 2.0
+    p[W3:B3] = rng.normal(0, np.sqrt(1.0 / N_H), N_OUT * N_H)
     return p
 
 
@@ -70,9 +54,10 @@ def from_bf16(w: np.ndarray) -> np.ndarray:
 
 
 def record_pairs(dt, seeds: np.ndarray, capacity: int, rate: float, rng_chunk: int = 25000):
-    """Track ``seeds`` with the plain getter and keep each accepted sub-step
-    with probability ``rate``.  Returns (device buffer of pairs, count)."""
-    pairs = dt.alloc(4 * _lib.PAIR_FLOATS * capacity)
+    """Track ``seeds`` with the plain getter, keeping each non-initial
+    sub-step with probability ``rate``.  Returns (device buffer of pairs,
+    count)."""
+    pairs = dt.alloc(4 * PAIR * capacity)
     cnt = dt.alloc(4)
     cnt.upload(np.zeros(1, np.int32))
     dt.set_proposal(None)
@@ -111,28 +96,29 @@ class Trainer:
         self.dt.synchronize()
         return float(self.loss.download(np.zeros(1, np.float32))[0]), self.g.download(np.zeros(NPARAM, np.float32))
 
-    def fit(self, pairs, n_pairs: int, epochs: int = 3, lr: float = 2e-3, log_every: int = 0):
-        """Mini-batch Adam over the pairs in order (the recording order is
-        already shuffled across seeds by the hash).  Returns the mean loss of
-        each epoch's last 50 batches."""
-        nb = max(1, n_pairs // self.batch)
+    def fit(self, pairs, n_pairs: int, epochs: int = 3, lr: float = 1e-3, lr_final: float = 1e-4,
+            holdout: int = 0):
+        """Mini-batch Adam over the pairs in recording order (the hash already
+        interleaves seeds), learning rate decayed geometrically from ``lr`` to
+        ``lr_final``; the last ``holdout`` pairs are not trained on.  Returns
+        the mean loss of each epoch's last 20 batches."""
+        n_train = n_pairs - holdout
+        nb = max(1, n_train // self.batch)
+        total = epochs * nb
         hist = []
-        stride = 4 * _lib.PAIR_FLOATS
+        stride = 4 * PAIR
         for ep in range(epochs):
             tail = []
             for b in range(nb):
-                n = min(self.batch, n_pairs - b * self.batch)
+                n = min(self.batch, n_train - b * self.batch)
+                rate = lr * (lr_final / lr) ** (self.step / max(1, total - 1))
                 self.dt.mlp_loss_grad_async(self.p.ptr, pairs.ptr + b * self.batch * stride, n, self.loss.ptr,
                                             self.g.ptr, self.work.ptr)
                 self.step += 1
-                self.dt.mlp_adam_async(self.p.ptr, self.g.ptr, self.m.ptr, self.v.ptr, self.step, lr)
-                if b >= nb - 50 or (log_every and b % log_every == 0):
+                self.dt.mlp_adam_async(self.p.ptr, self.g.ptr, self.m.ptr, self.v.ptr, self.step, rate)
+                if b >= nb - 20:
                     self.dt.synchronize()
-                    val = float(self.loss.download(np.zeros(1, np.float32))[0])
-                    if b >= nb - 50:
-                        tail.append(val)
-                    if log_every and b % log_every == 0:
-                        print(f"[train] epoch {ep} batch {b}/{nb} loss {val:.4f}", flush=True)
+                    tail.append(float(self.loss.download(np.zeros(1, np.float32))[0]))
             hist.append(float(np.mean(tail)))
         self.dt.synchronize()
         return hist
@@ -145,9 +131,27 @@ class Trainer:
             b.free()
 
 
+def train(dt, seeds: np.ndarray, rate: float = 0.05, capacity: int = 3_000_000, epochs: int = 4,
+          batch: int = 4096, rng_chunk: int = 25000, verbose: bool = False):
+    """Record pairs with the plain getter on ``seeds`` and train a network on
+    the GPU.  Returns (float32 parameters, info dict)."""
+    pairs, n = record_pairs(dt, seeds, capacity, rate, rng_chunk)
+    try:
+        tr = Trainer(dt, batch=batch)
+        loss0 = tr.loss_grad(pairs.ptr, min(batch, n))[0]
+        hist = tr.fit(pairs, n, epochs=epochs)
+        params = tr.params()
+        tr.free()
+    finally:
+        pairs.free()
+    if verbose:
+        print(f"[learned] {n} pairs, loss {loss0:.4f} -> {hist}", flush=True)
+    return params, dict(pairs=n, loss=[loss0] + hist)
+
+
 # ---------------------------------------------------------------------------
-# float64 restatement of the training objective (used by the tests as the
-# checker of the GPU gradient; finite differences pin it)
+# float64 restatement of the training objective (the checker of the GPU
+# gradient in the tests; finite differences pin it)
 # ---------------------------------------------------------------------------
 def forward_np(params: np.ndarray, X: np.ndarray):
     p = np.asarray(params, dtype=np.float64)
@@ -159,49 +163,17 @@ def forward_np(params: np.ndarray, X: np.ndarray):
     return h1, h2, h2 @ w3.T + b3
 
 
-def loss_np(params: np.ndarray, X: np.ndarray, U: np.ndarray):
-    """Mean negative log-likelihood of unit targets U under the mixture."""
-    _, _, O = forward_np(params, X)
-    return float(np.mean(_nll_rows(O, U)[0]))
+def loss_np(params: np.ndarray, X: np.ndarray, T: np.ndarray) -> float:
+    """Mean over pairs of the squared error averaged over the 24 outputs."""
+    return float(np.mean((forward_np(params, X)[2] - T) ** 2))
 
 
-def _nll_rows(O, U):
-    O = O.reshape(len(O), N_K, 5)
-    m = O[:, :, :3]
-    nrm = np.linalg.norm(m, axis=2)
-    mh = m / nrm[:, :, None]
-    z = O[:, :, 3]
-    kap = np.exp(np.clip(z, -4, 6))
-    d = np.einsum("nkc,nc->nk", mh, U)
-    a = O[:, :, 4]
-    lpi = a - _lse(a)[:, None]
-    logc = np.log(kap) - np.log(2 * np.pi) - kap - np.log1p(-np.exp(-2 * kap))
-    ell = lpi + logc + kap * d
-    L = _lse(ell)
-    return -L, dict(mh=mh, nrm=nrm, z=z, kap=kap, d=d, lpi=lpi, ell=ell, L=L)
-
-
-def _lse(a):
-    mx = a.max(axis=1)
-    return mx + np.log(np.exp(a - mx[:, None]).sum(axis=1))
-
-
-def grad_np(params: np.ndarray, X: np.ndarray, U: np.ndarray) -> np.ndarray:
+def grad_np(params: np.ndarray, X: np.ndarray, T: np.ndarray) -> np.ndarray:
     """Analytic gradient of loss_np (back-propagation in float64)."""
     p = np.asarray(params, dtype=np.float64)
     n = len(X)
     h1, h2, O = forward_np(p, X)
-    _, c = _nll_rows(O, U)
-    r = np.exp(c["ell"] - c["L"][:, None])
-    pi = np.exp(c["lpi"])
-    kap, d, e2 = c["kap"], c["d"], np.exp(-2 * c["kap"])
-    dO = np.zeros((n, N_K, 5))
-    dO[:, :, 4] = pi - r
-    dl = 1 - 2 * kap * e2 / (1 - e2) - kap + kap * d
-    inside = (c["z"] >= -4) & (c["z"] <= 6)
-    dO[:, :, 3] = np.where(inside, -r * dl, 0.0)
-    dO[:, :, :3] = (-r * kap / c["nrm"])[:, :, None] * (U[:, None, :] - d[:, :, None] * c["mh"])
-    dO = dO.reshape(n, N_OUT) / n
+    dO = 2.0 * (O - T) / (N_OUT * n)
     w2 = p[W2:B2].reshape(N_H, N_H)
     w3 = p[W3:B3].reshape(N_OUT, N_H)
     g = np.zeros(NPARAM)

@@ -84,29 +84,27 @@ typedef struct cuslines_params {
 } cuslines_params;
 
 /*
- * Learned PTT proposal (no counterpart in the reference).  A BF16 MLP
- * 40 -> 128 -> 128 -> 40 (ReLU) maps (28 SH coefficients of the ODF at the
- * point, the incoming direction, the 9-float PTT frame) to an 8-component
- * von Mises-Fisher mixture on the hemisphere around the frame tangent, whose
- * orthographic projection onto the (k1, k2) curvature disc proposes PTT
- * candidates.  The reference's acceptance rule (probe support >= 0.4, first
- * lane wins) decides every candidate; when no candidate of a round passes,
- * the plain getter runs the sub-step.
+ * Learned PTT proposal (no counterpart in the reference).  At every
+ * non-initial PTT sub-step the reference measures the probe support of the 24
+ * disc vertices (8 probe taps each) to build its face-CDF proposal
+ * (ptt.cu:232-300).  The learned proposal predicts those 24 supports with a
+ * BF16 MLP 40 -> 128 -> 128 -> 24 (ReLU) from cheap local inputs, then samples
+ * and accepts candidates with the reference's own face CDF and acceptance
+ * rule (support >= 0.4 over 8 probes, lowest lane wins); a round without a
+ * winner falls back to the plain getter.
+ * Inputs: [0..23] for each disc vertex, the sum of two ODF taps along its
+ * curve at arc lengths 4 and 8 probe steps; [24] the ODF at the point along
+ * the tangent; [25..27] zero; [28..30] incoming direction; [31..39] frame.
  * Parameter layout (row-major, CUSLINES_MLP_NPARAM values): W1[128][40],
- * b1[128], W2[128][128], b2[128], W3[40][128], b3[40].  Output j of component
- * c: 5c..5c+2 mean direction in frame coordinates (tangent, normal,
- * binormal), 5c+3 log concentration (clamped to [-4, 6]), 5c+4 mixture logit.
- * A training pair is CUSLINES_PAIR_FLOATS floats: the 40 inputs, the accepted
- * candidate as a unit vector (tangent, k1/MAX_CURVATURE, k2/MAX_CURVATURE)
- * and one float of padding.
+ * b1[128], W2[128][128], b2[128], W3[24][128], b3[24].
+ * A training pair is CUSLINES_PAIR_FLOATS floats: the 40 inputs and the 24
+ * measured vertex supports.
  */
-#define CUSLINES_SH_COEFFS 28
 #define CUSLINES_MLP_IN 40
 #define CUSLINES_MLP_HIDDEN 128
-#define CUSLINES_MLP_OUT 40
-#define CUSLINES_MLP_COMPONENTS 8
-#define CUSLINES_MLP_NPARAM 26920
-#define CUSLINES_PAIR_FLOATS 44
+#define CUSLINES_MLP_OUT 24
+#define CUSLINES_MLP_NPARAM 24856
+#define CUSLINES_PAIR_FLOATS 64
 
 typedef struct cuslines_tracker cuslines_tracker;
 
@@ -181,28 +179,20 @@ int cuslines_set_lanes_per_seed(cuslines_tracker *t, int lanes);
  * the vertex-major ODF.  Results are identical either way. */
 int cuslines_set_ptt_layout(cuslines_tracker *t, int corner_blocks);
 
-/* SH coefficients of the (PTT-prepared) ODF, host float32
- * [dimx][dimy][dimz][ncoef] with ncoef == CUSLINES_SH_COEFFS (order 6):
- * the first 28 inputs of the learned proposal, trilinearly interpolated.
- * No counterpart in the reference. */
-int cuslines_set_sh_coeffs(cuslines_tracker *t, const float *sh, int32_t ncoef);
-
 /* Learned proposal for PTT phase 2: host bf16 parameters (uint16 bit
  * patterns, CUSLINES_MLP_NPARAM of them, layout above); NULL returns to the
- * plain getter.  Needs cuslines_set_sh_coeffs.  The network is evaluated once
- * per output step (every STEP_FRAC sub-steps) and its mixture proposes the
- * candidates of each sub-step; a streamline whose learned rounds fail more
- * than one time in four (after 16 sub-steps) continues with the plain
- * getter.  No counterpart in the reference: it is for callers that trade the
- * reference's face-CDF proposal for fewer probe taps. */
+ * plain getter.  A streamline whose learned rounds fail more than one time in
+ * four (after 16 sub-steps) continues with the plain getter.  No counterpart
+ * in the reference: it trades the 24 x 8 vertex probe taps of each sub-step
+ * for 24 x 2 taps and the network. */
 int cuslines_set_proposal(cuslines_tracker *t, const uint16_t *mlp_bf16);
 
-/* Record training pairs during later plain PTT phase-2 calls: each accepted
+/* Record training pairs during later plain PTT phase-2 calls: each
  * non-initial sub-step is kept with probability `rate` (a hash of seed, pass
  * and sub-step), appended at atomicAdd(count_d) into pairs_d
  * [capacity][CUSLINES_PAIR_FLOATS] while there is room (count_d keeps
- * counting past capacity).  pairs_d == NULL stops recording.  Needs
- * cuslines_set_sh_coeffs.  No counterpart in the reference. */
+ * counting past capacity).  pairs_d == NULL stops recording.  No counterpart
+ * in the reference. */
 int cuslines_set_recording(cuslines_tracker *t, float *pairs_d, int32_t capacity, int32_t *count_d,
                            float rate);
 
@@ -215,9 +205,9 @@ int cuslines_proposal_stats(cuslines_tracker *t, int64_t *out3, int reset);
 /* Training of the proposal network on device buffers (no counterpart in the
  * reference).  params_d: float32 master parameters [CUSLINES_MLP_NPARAM];
  * the forward pass rounds them to bf16 as inference does.
- * cuslines_mlp_loss_grad_async: mean negative log-likelihood of the targets
- * of n pairs under the predicted mixture into loss_d[0], and its gradient
- * into grad_d [CUSLINES_MLP_NPARAM]; work_d is a caller buffer of
+ * cuslines_mlp_loss_grad_async: mean over n pairs of the squared error of
+ * the 24 predicted supports, averaged over the 24, into loss_d[0], and its
+ * gradient into grad_d [CUSLINES_MLP_NPARAM]; work_d is a caller buffer of
  * cuslines_mlp_workspace_bytes(n) bytes.
  * cuslines_mlp_adam_async: one Adam step (Kingma and Ba, 2015) with step
  * count `step` (>= 1); m_d and v_d start at zero. */

@@ -278,7 +278,6 @@ struct Problem {
     double step_d = 0;  // STEP_SIZE as the double literal the macro holds
     int nx = 1, ny = 1, nz = 1;
     // learned PTT proposal (no counterpart in the reference)
-    std::vector<float> sh;         // [X][Y][Z][28]
     std::vector<uint16_t> mlp;     // bf16 parameters; empty: plain getter
     mutable long long stats[3] = {0, 0, 0};
 };
@@ -802,23 +801,12 @@ const int DISC_FACE[31 * 3] = {
     3, 1, 6,    1, 5, 6,    11, 10, 6,  6, 5, 11,   14, 19, 15, 15, 13, 14, 14, 13, 9};
 constexpr int DISC_VERT_CNT = 24, DISC_FACE_CNT = 31;
 
-int get_direction_ptt(const Problem &P, SeedScratch &S, bool init, const float *dir, float *frame,
-                      const float *pos, float *new_dir) {
+// Face CDF over the 31 disc faces from vert_pdf and up to `rounds` rounds
+// of 32 candidates (ptt.cu:270-410).  1 with new_dir (and the frame
+// advanced), 0 when the CDF is empty or no round has a winner.
+int ptt_sample(const Problem &P, SeedScratch &S, bool init, const float *dir, float *frame, const float *pos,
+               float first_val, const float *vert_pdf, int rounds, float *new_dir) {
     const double min_support = (double)P.p.probe_quality * PMF_THRESHOLD_P;
-    if (init) { frame[0] = dir[0]; frame[1] = dir[1]; frame[2] = dir[2]; }
-    const float first_val = interp4(P, pos, frame);
-    float vert_pdf[DISC_VERT_CNT];
-    bool found = false;
-    for (int ii = 0; ii < DISC_VERT_CNT; ++ii) {  // lane ii
-        const float k1 = (float)((double)DISC_VERT[ii * 2] * (double)P.p.max_curvature);
-        const float k2 = (float)((double)DISC_VERT[ii * 2 + 1] * (double)P.p.max_curvature);
-        float pf[9];
-        get_probing_frame(init, frame, S.rng[ii], pf);
-        const float sup = data_support(P, first_val, pos, k1, k2, pf);
-        if ((double)sup < min_support) vert_pdf[ii] = 0.0f;
-        else { vert_pdf[ii] = sup; found = true; }
-    }
-    if (!found) return 0;
     float cdf[DISC_FACE_CNT];
     for (int ii = 0; ii < DISC_FACE_CNT; ++ii) {
         cdf[ii] = 0.0f;
@@ -832,7 +820,7 @@ int get_direction_ptt(const Problem &P, SeedScratch &S, bool init, const float *
     const float last = cdf[DISC_FACE_CNT - 1];
     if (last == 0.0f) return 0;
     const double mc = (double)P.p.max_curvature;
-    for (int round = 0; round < 1024 / LANES; ++round) {
+    for (int round = 0; round < rounds; ++round) {
         float sup[LANES], k1s[LANES], k2s[LANES], pfs[LANES][9];
         for (int t = 0; t < LANES; ++t) {
             Philox &rng = S.rng[t];
@@ -881,6 +869,34 @@ int get_direction_ptt(const Problem &P, SeedScratch &S, bool init, const float *
     return 0;
 }
 
+// The 24 vertex supports of a sub-step (ptt.cu:232-260) into vert_pdf (0
+// below 0.4; raw values into `raw` when given).  Returns whether any passed.
+bool ptt_vertex_supports(const Problem &P, SeedScratch &S, bool init, const float *frame, const float *pos,
+                         float first_val, float *vert_pdf, float *raw) {
+    const double min_support = (double)P.p.probe_quality * PMF_THRESHOLD_P;
+    bool found = false;
+    for (int ii = 0; ii < DISC_VERT_CNT; ++ii) {  // lane ii
+        const float k1 = (float)((double)DISC_VERT[ii * 2] * (double)P.p.max_curvature);
+        const float k2 = (float)((double)DISC_VERT[ii * 2 + 1] * (double)P.p.max_curvature);
+        float pf[9];
+        get_probing_frame(init, frame, S.rng[ii], pf);
+        const float sup = data_support(P, first_val, pos, k1, k2, pf);
+        if (raw) raw[ii] = sup;
+        if ((double)sup < min_support) vert_pdf[ii] = 0.0f;
+        else { vert_pdf[ii] = sup; found = true; }
+    }
+    return found;
+}
+
+int get_direction_ptt(const Problem &P, SeedScratch &S, bool init, const float *dir, float *frame,
+                      const float *pos, float *new_dir, float *raw = nullptr) {
+    if (init) { frame[0] = dir[0]; frame[1] = dir[1]; frame[2] = dir[2]; }
+    const float first_val = interp4(P, pos, frame);
+    float vert_pdf[DISC_VERT_CNT];
+    if (!ptt_vertex_supports(P, S, init, frame, pos, first_val, vert_pdf, raw)) return 0;
+    return ptt_sample(P, S, init, dir, frame, pos, first_val, vert_pdf, 1024 / LANES, new_dir);
+}
+
 bool init_frame_ptt(const Problem &P, SeedScratch &S, const float *first_step, const float *seed) {
     float tmp[3];
     float *frame = S.frame;
@@ -901,21 +917,39 @@ bool init_frame_ptt(const Problem &P, SeedScratch &S, const float *first_step, c
 // Learned PTT proposal (no counterpart in the reference; the layout and
 // semantics are those of include/cuslines_hip.h).  Same operation order as
 // the kernels: neurons are fma chains over their inputs in order starting
-// from the bias; the mixture draw is the vMF sampler of Wood (1994).
+// from the bias.
 // ---------------------------------------------------------------------------
-constexpr int MLP_SH = CUSLINES_SH_COEFFS, MLP_IN = CUSLINES_MLP_IN, MLP_H = CUSLINES_MLP_HIDDEN,
-              MLP_OUT = CUSLINES_MLP_OUT, MLP_K = CUSLINES_MLP_COMPONENTS;
+constexpr int MLP_IN = CUSLINES_MLP_IN, MLP_H = CUSLINES_MLP_HIDDEN, MLP_OUT = CUSLINES_MLP_OUT;
 constexpr int MLP_W1 = 0, MLP_B1 = MLP_W1 + MLP_H * MLP_IN, MLP_W2 = MLP_B1 + MLP_H,
               MLP_B2 = MLP_W2 + MLP_H * MLP_H, MLP_W3 = MLP_B2 + MLP_H, MLP_B3 = MLP_W3 + MLP_OUT * MLP_H;
-constexpr int MIX_STRIDE = 12;
+constexpr int FEAT_STEPS = 4;
 
 inline float bf16f(uint16_t h) { return bits_float((uint32_t)h << 16); }
 
-void mlp_features(const Problem &P, const float *pt, const float *dir, const float *frame, float *x) {
-    if (trilinear(P, P.sh.data(), MLP_SH, pt, x) != 0)
-        for (int j = 0; j < MLP_SH; ++j) x[j] = 0.0f;
-    for (int j = 0; j < 3; ++j) x[MLP_SH + j] = dir[j];
-    for (int j = 0; j < 9; ++j) x[MLP_SH + 3 + j] = frame[j];
+// Inputs: per disc vertex the sum of the ODF taps after 4 and 8 probe steps
+// along its curve (one propagator of 4 probe steps, applied twice), the ODF
+// at the point, three zeros, the incoming direction and the frame.
+void mlp_features(const Problem &P, const float *pos, const float *dir, const float *frame, float first_val,
+                  float *x) {
+    for (int v = 0; v < DISC_VERT_CNT; ++v) {
+        const float k1 = (float)((double)DISC_VERT[v * 2] * (double)P.p.max_curvature);
+        const float k2 = (float)((double)DISC_VERT[v * 2 + 1] * (double)P.p.max_curvature);
+        float prop[9], pf[9], d[3];
+        prepare_propagator(k1, k2, (float)FEAT_STEPS * P.p.probe_step_size, prop);
+        for (int i = 0; i < 9; ++i) pf[i] = frame[i];
+        float q[3] = {pos[0], pos[1], pos[2]};
+        propagate_frame(prop, pf, d);
+        q[0] += d[0]; q[1] += d[1]; q[2] += d[2];
+        const float t1 = interp4(P, q, pf);
+        propagate_frame(prop, pf, d);
+        q[0] += d[0]; q[1] += d[1]; q[2] += d[2];
+        const float t2 = interp4(P, q, pf);
+        x[v] = t1 + t2;
+    }
+    x[24] = first_val;
+    x[25] = x[26] = x[27] = 0.0f;
+    for (int j = 0; j < 3; ++j) x[28 + j] = dir[j];
+    for (int j = 0; j < 9; ++j) x[31 + j] = frame[j];
 }
 
 float mlp_neuron(const uint16_t *w, uint16_t bias, const float *in, int K) {
@@ -937,91 +971,24 @@ void mlp_forward(const uint16_t *W, const float *x, float *o) {
     for (int j = 0; j < MLP_OUT; ++j) o[j] = mlp_neuron(W + MLP_W3 + j * MLP_H, W[MLP_B3 + j], h2, MLP_H);
 }
 
-// Per component: unit mean (fallback +tangent), orthonormal (e1, e2) normal
-// to it, kappa = exp(clamp(., -4, 6)), exp(-2 kappa), cumulative softmax
-// weight summed in component order.
-void mixture_setup(const float *o, float *mix) {
-    float amax = o[4];
-    for (int c = 1; c < MLP_K; ++c) amax = std::fmax(amax, o[5 * c + 4]);
-    for (int c = 0; c < MLP_K; ++c) {
-        const float *oc = o + 5 * c;
-        float mx = oc[0], my = oc[1], mz = oc[2];
-        const float n2 = std::fmaf(mz, mz, std::fmaf(my, my, mx * mx));
-        if (n2 > 1e-30f && n2 <= REAL_MAX) {
-            const float inv = rsqrt_nr(n2);
-            mx *= inv; my *= inv; mz *= inv;
-        } else {
-            mx = 1.0f; my = 0.0f; mz = 0.0f;
-        }
-        const float ax = (std::fabs(mx) < 0.9f) ? 1.0f : 0.0f, ay = 1.0f - ax;
-        const float am = std::fmaf(ay, my, ax * mx);
-        float e1x = std::fmaf(-am, mx, ax), e1y = std::fmaf(-am, my, ay), e1z = -am * mz;
-        const float inv1 = rsqrt_nr(std::fmaf(e1z, e1z, std::fmaf(e1y, e1y, e1x * e1x)));
-        e1x *= inv1; e1y *= inv1; e1z *= inv1;
-        float *mc = mix + MIX_STRIDE * c;
-        mc[0] = mx; mc[1] = my; mc[2] = mz;
-        mc[3] = e1x; mc[4] = e1y; mc[5] = e1z;
-        mc[6] = std::fmaf(my, e1z, -(mz * e1y));
-        mc[7] = std::fmaf(mz, e1x, -(mx * e1z));
-        mc[8] = std::fmaf(mx, e1y, -(my * e1x));
-        const float kap = uexp(std::fmin(std::fmax(oc[3], -4.0f), 6.0f));
-        mc[9] = kap;
-        mc[10] = uexp(-2.0f * kap);
-        mc[11] = uexp(oc[4] - amax);
-    }
-    float c = 0.0f;
-    for (int k = 0; k < MLP_K; ++k) {
-        c = c + mix[MIX_STRIDE * k + 11];
-        mix[MIX_STRIDE * k + 11] = c;
-    }
-}
-
-bool mixture_draw(const Problem &P, const float *mix, Philox &rng, float *k1, float *k2) {
-    const float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
-    const float tgt = u0 * mix[MIX_STRIDE * (MLP_K - 1) + 11];
-    int c = 0;
-    for (; c < MLP_K - 1; ++c)
-        if (mix[MIX_STRIDE * c + 11] >= tgt) break;
-    const float *mc = mix + MIX_STRIDE * c;
-    float w = 1.0f + ulog(std::fmaf(1.0f - u1, mc[10], u1)) / mc[9];
-    w = std::fmin(std::fmax(w, -1.0f), 1.0f);
-    const float sw = std::sqrt(std::fmax(std::fmaf(-w, w, 1.0f), 0.0f));
-    float sn, cs;
-    usincos(6.28318531f * u2, &sn, &cs);
-    const float a1 = sw * cs, a2 = sw * sn;
-    const float ut = std::fmaf(a2, mc[6], std::fmaf(a1, mc[3], w * mc[0]));
-    const float un = std::fmaf(a2, mc[7], std::fmaf(a1, mc[4], w * mc[1]));
-    const float ub = std::fmaf(a2, mc[8], std::fmaf(a1, mc[5], w * mc[2]));
-    *k1 = P.p.max_curvature * un;
-    *k2 = P.p.max_curvature * ub;
-    return ut > 0.0f;
-}
-
-// One learned round of 32 candidates (lane order, reference acceptance
-// rule).  1 with new_dir and the frame advanced, or -1 (no winner).
-int get_direction_ptt_learned(const Problem &P, SeedScratch &S, float *frame, const float *pos, float *new_dir) {
+// Learned sub-step: predicted supports (cut at 0.4) drive one round of the
+// reference's sampler; on failure *fell = 1 and the plain getter runs.
+int get_direction_ptt_learned(const Problem &P, SeedScratch &S, bool learned, const float *dir, float *frame,
+                              const float *pos, float *new_dir, int *fell) {
     const double min_support = (double)P.p.probe_quality * PMF_THRESHOLD_P;
     const float first_val = interp4(P, pos, frame);
-    float k1s[LANES], k2s[LANES];
-    int win = -1;
-    for (int t = 0; t < LANES; ++t) {
-        float sup = -1.0f;
-        if (mixture_draw(P, S.mix, S.rng[t], &k1s[t], &k2s[t])) {
-            float pf[9];
-            for (int i = 0; i < 9; ++i) pf[i] = frame[i];
-            sup = data_support(P, first_val, pos, k1s[t], k2s[t], pf);
-        }
-        if (win < 0 && (double)sup >= min_support) win = t;
+    float vert_pdf[DISC_VERT_CNT];
+    *fell = 0;
+    if (learned) {
+        float x[MLP_IN], o[MLP_OUT];
+        mlp_features(P, pos, dir, frame, first_val, x);
+        mlp_forward(P.mlp.data(), x, o);
+        for (int v = 0; v < DISC_VERT_CNT; ++v) vert_pdf[v] = ((double)o[v] < min_support) ? 0.0f : o[v];
+        if (ptt_sample(P, S, false, dir, frame, pos, first_val, vert_pdf, 1, new_dir)) return 1;
+        *fell = 1;
     }
-    if (win < 0) return -1;
-    float prop[9], d[3], pf[9];
-    prepare_propagator(k1s[win], k2s[win], (float)(P.step_d / P.p.step_frac), prop);
-    for (int i = 0; i < 9; ++i) pf[i] = frame[i];
-    propagate_frame(prop, pf, d);
-    norm3<0>(d);
-    new_dir[0] = d[0]; new_dir[1] = d[1]; new_dir[2] = d[2];
-    for (int i = 0; i < 9; ++i) frame[i] = pf[i];
-    return 1;
+    if (!ptt_vertex_supports(P, S, false, frame, pos, first_val, vert_pdf, nullptr)) return 0;
+    return ptt_sample(P, S, false, dir, frame, pos, first_val, vert_pdf, 1024 / LANES, new_dir);
 }
 
 // ---------------------------------------------------------------------------
@@ -1046,24 +1013,17 @@ int tracker(const Problem &P, SeedScratch &S, const float *seed, const float *fi
         int ndir;
         if (model == CUSLINES_PROB) ndir = get_direction_prob(P, S, false, dir, pt, nd, nullptr);
         else if (model == CUSLINES_PTT && use_mlp) {
-            // learned proposal: the network once per output step, plain
-            // getter when a round fails or the streamline left the proposal
-            ndir = -1;
+            // learned proposal; the plain getter when a round fails or the
+            // streamline left the proposal
+            int fell = 0;
+            ndir = get_direction_ptt_learned(P, S, learned, dir, ptt_frame, pt, nd, &fell);
             if (learned) {
-                if ((i - 1) % step_frac == 0) {
-                    float x[MLP_IN], o[MLP_OUT];
-                    mlp_features(P, pt, dir, ptt_frame, x);
-                    mlp_forward(P.mlp.data(), x, o);
-                    mixture_setup(o, S.mix);
-                }
-                ndir = get_direction_ptt_learned(P, S, ptt_frame, pt, nd);
                 ++ltries;
-                if (ndir < 0) ++lfalls;
+                lfalls += fell;
                 if (ltries >= 16 && 4 * lfalls > ltries) learned = false;
             } else {
                 ++nplain;
             }
-            if (ndir < 0) ndir = get_direction_ptt(P, S, false, dir, ptt_frame, pt, nd);
         }
         else if (model == CUSLINES_PTT) ndir = get_direction_ptt(P, S, false, dir, ptt_frame, pt, nd);
         else ndir = get_direction_boot(P, S, 5, dir, pt, nd, nullptr);
@@ -1160,12 +1120,6 @@ int oracle_set_boot_matrices(oracle_problem *h, const float *H, const float *R, 
     P->smat.assign(sm, sm + (size_t)P->p.nverts * P->p.delta_nr);
     return 0;
 }
-int oracle_set_sh(oracle_problem *h, const float *sh) {
-    Problem *P = reinterpret_cast<Problem *>(h);
-    const size_t n = (size_t)P->p.dimx * P->p.dimy * P->p.dimz * MLP_SH;
-    P->sh.assign(sh, sh + n);
-    return 0;
-}
 int oracle_set_proposal(oracle_problem *h, const uint16_t *w) {
     Problem *P = reinterpret_cast<Problem *>(h);
     if (!w) P->mlp.clear();
@@ -1182,15 +1136,18 @@ int oracle_proposal_stats(oracle_problem *h, long long *out, int reset) {
 }
 // Unit entry points of the learned proposal for the oracle's own tests.
 void oracle_mlp_forward(const uint16_t *w, const float *x, float *o) { mlp_forward(w, x, o); }
-void oracle_mixture_setup(const float *o, float *mix) { mixture_setup(o, mix); }
-// n draws of lane stream (seed, subsequence): k1, k2 and validity.
-void oracle_mixture_draws(const float *mix, float max_curv, uint64_t seed, uint64_t sub, int n, float *k1,
-                          float *k2, int *valid) {
-    Problem P;
-    P.p.max_curvature = max_curv;
-    Philox rng;
-    rng.init(seed, sub);
-    for (int i = 0; i < n; ++i) valid[i] = mixture_draw(P, mix, rng, &k1[i], &k2[i]) ? 1 : 0;
+int oracle_mlp_features(oracle_problem *h, const float *pos, const float *dir, const float *frame, float *x) {
+    const Problem &P = *reinterpret_cast<Problem *>(h);
+    mlp_features(P, pos, dir, frame, interp4(P, pos, frame), x);
+    return 0;
+}
+// The 24 measured vertex supports of a non-initial sub-step (no random draws).
+int oracle_vertex_supports(oracle_problem *h, const float *pos, const float *frame, float *raw) {
+    const Problem &P = *reinterpret_cast<Problem *>(h);
+    SeedScratch S;
+    float vp[DISC_VERT_CNT];
+    ptt_vertex_supports(P, S, false, frame, pos, interp4(P, pos, frame), vp, raw);
+    return 0;
 }
 int oracle_set_ptt_lut(oracle_problem *h, const int32_t *lut) {
     Problem *P = reinterpret_cast<Problem *>(h);

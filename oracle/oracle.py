@@ -48,7 +48,6 @@ def lib():
             ("oracle_trilinear", [vp, fp, fp]),
             ("oracle_check_point", [vp, fp]),
             ("oracle_boot_odf", [vp, fp, C.c_int, C.c_int, fp, fp]),
-            ("oracle_set_sh", [vp, fp]),
             ("oracle_set_proposal", [vp, C.POINTER(C.c_uint16)]),
             ("oracle_proposal_stats", [vp, C.POINTER(C.c_longlong), C.c_int]),
         ]:
@@ -74,8 +73,8 @@ def lib():
         L.oracle_propagate_frame.argtypes = [fp, fp, fp]
         L.oracle_num_threads.restype = C.c_int
         L.oracle_mlp_forward.argtypes = [C.POINTER(C.c_uint16), fp, fp]
-        L.oracle_mixture_setup.argtypes = [fp, fp]
-        L.oracle_mixture_draws.argtypes = [fp, C.c_float, C.c_uint64, C.c_uint64, C.c_int, fp, fp, ip]
+        L.oracle_mlp_features.argtypes = [vp, fp, fp, fp, fp]
+        L.oracle_vertex_supports.argtypes = [vp, fp, fp, fp]
         _lib = L
     return _lib
 
@@ -170,16 +169,26 @@ class Oracle:
         pt = np.ascontiguousarray(pt, dtype=np.float32)
         return self.L.oracle_check_point(self.h, _f(pt))
 
-    def set_learned(self, sh, mlp_bf16):
-        """Learned PTT proposal: SH coefficients [X,Y,Z,28] and bf16
-        parameters (uint16), or mlp_bf16=None for the plain getter."""
-        self._sh = np.ascontiguousarray(sh, dtype=np.float32)
-        self.L.oracle_set_sh(self.h, _f(self._sh))
+    def set_learned(self, mlp_bf16):
+        """Learned PTT proposal: bf16 parameters (uint16), or None for the
+        plain getter."""
         if mlp_bf16 is None:
             self.L.oracle_set_proposal(self.h, None)
         else:
             w = np.ascontiguousarray(mlp_bf16, dtype=np.uint16)
             self.L.oracle_set_proposal(self.h, w.ctypes.data_as(C.POINTER(C.c_uint16)))
+
+    def mlp_features(self, pos, direction, frame):
+        x = np.zeros(40, np.float32)
+        self.L.oracle_mlp_features(self.h, _f(np.asarray(pos, np.float32)), _f(np.asarray(direction, np.float32)),
+                                   _f(np.asarray(frame, np.float32)), _f(x))
+        return x
+
+    def vertex_supports(self, pos, frame):
+        out = np.zeros(24, np.float32)
+        self.L.oracle_vertex_supports(self.h, _f(np.asarray(pos, np.float32)), _f(np.asarray(frame, np.float32)),
+                                      _f(out))
+        return out
 
     def proposal_stats(self, reset=True):
         out = (C.c_longlong * 3)()
@@ -226,25 +235,9 @@ def usincos(x):
 def mlp_forward(mlp_bf16, x):
     w = np.ascontiguousarray(mlp_bf16, dtype=np.uint16)
     x = np.ascontiguousarray(x, dtype=np.float32)
-    o = np.zeros(40, dtype=np.float32)
+    o = np.zeros(24, dtype=np.float32)
     lib().oracle_mlp_forward(w.ctypes.data_as(C.POINTER(C.c_uint16)), _f(x), _f(o))
     return o
-
-
-def mixture_setup(o):
-    o = np.ascontiguousarray(o, dtype=np.float32)
-    mix = np.zeros(96, dtype=np.float32)
-    lib().oracle_mixture_setup(_f(o), _f(mix))
-    return mix.reshape(8, 12)
-
-
-def mixture_draws(mix, max_curv, seed, sub, n):
-    mix = np.ascontiguousarray(mix, dtype=np.float32).reshape(-1)
-    k1 = np.zeros(n, np.float32)
-    k2 = np.zeros(n, np.float32)
-    ok = np.zeros(n, np.int32)
-    lib().oracle_mixture_draws(_f(mix), max_curv, seed, sub, n, _f(k1), _f(k2), _i(ok))
-    return k1, k2, ok.astype(bool)
 
 
 def uexp(x):

@@ -12,21 +12,11 @@ from conftest import packed_points
 pytestmark = pytest.mark.gpu
 
 
-def _sh(w):
-    from gpustreamlines_amd import learned as LP
-
-    key = "_sh"
-    if key not in w.extra:
-        w.extra[key] = LP.sh_coefficients(w.dataf, w.vertices)
-    return w.extra[key]
-
-
 def _tracker(w):
     from gpustreamlines_amd._device import DeviceTracker
 
     dt = DeviceTracker(w.params, device=0)
     dt.upload(w.dataf, w.stop_map, w.vertices, w.edges, ptt_lut=w.ptt_lut)
-    dt.set_sh_coeffs(_sh(w))
     return dt
 
 
@@ -47,7 +37,7 @@ def test_learned_bit_exact_vs_oracle(small_workloads):
     finally:
         dt.close()
     o = Oracle(w.params, w.dataf, w.stop_map, w.vertices, w.edges, ptt_lut=w.ptt_lut)
-    o.set_learned(_sh(w), wts)
+    o.set_learned(wts)
     r = o.track(seeds)
     ostats = o.proposal_stats()
     assert gstats.sum() > 0 and gstats[0] > 0
@@ -63,13 +53,11 @@ def test_training_gradient_matches_float64(small_workloads):
     rng = np.random.default_rng(0)
     n = 512
     X = rng.normal(size=(n, LP.N_IN)).astype(np.float32)
-    a = rng.uniform(-0.5, 0.5, size=(n, 2))
-    U = np.column_stack([np.sqrt(1 - (a ** 2).sum(1)), a]).astype(np.float32)
-    pairs = np.zeros((n, 44), np.float32)
-    pairs[:, :40] = X
-    pairs[:, 40:43] = U
+    T = rng.uniform(0, 6, size=(n, LP.N_OUT)).astype(np.float32)
+    pairs = np.zeros((n, LP.PAIR), np.float32)
+    pairs[:, :LP.N_IN] = X
+    pairs[:, LP.N_IN:LP.N_IN + LP.N_OUT] = T
     p = LP.init_params(4)
-    p[LP.W3:LP.B3] *= 10
     dt = _tracker(w)
     try:
         pd = dt.alloc(pairs.nbytes)
@@ -81,8 +69,8 @@ def test_training_gradient_matches_float64(small_workloads):
     finally:
         dt.close()
     pr = LP.from_bf16(LP.to_bf16(p)).astype(np.float64)
-    want_loss = LP.loss_np(pr, X.astype(np.float64), U.astype(np.float64))
-    want = LP.grad_np(pr, X.astype(np.float64), U.astype(np.float64))
+    want_loss = LP.loss_np(pr, X.astype(np.float64), T.astype(np.float64))
+    want = LP.grad_np(pr, X.astype(np.float64), T.astype(np.float64))
     assert abs(loss - want_loss) < 1e-4 * max(1.0, abs(want_loss))
     for s, e in [(LP.W1, LP.B1), (LP.B1, LP.W2), (LP.W2, LP.B2), (LP.B2, LP.W3), (LP.W3, LP.B3), (LP.B3, LP.NPARAM)]:
         scale = np.abs(want[s:e]).max()
@@ -91,7 +79,7 @@ def test_training_gradient_matches_float64(small_workloads):
 
 @pytest.fixture(scope="module")
 def trained(small_workloads):
-    """Pairs recorded by the plain getter on 2,000 seeds, 3 epochs of Adam."""
+    """Pairs recorded by the plain getter on 2,000 seeds, 4 epochs of Adam."""
     from gpustreamlines_amd import learned as LP
     from gpustreamlines_amd import workloads as W
 
@@ -101,29 +89,29 @@ def trained(small_workloads):
     dt = _tracker(w)
     try:
         pairs, n = LP.record_pairs(dt, seeds, capacity=400_000, rate=0.25)
-        host = pairs.download(np.zeros((n, 44), np.float32))
+        host = pairs.download(np.zeros((n, LP.PAIR), np.float32))
         tr = LP.Trainer(dt, batch=4096)
         l0 = tr.loss_grad(pairs.ptr, 4096)[0]
-        hist = tr.fit(pairs, n, epochs=3, lr=2e-3)
+        hist = tr.fit(pairs, n, epochs=4)
         params = tr.params()
         tr.free()
         pairs.free()
     finally:
         dt.close()
+    print(f"[learned] {n} pairs, loss {l0:.4f} -> {hist}", flush=True)
     return dict(n=n, pairs=host, loss0=l0, hist=hist, params=params)
 
 
-def test_recorded_pairs_are_well_formed(trained):
+def test_recorded_pairs_are_well_formed(small_workloads, trained):
+    """Recorded pairs hold finite inputs, unit frame rows and supports within
+    [0, 9] (first tap plus 8 probes of an ODF normalised to 1 per voxel)."""
     P = trained["pairs"]
     assert trained["n"] > 50_000
     assert np.isfinite(P).all()
-    u = P[:, 40:43].astype(np.float64)
-    np.testing.assert_allclose((u ** 2).sum(1), 1.0, atol=1e-5)
-    assert (u[:, 0] >= 0).all()
-    f = P[:, 31:40].reshape(-1, 3, 3).astype(np.float64)  # frame rows are unit vectors
+    f = P[:, 31:40].reshape(-1, 3, 3).astype(np.float64)
     np.testing.assert_allclose(np.linalg.norm(f, axis=2), 1.0, atol=1e-4)
-
-
+    T = P[:, 40:64]
+    assert T.min() >= 0.0 and T.max() <= 9.0 + 1e-4
 def _ks_stat(a, b):
     from scipy.stats import ks_2samp
 
@@ -138,7 +126,7 @@ def test_training_lowers_loss_and_improves_fidelity(small_workloads, trained):
     from gpustreamlines_amd import learned as LP
     from gpustreamlines_amd import workloads as W
 
-    assert trained["hist"][-1] < trained["loss0"] - 0.5
+    assert trained["hist"][-1] < 0.1 * trained["loss0"]
     w = small_workloads["ptt"]
     seeds = W.seeds_for(w, 1500, placement="mixed", seed=9)
     lens, rates = {}, {}
