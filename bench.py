@@ -117,10 +117,10 @@ def _density(slots, lens, shape):
 
 def measure_learned(args, w, dt, step, plain_lens, sld, sl, nsl, L):
     """Train the BF16 proposal network on the GPU from pairs the plain getter
-    records (20,000 seeds in the mask, 5% of accepted sub-steps, 4 epochs of
-    Adam), then time the same pass with the learned proposal.  Compares the
-    streamline lengths (two-sample KS) and, on the first 20,000 streamlines,
-    the voxel visit density with the plain pass."""
+    records (20,000 seeds in the mask, 5% of its sub-steps, 4 epochs of Adam),
+    then time the same pass with the learned proposal (no warmup pass of its
+    own).  Compares the streamline lengths (two-sample KS) and, on the first
+    20,000 streamlines, the voxel visit density with the plain pass."""
     from scipy.stats import ks_2samp
 
     from gpustreamlines_amd import learned as LP
@@ -136,7 +136,7 @@ def measure_learned(args, w, dt, step, plain_lens, sld, sl, nsl, L):
                                     rate=0.05, rng_chunk=CHUNK)
     tr = LP.Trainer(dt, batch=4096)
     loss0 = tr.loss_grad(pairs.ptr, 4096)[0]
-    hist = tr.fit(pairs, npairs, epochs=4, lr=2e-3)
+    hist = tr.fit(pairs, npairs, epochs=4)
     params = tr.params()
     tr.free()
     pairs.free()
@@ -144,10 +144,7 @@ def measure_learned(args, w, dt, step, plain_lens, sld, sl, nsl, L):
     print(f"[bench] learned proposal: {npairs} pairs, loss {loss0:.3f} -> {hist[-1]:.3f}, {t_train:.1f}s",
           file=sys.stderr, flush=True)
     dt.set_proposal(LP.to_bf16(params))
-    for i in range(args.warmup):
-        r = step()
-        print(f"[bench] learned warmup {i}: {r[3] / 1e3:.2f}s", file=sys.stderr, flush=True)
-    dt.proposal_stats(reset=True)
+    dt.proposal_stats(reset=True)  # no warmup pass: the plain warmup already loaded the code object
     times = []
     for i in range(args.steps):
         r = step()

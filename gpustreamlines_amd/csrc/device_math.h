@@ -108,27 +108,6 @@ __device__ __forceinline__ float ulog(float x) {
     return __builtin_fmaf(fe, 0.693145751953125f, __builtin_fmaf(fe, 1.4286068203094172e-06f, l1p));
 }
 
-// e^x: n = rint(x log2 e), r = x - n ln2 (two-part split), Taylor
-// polynomial of e^r to r^7 on |r| <= ln2/2, times 2^n built from exponent
-// bits.  0 below -87.3, +inf above 88.7.  Same sequence as the CPU oracle.
-__device__ __forceinline__ float uexp(float x) {
-    if (x < -87.3f) return 0.0f;
-    if (x > 88.7f) return __builtin_inff();
-    const float n = __builtin_rintf(x * 1.44269504f);
-    float r = __builtin_fmaf(-n, 0.693145751953125f, x);
-    r = __builtin_fmaf(-n, 1.4286068203094172e-06f, r);
-    float p = __builtin_fmaf(r, 1.98412698e-04f, 1.38888889e-03f);
-    p = __builtin_fmaf(r, p, 8.33333333e-03f);
-    p = __builtin_fmaf(r, p, 4.16666667e-02f);
-    p = __builtin_fmaf(r, p, 1.66666667e-01f);
-    p = __builtin_fmaf(r, p, 0.5f);
-    p = __builtin_fmaf(r, p, 1.0f);
-    p = __builtin_fmaf(r, p, 1.0f);
-    const int e = (int)n + 127;
-    if (e <= 0) return p * 1.17549435e-38f * __uint_as_float((uint32_t)(e + 126) << 23);
-    return p * __uint_as_float((uint32_t)e << 23);
-}
-
 // sin and cos: quadrant reduction by pi/2 (three-part Cody-Waite split) and
 // Taylor polynomials on |r| <= pi/4.  Same sequence as the CPU oracle.
 __device__ __forceinline__ void usincos(float x, float *sn, float *cs) {

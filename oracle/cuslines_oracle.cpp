@@ -91,31 +91,12 @@ float ulog(float x) {
     return std::fmaf(fe, 0.693145751953125f, std::fmaf(fe, 1.4286068203094172e-06f, l1p));
 }
 
-// e^x: n = rint(x log2 e), r = x - n ln2 (two-part split), Taylor
-// polynomial of e^r to r^7 on |r| <= ln2/2, times 2^n built from exponent
-// bits.  0 below -87.3, +inf above 88.7.
 inline float bits_float(uint32_t b) {
     float f;
     std::memcpy(&f, &b, 4);
     return f;
 }
-float uexp(float x) {
-    if (x < -87.3f) return 0.0f;
-    if (x > 88.7f) return INFINITY;
-    const float n = std::rint(x * 1.44269504f);
-    float r = std::fmaf(-n, 0.693145751953125f, x);
-    r = std::fmaf(-n, 1.4286068203094172e-06f, r);
-    float p = std::fmaf(r, 1.98412698e-04f, 1.38888889e-03f);
-    p = std::fmaf(r, p, 8.33333333e-03f);
-    p = std::fmaf(r, p, 4.16666667e-02f);
-    p = std::fmaf(r, p, 1.66666667e-01f);
-    p = std::fmaf(r, p, 0.5f);
-    p = std::fmaf(r, p, 1.0f);
-    p = std::fmaf(r, p, 1.0f);
-    const int e = (int)n + 127;
-    if (e <= 0) return p * 1.17549435e-38f * bits_float((uint32_t)(e + 126) << 23);
-    return p * bits_float((uint32_t)e << 23);
-}
+
 
 // sin and cos of |x| < ~1e4: q = rint(x*2/pi), r = x - q*pi/2 with a
 // three-part Cody-Waite split of pi/2, Taylor polynomials on |r| <= pi/4
@@ -1267,7 +1248,6 @@ void oracle_philox_mixed(uint64_t seed, uint64_t subseq, int n, const int *kinds
 float oracle_log(float x) { return ulog(x); }
 void oracle_sincos(float x, float *s, float *c) { usincos(x, s, c); }
 float oracle_rsqrt(float x) { return rsqrt_nr(x); }
-float oracle_exp(float x) { return uexp(x); }
 int oracle_sortnet_partner(int layer, int lane) {
     const SortNet &s = sortnet();
     if (layer < 0 || layer >= s.nlayers || lane < 0 || lane >= 32) return -1;

@@ -58,8 +58,6 @@ def lib():
         L.oracle_log.restype = C.c_float
         L.oracle_log.argtypes = [C.c_float]
         L.oracle_sincos.argtypes = [C.c_float, fp, fp]
-        L.oracle_exp.restype = C.c_float
-        L.oracle_exp.argtypes = [C.c_float]
         L.oracle_rsqrt.restype = C.c_float
         L.oracle_rsqrt.argtypes = [C.c_float]
         L.oracle_sortnet_partner.restype = C.c_int
@@ -238,10 +236,6 @@ def mlp_forward(mlp_bf16, x):
     o = np.zeros(24, dtype=np.float32)
     lib().oracle_mlp_forward(w.ctypes.data_as(C.POINTER(C.c_uint16)), _f(x), _f(o))
     return o
-
-
-def uexp(x):
-    return float(lib().oracle_exp(float(x)))
 
 
 def rsqrt(x):
