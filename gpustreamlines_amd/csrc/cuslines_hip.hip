@@ -101,6 +101,8 @@ struct DevParams {
     int step_frac, probe_quality, lut_res, nx, ny, nz, log2x, log2y;
     float probe_step, max_curv, min_support_f, ptt_seed_thr_f;
     float ptt_sub_step;  // (float)(STEP_SIZE / STEP_FRAC)
+    uint32_t cells_per_vertex;  // PTT corner blocks: (dimx+1)(dimy+1)(dimz+1)
+    int lut_brick;               // PTT LUT stored as 4x4x4 bricks
     uint32_t rec_thresh;  // record a sub-step when its hash is below this
     int rec_cap;          // capacity of the pair buffer
 };
@@ -350,24 +352,39 @@ __device__ __forceinline__ float interp4(const DevParams &P, const DevBufs &B, f
     const float rf = (float)res, rmax = (float)(res - 1);
     const float u0 = __builtin_fmaf(f0, 0.5f, 0.5f), u1 = __builtin_fmaf(f1, 0.5f, 0.5f),
                 u2 = __builtin_fmaf(f2, 0.5f, 0.5f);
-    const int i0 = (int)fminf(fmaxf(floorf(u0 * rf), 0.0f), rmax);
-    const int i1 = (int)fminf(fmaxf(floorf(u1 * rf), 0.0f), rmax);
-    const int i2 = (int)fminf(fmaxf(floorf(u2 * rf), 0.0f), rmax);
-    const int idx = (int)B.lut[((size_t)i0 * res + i1) * res + i2];
+    const uint32_t i0 = (uint32_t)fminf(fmaxf(floorf(u0 * rf), 0.0f), rmax);
+    const uint32_t i1 = (uint32_t)fminf(fmaxf(floorf(u1 * rf), 0.0f), rmax);
+    const uint32_t i2 = (uint32_t)fminf(fmaxf(floorf(u2 * rf), 0.0f), rmax);
+    // Index products use 24-bit multiplies (full rate; every operand is
+    // below 2^24: LUT coordinates, tile indices, cell coordinates).
+    uint32_t li;
+    if (P.lut_brick) {
+        const uint32_t nb = (uint32_t)res >> 2;
+        li = (__umul24(__umul24(i0 >> 2, nb) + (i1 >> 2), nb) + (i2 >> 2)) * 64u + ((i0 & 3u) << 4) +
+             ((i1 & 3u) << 2) + (i2 & 3u);
+    } else {
+        li = __umul24(__umul24(i0, (uint32_t)res) + i1, (uint32_t)res) + i2;
+    }
+    const int idx = (int)B.lut[li];
     const int tx = idx & (P.nx - 1);
     const int ty = (idx >> P.log2x) & (P.ny - 1);
     const int tz = idx >> (P.log2x + P.log2y);
-    const float xq = (float)(tx * P.dimx) + px;
-    const float yq = (float)(ty * P.dimy) + py;
-    const float zq = (float)(tz * P.dimz) + pz;
+    const int ox = (int)__umul24((uint32_t)tx, (uint32_t)P.dimx);
+    const int oy = (int)__umul24((uint32_t)ty, (uint32_t)P.dimy);
+    const int oz = (int)__umul24((uint32_t)tz, (uint32_t)P.dimz);
+    const float xq = (float)ox + px;
+    const float yq = (float)oy + py;
+    const float zq = (float)oz + pz;
     if (B.blocks) {
         int i, j, k;
         const float a = tex_weight(xq, &i), b = tex_weight(yq, &j), g = tex_weight(zq, &k);
-        const int cx = i - tx * P.dimx, cy = j - ty * P.dimy, cz = k - tz * P.dimz;
+        const int cx = i - ox, cy = j - oy, cz = k - oz;
         if ((unsigned)(cx + 1) <= (unsigned)P.dimx && (unsigned)(cy + 1) <= (unsigned)P.dimy &&
             (unsigned)(cz + 1) <= (unsigned)P.dimz) {
-            const size_t cell = (((size_t)idx * (P.dimx + 1) + (cx + 1)) * (P.dimy + 1) + (cy + 1)) * (P.dimz + 1) +
-                                (cz + 1);
+            const uint32_t inner =
+                __umul24(__umul24((uint32_t)(cx + 1), (uint32_t)(P.dimy + 1)) + (uint32_t)(cy + 1),
+                         (uint32_t)(P.dimz + 1)) + (uint32_t)(cz + 1);
+            const uint64_t cell = (uint64_t)(uint32_t)idx * P.cells_per_vertex + inner;
             const float4 *blk = reinterpret_cast<const float4 *>(B.blocks + cell * 8);
             const float4 lo = blk[0], hi = blk[1];
             const float a0 = 1.0f - a, b0 = 1.0f - b, g0 = 1.0f - g;
@@ -2249,6 +2266,12 @@ int cuslines_tracker_create(const cuslines_params *p, int device, cuslines_track
             return fail(CUSLINES_EINVAL, "invalid PTT parameters");
         const long long slots = 1LL << (p->tile_log2_x + p->tile_log2_y + p->tile_log2_z);
         if (slots < p->dimt) return fail(CUSLINES_EINVAL, "PTT tiling has fewer slots than vertices");
+        // limits of the 24-bit index products in interp4
+        const long long X1 = p->dimx + 1LL, Y1 = p->dimy + 1LL, Z1 = p->dimz + 1LL;
+        if (p->lut_res > 1024 || X1 * Y1 >= (1LL << 24) || X1 * Y1 * Z1 >= (1LL << 32) ||
+            (1LL << p->tile_log2_x) * p->dimx >= (1LL << 24) || (1LL << p->tile_log2_y) * p->dimy >= (1LL << 24) ||
+            (1LL << p->tile_log2_z) * p->dimz >= (1LL << 24))
+            return fail(CUSLINES_EINVAL, "PTT volume or LUT too large for the tap index arithmetic");
     }
     if ((p->model_type == CUSLINES_OPDT || p->model_type == CUSLINES_CSA) && p->delta_nr <= 0)
         return fail(CUSLINES_EINVAL, "bootstrap needs delta_nr > 0");
@@ -2306,6 +2329,7 @@ int cuslines_tracker_create(const cuslines_params *p, int device, cuslines_track
     d.min_support_f = smallest_float_at_least((double)p->probe_quality * PMF_THRESHOLD_P);
     d.ptt_seed_thr_f = smallest_float_at_least(PMF_THRESHOLD_P);
     d.ptt_sub_step = (float)(d.step_d / (double)d.step_frac);
+    d.cells_per_vertex = (uint32_t)((uint64_t)(p->dimx + 1) * (p->dimy + 1) * (p->dimz + 1));
     *out = t;
     return CUSLINES_OK;
 }
@@ -2445,10 +2469,22 @@ int cuslines_set_ptt_lut(cuslines_tracker *t, const int32_t *lut) {
     HIPC(hipSetDevice(t->device));
     const size_t r = (size_t)t->dp.lut_res;
     std::vector<uint16_t> h(r * r * r);
-    for (size_t i = 0; i < h.size(); ++i) {
-        if (lut[i] < 0 || lut[i] >= t->dp.nverts) return fail(CUSLINES_EINVAL, "LUT vertex index out of range");
-        h[i] = (uint16_t)lut[i];
-    }
+    // Unit directions touch a thin shell of the cube: with res % 4 == 0 the
+    // table is stored as 4x4x4 bricks of 128 bytes (one cache line), so the
+    // shell spans ~10x fewer lines than with rows along the last axis.
+    const bool brick = (r % 4) == 0;
+    const size_t nb = r / 4;
+    for (size_t i0 = 0; i0 < r; ++i0)
+        for (size_t i1 = 0; i1 < r; ++i1)
+            for (size_t i2 = 0; i2 < r; ++i2) {
+                const int32_t v = lut[(i0 * r + i1) * r + i2];
+                if (v < 0 || v >= t->dp.nverts) return fail(CUSLINES_EINVAL, "LUT vertex index out of range");
+                const size_t dst = brick ? (((i0 >> 2) * nb + (i1 >> 2)) * nb + (i2 >> 2)) * 64 +
+                                               ((i0 & 3) << 4) + ((i1 & 3) << 2) + (i2 & 3)
+                                         : (i0 * r + i1) * r + i2;
+                h[dst] = (uint16_t)v;
+            }
+    t->dp.lut_brick = brick ? 1 : 0;
     dfree(t->d_lut);
     t->d_lut = nullptr;
     HIPC(hipMalloc(&t->d_lut, h.size() * sizeof(uint16_t)));
