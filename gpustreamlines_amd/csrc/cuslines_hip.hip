@@ -732,14 +732,8 @@ struct PttLds {
 };
 
 // Face CDF over the 31 disc faces from S.vert_pdf (prefix_sum_sh_d, one
-// block of 32) and up to `rounds` rejection rounds of 32 candidates
-// (ptt.cu:270-410).  Returns 1 with the new direction, 0 when the CDF is
-// empty or no round has a winner.  `wk` (optional) receives the winner's
-// curvature.
-template <bool INIT>
-__device__ int ptt_sample(const DevParams &P, const DevBufs &B, PttLds &S, float *frame, const float *fr,
-                          Philox &rng, float first_val, int rounds, float dx, float dy, float dz, float px, float py,
-                          float pz, float *ndx, float *ndy, float *ndz, float *wk) {
+// block of 32); returns the total.
+__device__ __forceinline__ float ptt_face_cdf(PttLds &S) {
     const int l = lane32();
     float face = 0.0f;
     if (l < DISC_FACE_CNT) {
@@ -758,27 +752,71 @@ __device__ int ptt_sample(const DevParams &P, const DevBufs &B, PttLds &S, float
     if (l < DISC_FACE_CNT) S.face_cdf[l] = face;
     const float last = shfl32(face, DISC_FACE_CNT - 1);
     WAVE_SYNC();
-    if (last == 0.0f) return 0;
+    return last;
+}
+
+// One PTT sub-step (ptt.cu:194-412) as a single loop over phases: phase 0
+// measures the 24 vertex supports (lanes < 24) into S.vert_pdf, phases
+// 1..rounds are rejection rounds of 32 candidates drawn from the face CDF.
+// Every phase ends in the same data_support call, so the two seeds of a
+// wavefront probe together whichever phase each one is in.  first_phase 1
+// starts from supports already in S.vert_pdf.  Returns 1 with the new
+// direction, 0 when no vertex passes, the CDF is empty or no round has a
+// winner.  `wk` (optional) receives the winner's curvature, `raw` (optional,
+// LDS [24]) the vertex supports before the 0.4 cut.
+template <bool INIT>
+__device__ int ptt_substep(const DevParams &P, const DevBufs &B, PttLds &S, float *frame, const float *fr,
+                           Philox &rng, float first_val, int first_phase, int rounds, float dx, float dy, float dz,
+                           float px, float py, float pz, float *ndx, float *ndy, float *ndz, float *wk, float *raw) {
+    const int l = lane32();
     const double mc = (double)P.max_curv;
-    for (int round = 0; round < rounds; ++round) {
-        float r1 = rng.uniform(), r2 = rng.uniform();
-        if (r1 + r2 > 1.0f) { r1 = 1.0f - r1; r2 = 1.0f - r2; }
-        const float tmp = rng.uniform() * last;
-        int jj = 0;
-        for (; jj < DISC_FACE_CNT - 1; ++jj)
-            if (S.face_cdf[jj] >= tmp) break;
-        const int f0 = c_disc_face[jj * 3], f1 = c_disc_face[jj * 3 + 1], f2 = c_disc_face[jj * 3 + 2];
-        const float vx0 = (float)(mc * (double)c_disc_vert[f0 * 2]);
-        const float vx1 = (float)(mc * (double)c_disc_vert[f1 * 2]);
-        const float vx2 = (float)(mc * (double)c_disc_vert[f2 * 2]);
-        const float vy0 = (float)(mc * (double)c_disc_vert[f0 * 2 + 1]);
-        const float vy1 = (float)(mc * (double)c_disc_vert[f1 * 2 + 1]);
-        const float vy2 = (float)(mc * (double)c_disc_vert[f2 * 2 + 1]);
-        const float k1 = __builtin_fmaf(r2, vx2 - vx0, __builtin_fmaf(r1, vx1 - vx0, vx0));
-        const float k2 = __builtin_fmaf(r2, vy2 - vy0, __builtin_fmaf(r1, vy1 - vy0, vy0));
+    float last = 0.0f;
+    for (int ph = first_phase; ph <= rounds; ++ph) {
+        if (ph == 1) {
+            last = ptt_face_cdf(S);
+            if (last == 0.0f) return 0;
+        }
+        float k1 = 0.0f, k2 = 0.0f;
+        bool active = true;
+        if (ph == 0) {
+            active = l < DISC_VERT_CNT;
+            if (active) {
+                k1 = (float)((double)c_disc_vert[2 * l] * mc);
+                k2 = (float)((double)c_disc_vert[2 * l + 1] * mc);
+            }
+        } else {
+            float r1 = rng.uniform(), r2 = rng.uniform();
+            if (r1 + r2 > 1.0f) { r1 = 1.0f - r1; r2 = 1.0f - r2; }
+            const float tmp = rng.uniform() * last;
+            int jj = 0;
+            for (; jj < DISC_FACE_CNT - 1; ++jj)
+                if (S.face_cdf[jj] >= tmp) break;
+            const int f0 = c_disc_face[jj * 3], f1 = c_disc_face[jj * 3 + 1], f2 = c_disc_face[jj * 3 + 2];
+            const float vx0 = (float)(mc * (double)c_disc_vert[f0 * 2]);
+            const float vx1 = (float)(mc * (double)c_disc_vert[f1 * 2]);
+            const float vx2 = (float)(mc * (double)c_disc_vert[f2 * 2]);
+            const float vy0 = (float)(mc * (double)c_disc_vert[f0 * 2 + 1]);
+            const float vy1 = (float)(mc * (double)c_disc_vert[f1 * 2 + 1]);
+            const float vy2 = (float)(mc * (double)c_disc_vert[f2 * 2 + 1]);
+            k1 = __builtin_fmaf(r2, vx2 - vx0, __builtin_fmaf(r1, vx1 - vx0, vx0));
+            k2 = __builtin_fmaf(r2, vy2 - vy0, __builtin_fmaf(r1, vy1 - vy0, vy0));
+        }
         float pf[9];
-        probing_frame<INIT>(fr, rng, pf);
-        const float sup = data_support(P, B, first_val, px, py, pz, k1, k2, pf);
+        float sup = 0.0f;
+        if (active) {
+            probing_frame<INIT>(fr, rng, pf);
+            sup = data_support(P, B, first_val, px, py, pz, k1, k2, pf);
+        }
+        if (ph == 0) {
+            const bool ok = active && !(sup < P.min_support_f);
+            if (active) {
+                S.vert_pdf[l] = ok ? sup : 0.0f;
+                if (raw) raw[l] = sup;
+            }
+            if (half_ballot(ok) == 0u) return 0;
+            WAVE_SYNC();
+            continue;
+        }
         uint32_t msk;
         if (INIT) {
             float mx = sup;
@@ -815,7 +853,6 @@ __device__ int ptt_sample(const DevParams &P, const DevBufs &B, PttLds &S, float
 }
 
 // get_direction_ptt_d (ptt.cu:194-412); `frame` is the 9-float frame in use.
-// `raw` (optional, LDS [24]) receives the vertex supports before the 0.4 cut.
 template <bool INIT>
 __device__ int get_direction_ptt(const DevParams &P, const DevBufs &B, PttLds &S, float *frame,
                                  Philox &rng, float dx, float dy, float dz, float px, float py, float pz,
@@ -829,21 +866,8 @@ __device__ int get_direction_ptt(const DevParams &P, const DevBufs &B, PttLds &S
 #pragma unroll
     for (int i = 0; i < 9; ++i) fr[i] = frame[i];
     const float first_val = interp4(P, B, px, py, pz, fr[0], fr[1], fr[2]);
-    bool found = false;
-    if (l < DISC_VERT_CNT) {
-        const float k1 = (float)((double)c_disc_vert[2 * l] * (double)P.max_curv);
-        const float k2 = (float)((double)c_disc_vert[2 * l + 1] * (double)P.max_curv);
-        float pf[9];
-        probing_frame<INIT>(fr, rng, pf);
-        const float sup = data_support(P, B, first_val, px, py, pz, k1, k2, pf);
-        const bool ok = !(sup < P.min_support_f);
-        S.vert_pdf[l] = ok ? sup : 0.0f;
-        if (raw) raw[l] = sup;
-        found = ok;
-    }
-    if (half_ballot(found) == 0u) return 0;
-    WAVE_SYNC();
-    return ptt_sample<INIT>(P, B, S, frame, fr, rng, first_val, 1024 / 32, dx, dy, dz, px, py, pz, ndx, ndy, ndz, wk);
+    return ptt_substep<INIT>(P, B, S, frame, fr, rng, first_val, 0, 1024 / 32, dx, dy, dz, px, py, pz, ndx, ndy, ndz,
+                             wk, raw);
 }
 
 // init_frame_ptt_d (ptt.cu:415-474); frame[15..17] is set to the binormal.
@@ -982,32 +1006,20 @@ __device__ int get_direction_ptt_learned(const DevParams &P, const DevBufs &B, P
     for (int i = 0; i < 9; ++i) fr[i] = frame[i];
     const float first_val = interp4(P, B, px, py, pz, fr[0], fr[1], fr[2]);
     *fell = 0;
-    for (int attempt = learned ? 0 : 1; attempt < 2; ++attempt) {
-        if (attempt == 0) {
-            mlp_features(P, B, px, py, pz, dx, dy, dz, fr, first_val, S.act + ACT_X);
-            mlp_forward(B.mlp, S.act);
-            if (l < DISC_VERT_CNT) {
-                const float sup = S.act[ACT_O + l];
-                S.vert_pdf[l] = (sup < P.min_support_f) ? 0.0f : sup;
-            }
-            WAVE_SYNC();
-        } else {
-            bool found = false;
-            if (l < DISC_VERT_CNT) {
-                const float k1 = (float)((double)c_disc_vert[2 * l] * (double)P.max_curv);
-                const float k2 = (float)((double)c_disc_vert[2 * l + 1] * (double)P.max_curv);
-                float pf[9];
-#pragma unroll
-                for (int i = 0; i < 9; ++i) pf[i] = fr[i];
-                const float sup = data_support(P, B, first_val, px, py, pz, k1, k2, pf);
-                found = !(sup < P.min_support_f);
-                S.vert_pdf[l] = found ? sup : 0.0f;
-            }
-            if (half_ballot(found) == 0u) return 0;
-            WAVE_SYNC();
+    if (learned) {
+        mlp_features(P, B, px, py, pz, dx, dy, dz, fr, first_val, S.act + ACT_X);
+        mlp_forward(B.mlp, S.act);
+        if (l < DISC_VERT_CNT) {
+            const float sup = S.act[ACT_O + l];
+            S.vert_pdf[l] = (sup < P.min_support_f) ? 0.0f : sup;
         }
-        const int r = ptt_sample<false>(P, B, S, frame, fr, rng, first_val, attempt ? 1024 / 32 : 1, dx, dy, dz, px,
-                                        py, pz, ndx, ndy, ndz, nullptr);
+        WAVE_SYNC();
+    }
+    // learned: one round from the predicted supports, then (fell back) the
+    // plain sub-step; the loop keeps a single ptt_substep call site
+    for (int attempt = learned ? 0 : 1; attempt < 2; ++attempt) {
+        const int r = ptt_substep<false>(P, B, S, frame, fr, rng, first_val, attempt ? 0 : 1, attempt ? 1024 / 32 : 1,
+                                         dx, dy, dz, px, py, pz, ndx, ndy, ndz, nullptr, nullptr);
         if (r || attempt) return r;
         *fell = 1;
     }
