@@ -321,7 +321,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": f"phase2_queue_kernel<{MODEL_IDS[args.getter]}>",
+            "kernel": f"phase2_queue_kernel<{MODEL_IDS[args.getter]}, 0>",
             "alg_bytes_per_step": abytes,
             "alg_bytes_per_launch": abytes * steps_per_pass,
         },
