@@ -165,37 +165,47 @@ __device__ void prefix_sum(float *a, int len) {
 
 // peak_directions_d (tracking_helpers.cu:75-245).  odf, marks and cand are
 // this slot's LDS arrays; the kept vertex indices end in cand[0..return).
-__device__ int peak_directions(const DevParams &P, const DevBufs &B, const float *odf, int *marks,
-                               int *cand, const float *sv) {
+// marks holds 2 bits per vertex, 16 vertices per word: bit 1 "has a strictly
+// larger neighbour", bit 0 "has a strictly smaller one"; the reference's
+// positive mark is exactly "bit 0 only".
+__device__ __forceinline__ uint32_t peak_mark(const uint32_t *marks, int j) {
+    return (marks[j >> 4] >> (2 * (j & 15))) & 3u;
+}
+
+__device__ int peak_directions(const DevParams &P, const DevBufs &B, const float *odf, uint32_t *marks,
+                               uint16_t *cand, const float *sv) {
     const int l = lane32();
     const int nv = P.nverts;
     float mn = REAL_MAX;
     for (int j = l; j < nv; j += 32) mn = (mn < odf[j]) ? mn : odf[j];
     mn = min32(mn);
     mn = (0.0f > mn) ? 0.0f : mn;
-    for (int j = l; j < nv; j += 32) marks[j] = 0;
+    for (int w = l; w < (nv + 15) >> 4; w += 32) marks[w] = 0u;
     WAVE_SYNC();
-    // bit 1: has a strictly larger neighbour; bit 0: has a strictly smaller one.
-    // The reference's positive mark is exactly "bit 0 only".
     for (int e = l; e < P.nedges; e += 32) {
         const uint32_t uv = B.edges[e];
         const int u = (int)(uv & 0xffffu), v = (int)(uv >> 16);
         const float a = odf[u], b = odf[v];
-        if (a < b) { atomicOr(&marks[u], 2); atomicOr(&marks[v], 1); }
-        else if (b < a) { atomicOr(&marks[v], 2); atomicOr(&marks[u], 1); }
+        if (a < b) {
+            atomicOr(&marks[u >> 4], 2u << (2 * (u & 15)));
+            atomicOr(&marks[v >> 4], 1u << (2 * (v & 15)));
+        } else if (b < a) {
+            atomicOr(&marks[v >> 4], 2u << (2 * (v & 15)));
+            atomicOr(&marks[u >> 4], 1u << (2 * (u & 15)));
+        }
     }
     WAVE_SYNC();
     float m = REAL_MIN;
     for (int j = l; j < nv; j += 32)
-        if (marks[j] == 1) { const float x = odf[j] + (-mn); m = (m > x) ? m : x; }
+        if (peak_mark(marks, j) == 1u) { const float x = odf[j] + (-mn); m = (m > x) ? m : x; }
     m = max32(m);
     const float comp = (float)((double)P.rel_peak * (double)m);
     int n = 0;
     for (int j0 = 0; j0 < nv; j0 += 32) {
         const int j = j0 + l;
-        const bool keep = (j < nv) && (marks[j] == 1) && ((odf[j] - mn) >= comp);
+        const bool keep = (j < nv) && (peak_mark(marks, j) == 1u) && ((odf[j] - mn) >= comp);
         const uint32_t msk = half_ballot(keep);
-        if (keep) cand[n + __popc(msk & ((1u << l) - 1u))] = j;
+        if (keep) cand[n + __popc(msk & ((1u << l) - 1u))] = (uint16_t)j;
         n += __popc(msk);
     }
     WAVE_SYNC();
@@ -204,7 +214,7 @@ __device__ int peak_directions(const DevParams &P, const DevBufs &B, const float
         int v = 0;
         if (l < n) { v = cand[l]; k = odf[v]; }
         sort_dec(k, v);
-        if (l < n) cand[l] = v;
+        if (l < n) cand[l] = (uint16_t)v;
         WAVE_SYNC();
     }
     if (n == 0) return 0;
@@ -222,7 +232,7 @@ __device__ int peak_directions(const DevParams &P, const DevBufs &B, const float
                 if (P.sphere_symm) c = fabsf(c);
                 if (c > P.cos_min_sep) break;
             }
-            if (j == kept) cand[kept++] = ci;
+            if (j == kept) cand[kept++] = (uint16_t)ci;
         }
     }
     kept = shfl32(kept, 0);
@@ -462,7 +472,8 @@ __device__ void gemv_rows(int N, int M, const float *v, const float *__restrict_
 // -------------------------------------------------------------------------
 struct ProbLds {
     float *pmf;
-    int *marks, *cand;
+    uint32_t *marks;
+    uint16_t *cand;
 };
 
 template <bool IS_START>
@@ -522,14 +533,14 @@ struct BootLds {
     float *vox, *msk;  // region A: vox [t32], msk [hr32]
     float *r;          // region B: R.d / model coefficients; later peak candidates
     float *h;          // region C: H.d / ODF (pmf)
-    int *marks;        // region A reused as peak marks
-    int *cand;         // region B reused as peak candidates
+    uint32_t *marks;   // peak marks, 2 bits per vertex
+    uint16_t *cand;    // region B reused as peak candidates
 };
 
 // closest_peak_d (boot.cu:115-193): lane-strided best |dot| with strict
 // improvement, then an xor tree with the same rule; lane 0's winner counts.
 __device__ int closest_peak(const DevParams &P, const float *sv, float dx, float dy, float dz, int npeaks,
-                            const int *peaks, float *ox, float *oy, float *oz) {
+                            const uint16_t *peaks, float *ox, float *oy, float *oz) {
     const int l = lane32();
     float cd = 0.0f;
     int ci = -1;
@@ -1592,16 +1603,21 @@ __global__ void __launch_bounds__(256) phase2_ptt_queue_kernel(DevParams P, DevB
 // -------------------------------------------------------------------------
 
 
-__host__ __device__ inline int boot_region_a(int nv32, int t32, int hr32) {
-    return (nv32 > t32 + hr32) ? nv32 : t32 + hr32;
+// Floats of LDS per seed slot.  Peak marks take nv32/16 words, peak
+// candidates nv32/2 floats (uint16 indices).
+__host__ __device__ inline int marks_floats(int nv32) { return ((nv32 >> 4) + 3) & ~3; }
+__host__ __device__ inline int cand_floats(int nv32) { return ((nv32 >> 1) + 3) & ~3; }
+__host__ __device__ inline int boot_region_b(int nv32, int hr32) {
+    const int c = cand_floats(nv32);
+    return (c > hr32) ? c : hr32;
 }
 
-// Floats of LDS per seed slot.
 __host__ __device__ inline int slot_floats(int model, bool phase1, int nv32, int t32, int hr32) {
-    if (model == CUSLINES_PTT) return phase1 ? 3 * nv32 : PTT_SLOT_FLOATS;
-    if (model == CUSLINES_PROB) return phase1 ? 3 * nv32 : nv32;
-    const int bc = (nv32 > hr32) ? nv32 : hr32;
-    return boot_region_a(nv32, t32, hr32) + 2 * bc;
+    if (model == CUSLINES_PTT && !phase1) return PTT_SLOT_FLOATS;
+    if (model == CUSLINES_PROB && !phase1) return nv32;
+    if (model == CUSLINES_PROB || model == CUSLINES_PTT) return nv32 + marks_floats(nv32) + cand_floats(nv32);
+    const int c = (nv32 > hr32) ? nv32 : hr32;
+    return t32 + hr32 + marks_floats(nv32) + boot_region_b(nv32, hr32) + c;
 }
 
 template <int MODEL>
@@ -1613,19 +1629,24 @@ __device__ void bind_slot(const DevParams &P, bool phase1, float *base, SlotLds 
         S.ptt.act = base + 96;
     } else if (MODEL == CUSLINES_PROB || MODEL == CUSLINES_PTT) {
         S.prob.pmf = base;
-        S.prob.marks = reinterpret_cast<int *>(base + P.nv32);
-        S.prob.cand = reinterpret_cast<int *>(base + 2 * P.nv32);
+        S.prob.marks = reinterpret_cast<uint32_t *>(base + P.nv32);
+        S.prob.cand = reinterpret_cast<uint16_t *>(base + P.nv32 + marks_floats(P.nv32));
     } else {
-        const int a = boot_region_a(P.nv32, P.t32, P.hr32);
-        const int bc = (P.nv32 > P.hr32) ? P.nv32 : P.hr32;
         S.boot.vox = base;
         S.boot.msk = base + P.t32;
-        S.boot.marks = reinterpret_cast<int *>(base);
-        S.boot.r = base + a;
-        S.boot.cand = reinterpret_cast<int *>(base + a);
-        S.boot.h = base + a + bc;
+        float *m = base + P.t32 + P.hr32;
+        S.boot.marks = reinterpret_cast<uint32_t *>(m);
+        float *b = m + marks_floats(P.nv32);
+        S.boot.r = b;
+        S.boot.cand = reinterpret_cast<uint16_t *>(b);
+        S.boot.h = b + boot_region_b(P.nv32, P.hr32);
     }
 }
+
+// Sphere vertices are staged in LDS for prob and PTT phase 1; bootstrap
+// reads them from global memory (L1/L2), leaving its LDS to the seed slots.
+template <int MODEL>
+__host__ __device__ inline bool verts_in_lds() { return MODEL == CUSLINES_PROB || MODEL == CUSLINES_PTT; }
 
 __device__ __forceinline__ int verts_floats(const DevParams &P) { return (3 * P.nverts + 3) & ~3; }
 
@@ -1636,22 +1657,27 @@ __global__ void __launch_bounds__(256) phase1_kernel(DevParams P, DevBufs B, con
                                                      int nseeds, int rng_chunk, int *__restrict__ counts,
                                                      int *__restrict__ dirs) {
     extern __shared__ float smem[];
-    float *sv = smem;
-    for (int i = threadIdx.x; i < 3 * P.nverts; i += blockDim.x) sv[i] = B.verts[i];
+    const float *sv = B.verts;
+    int svf = 0;
+    if (verts_in_lds<MODEL>()) {
+        for (int i = threadIdx.x; i < 3 * P.nverts; i += blockDim.x) smem[i] = B.verts[i];
+        sv = smem;
+        svf = verts_floats(P);
+    }
     __syncthreads();
     const int nslots = blockDim.x >> 5;
     const int slot = threadIdx.x >> 5;
     const int s = blockIdx.x * nslots + slot;
     if (s >= nseeds) return;
     const int l = lane32();
-    float *base = smem + verts_floats(P) + slot * slot_floats(MODEL, true, P.nv32, P.t32, P.hr32);
+    float *base = smem + svf + slot * slot_floats(MODEL, true, P.nv32, P.t32, P.hr32);
     SlotLds S;
     bind_slot<MODEL>(P, true, base, S);
     const float px = seeds[3 * s], py = seeds[3 * s + 1], pz = seeds[3 * s + 2];
     Philox rng;
     rng.init(((uint64_t)P.seed_hi << 32) | P.seed_lo, (uint64_t)32 * (uint64_t)(s % rng_chunk) + l);
     int n;
-    const int *kept;
+    const uint16_t *kept;
     if (MODEL == CUSLINES_PROB) {
         float a, b, c;
         n = get_direction_prob<true>(P, B, sv, S.prob, rng, 0.0f, 0.0f, 0.0f, px, py, pz, &a, &b, &c);
@@ -1737,10 +1763,11 @@ __global__ void __launch_bounds__(256) phase2_kernel(DevParams P, DevBufs B, con
                                                      const int *__restrict__ dirs, int *__restrict__ sline_seed,
                                                      int *__restrict__ sline_len, float *__restrict__ sline) {
     extern __shared__ float smem[];
-    float *sv = smem;
+    const float *sv = B.verts;
     int svf = 0;
-    if (MODEL != CUSLINES_PTT) {
-        for (int i = threadIdx.x; i < 3 * P.nverts; i += blockDim.x) sv[i] = B.verts[i];
+    if (MODEL == CUSLINES_PROB) {  // the cone test reads every vertex at every step
+        for (int i = threadIdx.x; i < 3 * P.nverts; i += blockDim.x) smem[i] = B.verts[i];
+        sv = smem;
         svf = verts_floats(P);
     }
     __syncthreads();
@@ -1757,18 +1784,21 @@ __global__ void __launch_bounds__(256) phase2_kernel(DevParams P, DevBufs B, con
 // next seed from a global counter as soon as its previous seed is done, so
 // seeds of very uneven length keep every slot busy.  `work` must hold 0.
 // Plain PTT is held to 128 VGPRs (4 waves per SIMD): the few spills cost less
-// than the latency hiding gained (+8% at 200k seeds).
+// than the latency hiding gained (+8% at 200k seeds).  Bootstrap is held to
+// 168 VGPRs (3 waves per SIMD), which its LDS per slot also allows.
 template <int MODEL, int V = 0>
-__global__ void __launch_bounds__(256, (MODEL == CUSLINES_PTT && V != 2) ? 4 : 1) phase2_queue_kernel(DevParams P, DevBufs B, const float *__restrict__ seeds,
+__global__ void __launch_bounds__(256, (MODEL == CUSLINES_PTT && V != 2) ? 4 : (MODEL <= CUSLINES_CSA ? 3 : 1))
+    phase2_queue_kernel(DevParams P, DevBufs B, const float *__restrict__ seeds,
                                                            int nseeds, int rng_chunk, const int *__restrict__ offsets,
                                                            const int *__restrict__ dirs, int *__restrict__ sline_seed,
                                                            int *__restrict__ sline_len, float *__restrict__ sline,
                                                            int *__restrict__ work) {
     extern __shared__ float smem[];
-    float *sv = smem;
+    const float *sv = B.verts;
     int svf = 0;
-    if (MODEL != CUSLINES_PTT) {
-        for (int i = threadIdx.x; i < 3 * P.nverts; i += blockDim.x) sv[i] = B.verts[i];
+    if (MODEL == CUSLINES_PROB) {  // the cone test reads every vertex at every step
+        for (int i = threadIdx.x; i < 3 * P.nverts; i += blockDim.x) smem[i] = B.verts[i];
+        sv = smem;
         svf = verts_floats(P);
     }
     __syncthreads();
@@ -2130,8 +2160,8 @@ static int launch_phase1(cuslines_tracker *t, const float *seeds, int n, int chu
     const DevParams &d = t->dp;
     const int threads = 128;
     const int nslots = threads / 32;
-    const size_t lds = sizeof(float) * ((size_t)((3 * d.nverts + 3) & ~3) +
-                                        (size_t)nslots * slot_floats(MODEL, true, d.nv32, d.t32, d.hr32));
+    const size_t vf = verts_in_lds<MODEL>() ? (size_t)((3 * d.nverts + 3) & ~3) : 0;
+    const size_t lds = sizeof(float) * (vf + (size_t)nslots * slot_floats(MODEL, true, d.nv32, d.t32, d.hr32));
     if (lds > 160 * 1024) return fail(CUSLINES_EINVAL, "sphere too large for LDS");
     HIPC(hipFuncSetAttribute((const void *)phase1_kernel<MODEL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const unsigned grid = (unsigned)((n + nslots - 1) / nslots);
@@ -2180,7 +2210,7 @@ static int launch_phase2_v(cuslines_tracker *t, const float *seeds, int n, int c
     int threads = t->threads;
     if (threads <= 0) threads = (MODEL == CUSLINES_OPDT || MODEL == CUSLINES_CSA) ? 128 : 256;
     const int nslots = threads / 32;
-    const size_t vf = (MODEL == CUSLINES_PTT) ? 0 : (size_t)((3 * d.nverts + 3) & ~3);
+    const size_t vf = (MODEL == CUSLINES_PROB) ? (size_t)((3 * d.nverts + 3) & ~3) : 0;
     const size_t lds = sizeof(float) * (vf + (size_t)nslots * slot_floats(MODEL, false, d.nv32, d.t32, d.hr32));
     if (lds > 160 * 1024) return fail(CUSLINES_EINVAL, "sphere too large for LDS");
     if (t->schedule == CUSLINES_SCHED_STATIC) {
