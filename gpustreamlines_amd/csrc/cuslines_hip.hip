@@ -41,6 +41,7 @@ constexpr float CSA_C0 = 0.28209479177387814f;         // boot.cu:363
 constexpr int DISC_VERT_CNT = 24;                      // disc.h:20
 constexpr int DISC_FACE_CNT = 31;                      // disc.h:21
 constexpr int SORT_LAYERS = 15;
+constexpr int QUEUES = 8;  // persistent-queue counters, one per XCD
 
 // PTT disc tables (quality 2): numeric data of cuslines/cuda_c/disc.h:20-84
 // (nibrary, BSD-3-Clause, Copyright (c) 2024 Dogu Baran Aydogan).
@@ -413,10 +414,9 @@ __device__ __forceinline__ float interp4(const DevParams &P, const DevBufs &B, f
 }
 
 // -------------------------------------------------------------------------
-// Half-wave matrix-vector products with the reference's per-row summation:
-// lane partials over j = l, l+32, ... (fma), then the 16/8/4/2/1 pair tree.
-// Rows are processed 32 at a time and reduced by a butterfly reduce-scatter
-// (31 shuffles per 32 rows); lane r ends with row r's sum.
+// Half-wave matrix-vector products with the reference's per-row summation
+// (ndotp_d, boot.cu:195-235): per row, 32 lane partials over j = l, l+32, ...
+// (fma, in order), then the 16/8/4/2/1 pair tree.
 // XF selects the elementwise transform of the vector: 0 none, 1 OPDT, 2 CSA.
 // -------------------------------------------------------------------------
 template <int XF>
@@ -431,38 +431,47 @@ __device__ __forceinline__ float gemv_xf(float x) {
     return x;
 }
 
-template <int XF>
-__device__ void gemv_rows(int N, int M, const float *v, const float *__restrict__ mat, float *out) {
+// One row per lane: lane r of the slot sums row i0 + r itself, keeping the
+// 32 partials in registers and adding them with the pair tree, so a row
+// costs M fma + 31 adds (a shuffle reduction across lanes costs ~4 VALU per
+// row-lane pair more).  Rows are read as 16-byte vectors when M % 4 == 0; x
+// comes from LDS as broadcast reads and must be 16-byte aligned, already
+// transformed (gemv_xf).
+__device__ void gemv_lanerow(int N, int M, const float *x, const float *__restrict__ mat, float *out) {
     const int l = lane32();
+    const bool vec = (M & 3) == 0;
     for (int i0 = 0; i0 < N; i0 += 32) {
-        float acc[32];
+        const int i = i0 + l;
+        if (i < N) {
+            const float *row = mat + (size_t)i * M;
+            float p[32];
 #pragma unroll
-        for (int r = 0; r < 32; ++r) acc[r] = 0.0f;
-        // Rows past N read row N-1 (valid memory, result discarded) so the 32
-        // loads of one column step issue back to back without branches.
-        const float *rowp[32];
+            for (int q = 0; q < 32; ++q) p[q] = 0.0f;
+            for (int j0 = 0; j0 < M; j0 += 32) {
+                if (vec) {  // M % 4 == 0: a vector is all in or all out
 #pragma unroll
-        for (int r = 0; r < 32; ++r) rowp[r] = mat + (size_t)min(i0 + r, N - 1) * M;
-        for (int j = l; j < M; j += 32) {
-            float m[32];
+                    for (int q = 0; q < 32; q += 4) {
+                        if (j0 + q < M) {
+                            const float4 m = *reinterpret_cast<const float4 *>(row + j0 + q);
+                            const float4 v = *reinterpret_cast<const float4 *>(x + j0 + q);
+                            p[q] = __builtin_fmaf(v.x, m.x, p[q]);
+                            p[q + 1] = __builtin_fmaf(v.y, m.y, p[q + 1]);
+                            p[q + 2] = __builtin_fmaf(v.z, m.z, p[q + 2]);
+                            p[q + 3] = __builtin_fmaf(v.w, m.w, p[q + 3]);
+                        }
+                    }
+                } else {
 #pragma unroll
-            for (int r = 0; r < 32; ++r) m[r] = rowp[r][j];
-            const float x = gemv_xf<XF>(v[j]);
-#pragma unroll
-            for (int r = 0; r < 32; ++r) acc[r] = __builtin_fmaf(x, m[r], acc[r]);
-        }
-        // reduce-scatter: after the stage with distance d each lane keeps d rows
-#pragma unroll
-        for (int d = 16; d >= 1; d >>= 1) {
-            const bool up = (l & d) != 0;
-#pragma unroll
-            for (int r = 0; r < d; ++r) {
-                const float send = up ? acc[r] : acc[r + d];
-                const float keep = up ? acc[r + d] : acc[r];
-                acc[r] = keep + shfl_xor32(send, d);
+                    for (int q = 0; q < 32; ++q)
+                        if (j0 + q < M) p[q] = __builtin_fmaf(x[j0 + q], row[j0 + q], p[q]);
+                }
             }
+#pragma unroll
+            for (int d = 16; d >= 1; d >>= 1)
+#pragma unroll
+                for (int q = 0; q < d; ++q) p[q] = p[q] + p[q + d];
+            out[i] = p[0];
         }
-        if (i0 + l < N) out[i0 + l] = acc[0];
     }
     WAVE_SYNC();
 }
@@ -530,11 +539,11 @@ __device__ int get_direction_prob(const DevParams &P, const DevBufs &B, const fl
 // Bootstrap getter (boot.cu:19-113, 115-193, 330-608)
 // -------------------------------------------------------------------------
 struct BootLds {
-    float *vox, *msk;  // region A: vox [t32], msk [hr32]
-    float *r;          // region B: R.d / model coefficients; later peak candidates
-    float *h;          // region C: H.d / ODF (pmf)
-    uint32_t *marks;   // peak marks, 2 bits per vertex
-    uint16_t *cand;    // region B reused as peak candidates
+    float *h;          // region C: H.d, then the ODF
+    float *vox, *msk;  // region A during the refit: vox [t32], msk [hr32]
+    float *r;          // region A: R.d, then the model coefficients
+    uint32_t *marks;   // region A once the ODF is computed: peak marks, 2 bits per vertex
+    uint16_t *cand;    // region A after the marks: peak candidates
 };
 
 // closest_peak_d (boot.cu:115-193): lane-strided best |dot| with strict
@@ -576,8 +585,8 @@ __device__ int get_direction_boot(const DevParams &P, const DevBufs &B, const fl
         if (rv == 0) {
             for (int m = l; m < hr; m += 32) S.msk[m] = S.vox[B.dwi_idx[m]];
             WAVE_SYNC();
-            gemv_rows<0>(hr, hr, S.msk, B.R, S.r);
-            gemv_rows<0>(hr, hr, S.msk, B.H, S.h);
+            gemv_lanerow(hr, hr, S.msk, B.R, S.r);
+            gemv_lanerow(hr, hr, S.msk, B.H, S.h);
             for (int j = l; j < hr; j += 32) {
                 const int src = (int)(rng.next() % (uint32_t)hr);
                 S.h[j] = S.h[j] + S.r[src];
@@ -600,16 +609,18 @@ __device__ int get_direction_boot(const DevParams &P, const DevBufs &B, const fl
             WAVE_SYNC();
             for (int m = l; m < hr; m += 32) S.msk[m] = S.vox[B.dwi_idx[m]];
             WAVE_SYNC();
+            // the model's transform of the signal, staged in h (free here)
+            for (int j = l; j < hr; j += 32) S.h[j] = gemv_xf<(MODEL == CUSLINES_OPDT) ? 1 : 2>(S.msk[j]);
+            WAVE_SYNC();
+            gemv_lanerow(dn, hr, S.h, B.dq, S.r);
             if (MODEL == CUSLINES_OPDT) {
-                gemv_rows<1>(dn, hr, S.msk, B.dq, S.r);
-                gemv_rows<0>(dn, hr, S.msk, B.db, S.h);
+                gemv_lanerow(dn, hr, S.msk, B.db, S.h);
                 for (int j = l; j < dn; j += 32) S.r[j] = S.r[j] - S.h[j];
             } else {
-                gemv_rows<2>(dn, hr, S.msk, B.dq, S.r);
                 if (l == 0) S.r[0] = CSA_C0;
             }
             WAVE_SYNC();
-            gemv_rows<0>(nv, dn, S.r, B.smat, S.h);
+            gemv_lanerow(nv, dn, S.r, B.smat, S.h);
         } else {
             for (int j = l; j < nv; j += 32) S.h[j] = 0.0f;
             WAVE_SYNC();
@@ -1607,9 +1618,14 @@ __global__ void __launch_bounds__(256) phase2_ptt_queue_kernel(DevParams P, DevB
 // candidates nv32/2 floats (uint16 indices).
 __host__ __device__ inline int marks_floats(int nv32) { return ((nv32 >> 4) + 3) & ~3; }
 __host__ __device__ inline int cand_floats(int nv32) { return ((nv32 >> 1) + 3) & ~3; }
-__host__ __device__ inline int boot_region_b(int nv32, int hr32) {
-    const int c = cand_floats(nv32);
-    return (c > hr32) ? c : hr32;
+
+// Bootstrap slot: region C holds H.d and then the ODF; region A holds vox,
+// msk and R.d / the model coefficients while the refit runs, and the peak
+// marks and candidates once the ODF is computed (A is dead by then).
+__host__ __device__ inline int boot_region_a(int nv32, int t32, int hr32) {
+    const int refit = t32 + 2 * hr32;
+    const int peaks = marks_floats(nv32) + cand_floats(nv32);
+    return (refit > peaks) ? refit : peaks;
 }
 
 __host__ __device__ inline int slot_floats(int model, bool phase1, int nv32, int t32, int hr32) {
@@ -1617,7 +1633,7 @@ __host__ __device__ inline int slot_floats(int model, bool phase1, int nv32, int
     if (model == CUSLINES_PROB && !phase1) return nv32;
     if (model == CUSLINES_PROB || model == CUSLINES_PTT) return nv32 + marks_floats(nv32) + cand_floats(nv32);
     const int c = (nv32 > hr32) ? nv32 : hr32;
-    return t32 + hr32 + marks_floats(nv32) + boot_region_b(nv32, hr32) + c;
+    return c + boot_region_a(nv32, t32, hr32);
 }
 
 template <int MODEL>
@@ -1632,14 +1648,14 @@ __device__ void bind_slot(const DevParams &P, bool phase1, float *base, SlotLds 
         S.prob.marks = reinterpret_cast<uint32_t *>(base + P.nv32);
         S.prob.cand = reinterpret_cast<uint16_t *>(base + P.nv32 + marks_floats(P.nv32));
     } else {
-        S.boot.vox = base;
-        S.boot.msk = base + P.t32;
-        float *m = base + P.t32 + P.hr32;
-        S.boot.marks = reinterpret_cast<uint32_t *>(m);
-        float *b = m + marks_floats(P.nv32);
-        S.boot.r = b;
-        S.boot.cand = reinterpret_cast<uint16_t *>(b);
-        S.boot.h = b + boot_region_b(P.nv32, P.hr32);
+        const int c = (P.nv32 > P.hr32) ? P.nv32 : P.hr32;
+        float *a = base + c;
+        S.boot.h = base;
+        S.boot.vox = a;
+        S.boot.msk = a + P.t32;
+        S.boot.r = a + P.t32 + P.hr32;
+        S.boot.marks = reinterpret_cast<uint32_t *>(a);
+        S.boot.cand = reinterpret_cast<uint16_t *>(a + marks_floats(P.nv32));
     }
 }
 
@@ -1781,18 +1797,21 @@ __global__ void __launch_bounds__(256) phase2_kernel(DevParams P, DevBufs B, con
 }
 
 // Persistent work queue: a fixed grid of slots; each 32-lane slot takes the
-// next seed from a global counter as soon as its previous seed is done, so
-// seeds of very uneven length keep every slot busy.  `work` must hold 0.
+// next seed from a counter as soon as its previous seed is done, so seeds of
+// very uneven length keep every slot busy.  `order` (or the identity) lists
+// the seeds in processing order; `work` holds QUEUES zeroed counters.  The
+// order changes no result: random streams and output slots follow the seed
+// index.
 // Plain PTT is held to 128 VGPRs (4 waves per SIMD): the few spills cost less
-// than the latency hiding gained (+8% at 200k seeds).  Bootstrap is held to
-// 168 VGPRs (3 waves per SIMD), which its LDS per slot also allows.
+// than the latency hiding gained (+8% at 200k seeds).  Bootstrap likewise
+// (4 waves per SIMD, which its 5 KB of LDS per slot allows: +13% CSA).
 template <int MODEL, int V = 0>
-__global__ void __launch_bounds__(256, (MODEL == CUSLINES_PTT && V != 2) ? 4 : (MODEL <= CUSLINES_CSA ? 3 : 1))
+__global__ void __launch_bounds__(256, (MODEL == CUSLINES_PTT && V != 2) ? 4 : (MODEL <= CUSLINES_CSA ? 4 : 1))
     phase2_queue_kernel(DevParams P, DevBufs B, const float *__restrict__ seeds,
                                                            int nseeds, int rng_chunk, const int *__restrict__ offsets,
                                                            const int *__restrict__ dirs, int *__restrict__ sline_seed,
                                                            int *__restrict__ sline_len, float *__restrict__ sline,
-                                                           int *__restrict__ work) {
+                                                           const int *__restrict__ order, int *__restrict__ work) {
     extern __shared__ float smem[];
     const float *sv = B.verts;
     int svf = 0;
@@ -1806,16 +1825,64 @@ __global__ void __launch_bounds__(256, (MODEL == CUSLINES_PTT && V != 2) ? 4 : (
     const int l = lane32();
     SlotLds S;
     bind_slot<MODEL>(P, false, smem + svf + slot * slot_floats(MODEL, false, P.nv32, P.t32, P.hr32), S);
-    for (;;) {
-        int s = 0;
-        if (l == 0) s = atomicAdd(work, 1);
-        s = shfl32(s, 0);
-        if (s >= nseeds) break;
-        track_seed<MODEL, V>(P, B, sv, S, s, seeds, rng_chunk, offsets, dirs, sline_seed, sline_len, sline);
+    // Workgroups are dispatched to the 8 XCDs round-robin.  Each XCD first
+    // drains its own eighth of the (spatially sorted) seed order, so an XCD's
+    // L2 serves one region of the volume, then helps with the others.
+    const int xcd = blockIdx.x & (QUEUES - 1);
+    for (int k = 0; k < QUEUES; ++k) {
+        const int qx = (xcd + k) & (QUEUES - 1);
+        const int lo = (int)((long long)nseeds * qx / QUEUES);
+        const int hi = (int)((long long)nseeds * (qx + 1) / QUEUES);
+        for (;;) {
+            int q = 0;
+            if (l == 0) q = atomicAdd(work + qx, 1);
+            q = shfl32(q, 0);
+            if (lo + q >= hi) break;
+            const int s = order ? order[lo + q] : lo + q;
+            track_seed<MODEL, V>(P, B, sv, S, s, seeds, rng_chunk, offsets, dirs, sline_seed, sline_len, sline);
+        }
     }
 }
 
-__global__ void zero_int(int *p) { *p = 0; }
+__global__ void zero_ints(int *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// Spatial processing order of the persistent queue: seeds bucketed by the
+// Morton code of their 2^brick_log2-voxel brick (counting sort).
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {  // bits 0..9 to every third bit
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ void seed_keys(DevParams P, const float *__restrict__ seeds, int n, int brick_log2, int *__restrict__ key,
+                          int *__restrict__ count) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    uint32_t c[3];
+    const int dims[3] = {P.dimx, P.dimy, P.dimz};
+    for (int a = 0; a < 3; ++a) {
+        const float f = floorf(seeds[3 * s + a]);
+        const int i = (f != f || f < 0.0f) ? 0 : (f > (float)(dims[a] - 1) ? dims[a] - 1 : (int)f);
+        c[a] = (uint32_t)i >> brick_log2;
+    }
+    const int k = (int)(spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2));
+    key[s] = k;
+    atomicAdd(count + k, 1);
+}
+
+__global__ void seed_scatter(const int *__restrict__ key, int n, const int *__restrict__ start, int *__restrict__ cursor,
+                             int *__restrict__ order) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int k = key[s];
+    order[start[k] + atomicAdd(cursor + k, 1)] = s;
+}
 
 // -------------------------------------------------------------------------
 // Training of the proposal network (no counterpart in the reference).  Plain
@@ -2099,7 +2166,7 @@ struct cuslines_tracker {
     float *d_H = nullptr, *d_R = nullptr, *d_db = nullptr, *d_dq = nullptr, *d_smat = nullptr;
     int *d_b0 = nullptr, *d_dwi = nullptr;
     uint16_t *d_lut = nullptr;
-    int *d_work = nullptr;      // work-queue counter of the persistent phase-2 kernel
+    int *d_work = nullptr;      // work-queue counters of the persistent phase-2 kernels (QUEUES)
     float *d_blocks = nullptr;  // PTT corner blocks
     int use_blocks = 1;
     int schedule = CUSLINES_SCHED_QUEUE;
@@ -2186,7 +2253,7 @@ static int launch_ptt_l(cuslines_tracker *t, int threads, const float *seeds, in
     const long long need = ((long long)n * L + threads - 1) / threads;
     if (grid > need) grid = need;
     if (grid < 1) grid = 1;
-    hipLaunchKernelGGL(zero_int, dim3(1), dim3(1), 0, t->stream, t->d_work);
+    hipLaunchKernelGGL(zero_ints, dim3(1), dim3(QUEUES), 0, t->stream, t->d_work, QUEUES);
     hipLaunchKernelGGL(phase2_ptt_queue_kernel<L>, dim3((unsigned)grid), dim3(threads), lds, t->stream, d, bufs_of(t),
                        seeds, n, chunk, offsets, dirs, sseed, slen, sl, t->d_work);
     HIPC(hipGetLastError());
@@ -2201,6 +2268,43 @@ static int launch_ptt_groups(cuslines_tracker *t, const float *seeds, int n, int
         case 16: return launch_ptt_l<16>(t, threads, seeds, n, chunk, offsets, dirs, sseed, slen, sl);
         default: return launch_ptt_l<32>(t, threads, seeds, n, chunk, offsets, dirs, sseed, slen, sl);
     }
+}
+
+template <typename TO>
+static int run_scan(cuslines_tracker *t, const int *in, int n, TO *out);
+
+// Seeds in Morton order of their 4-voxel brick (coarser bricks when a
+// dimension exceeds 512 voxels), by a counting sort on the tracker's stream.
+// *order is stream-ordered device memory the caller frees with hipFreeAsync.
+static int spatial_order(cuslines_tracker *t, const float *seeds, int n, int **order) {
+    const DevParams &d = t->dp;
+    int mx = d.dimx > d.dimy ? d.dimx : d.dimy;
+    mx = mx > d.dimz ? mx : d.dimz;
+    int brick_log2 = 2;
+    while (((mx - 1) >> brick_log2) >= 128) ++brick_log2;  // at most 2^21 buckets
+    int bits = 0;
+    while ((1 << bits) <= ((mx - 1) >> brick_log2)) ++bits;
+    const int nb = 1 << (3 * bits);
+    int *key = nullptr, *count = nullptr, *start = nullptr;
+    HIPC(hipMallocAsync((void **)order, (size_t)n * sizeof(int), t->stream));
+    HIPC(hipMallocAsync((void **)&key, (size_t)n * sizeof(int), t->stream));
+    HIPC(hipMallocAsync((void **)&count, (size_t)nb * sizeof(int), t->stream));
+    HIPC(hipMallocAsync((void **)&start, ((size_t)nb + 1) * sizeof(int), t->stream));
+    const dim3 gb((unsigned)((nb + 255) / 256)), gs((unsigned)((n + 255) / 256));
+    hipLaunchKernelGGL(zero_ints, gb, dim3(256), 0, t->stream, count, nb);
+    hipLaunchKernelGGL(seed_keys, gs, dim3(256), 0, t->stream, d, seeds, n, brick_log2, key, count);
+    int rc = run_scan<int>(t, count, nb, start);
+    if (rc == CUSLINES_OK) {
+        hipLaunchKernelGGL(zero_ints, gb, dim3(256), 0, t->stream, count, nb);
+        hipLaunchKernelGGL(seed_scatter, gs, dim3(256), 0, t->stream, key, n, start, count, *order);
+    }
+    hipError_t e = hipGetLastError();
+    (void)hipFreeAsync(key, t->stream);
+    (void)hipFreeAsync(count, t->stream);
+    (void)hipFreeAsync(start, t->stream);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(CUSLINES_EHIP, std::string("seed order: ") + hipGetErrorString(e));
+    return CUSLINES_OK;
 }
 
 template <int MODEL, int V>
@@ -2232,10 +2336,14 @@ static int launch_phase2_v(cuslines_tracker *t, const float *seeds, int n, int c
     const long long need = (n + nslots - 1) / nslots;
     if (grid > need) grid = need;
     if (grid < 1) grid = 1;
-    hipLaunchKernelGGL(zero_int, dim3(1), dim3(1), 0, t->stream, t->d_work);
+    int *order = nullptr;
+    if (int rc = spatial_order(t, seeds, n, &order)) return rc;
+    hipLaunchKernelGGL(zero_ints, dim3(1), dim3(QUEUES), 0, t->stream, t->d_work, QUEUES);
     hipLaunchKernelGGL((phase2_queue_kernel<MODEL, V>), dim3((unsigned)grid), dim3(threads), lds, t->stream, d,
-                       bufs_of(t), seeds, n, chunk, offsets, dirs, sseed, slen, sl, t->d_work);
-    HIPC(hipGetLastError());
+                       bufs_of(t), seeds, n, chunk, offsets, dirs, sseed, slen, sl, order, t->d_work);
+    hipError_t e = hipGetLastError();
+    (void)hipFreeAsync(order, t->stream);
+    if (e != hipSuccess) return fail(CUSLINES_EHIP, std::string("phase 2: ") + hipGetErrorString(e));
     return CUSLINES_OK;
 }
 
@@ -2329,7 +2437,7 @@ int cuslines_tracker_create(const cuslines_params *p, int device, cuslines_track
         delete t;
         return fail(CUSLINES_EHIP, std::string("hipStreamCreateWithFlags: ") + hipGetErrorString(e));
     }
-    e = hipMalloc(&t->d_work, sizeof(int));
+    e = hipMalloc(&t->d_work, QUEUES * sizeof(int));
     if (e == hipSuccess) e = hipDeviceGetAttribute(&t->num_cu, hipDeviceAttributeMultiprocessorCount, device);
     if (e != hipSuccess) {
         dfree(t->d_work);
