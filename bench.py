@@ -210,8 +210,8 @@ def main():
                     help="only run one device-to-device copy of BYTES (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build runs no multi-GPU configuration "
-              "(seeds shard contiguously per GPU with no exchange; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: multi-GPU runs are not supported (seeds would shard "
+              "contiguously per GPU with no exchange)", file=sys.stderr)
         sys.exit(2)
 
     from gpustreamlines_amd import workloads as W

@@ -3,8 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY.  This file is the parity checker for the gfx950
  * library (gpustreamlines_amd/csrc) and the timed CPU baseline of bench.py.
- * Nothing in the product loads it.  Only tests/, __graft_entry__.smoke() and
- * bench.py's cpu_baseline leg may use it.
+ * Nothing in the product loads it.  Only the tests, the repository's smoke
+ * check and bench.py's CPU-baseline leg use it.
  *
  * It restates the semantics of dipy/GPUStreamlines' CUDA path
  * (the .cu files of cuslines/cuda_c) per seed, with the 32-lane slice of the reference

@@ -1,7 +1,7 @@
 """ctypes loader for the CPU oracle (test infrastructure only).
 
-Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
-this module.  It exposes the same phase protocol as the gfx950 library on
+Only the tests, the repository's smoke check and bench.py's CPU-baseline leg
+import this module.  It exposes the same phase protocol as the gfx950 library on
 host arrays, so a parity test runs one input through both and compares.
 """
 
