@@ -103,3 +103,51 @@ def test_chunk_relative_streams(small_workloads):
     g2, _ = _run_both(w, seeds[64:], rng_chunk=0)
     n0 = g["offsets"][64]
     assert np.array_equal(g["sline_len"][n0:], g2["sline_len"])
+
+
+@pytest.mark.parametrize("kind", ["prob", "ptt", "csa"])
+def test_edge_seeds_bit_exact(small_workloads, kind):
+    """Seeds outside the volume, on its low-edge half voxel, in zero-ODF /
+    low-FA voxels and duplicated seeds: same counts, lengths and points as
+    the oracle (outside seeds give no direction)."""
+    from gpustreamlines_amd import workloads as W
+
+    w = small_workloads[kind]
+    X, Y, Z = (w.params[k] for k in ("dimx", "dimy", "dimz"))
+    inside = W.seeds_for(w, 6, placement="mixed", seed=3)
+    edge = np.array([[-0.6, 3, 3], [-0.25, 10, 10], [X - 0.51, Y / 2, Z / 2], [X + 4.0, 1, 1],
+                     [1, 1, 1], [0, 0, 0]], np.float32)
+    seeds = np.concatenate([inside, edge, inside[:2]]).astype(np.float32)
+    g, o = _run_both(w, seeds)
+    assert g["counts"][len(inside) + 3] == 0  # outside the volume
+    _assert_same(g, o)
+
+
+def test_empty_seed_batch(small_workloads):
+    from gpustreamlines_amd._device import DeviceTracker
+
+    w = small_workloads["prob"]
+    dt = DeviceTracker(w.params, device=0)
+    try:
+        dt.upload(w.dataf, w.stop_map, w.vertices, w.edges)
+        g = dt.track(np.zeros((0, 3), np.float32))
+    finally:
+        dt.close()
+    assert len(g["counts"]) == 0 and len(g["sline_len"]) == 0 and len(g["points"]) == 0
+
+
+@pytest.mark.parametrize("kind", ["prob", "ptt"])
+def test_max_length_truncation_bit_exact(small_workloads, kind):
+    """A short max_sline_len truncates streamlines at the same step on both
+    sides (generate_streamlines_cuda.cu:412-418 step-count formula)."""
+    import dataclasses
+
+    from gpustreamlines_amd import workloads as W
+
+    w0 = small_workloads[kind]
+    w = dataclasses.replace(w0, params=dict(w0.params, max_sline_len=12))
+    seeds = W.seeds_for(w, 24, placement="mask", seed=4)
+    g, o = _run_both(w, seeds)
+    assert g["sline_len"].max() <= 2 * 12 - 1
+    assert (g["sline_len"] >= 12).any()
+    _assert_same(g, o)
