@@ -2,7 +2,8 @@
 
 ``generate_sft`` and ``generate_trx`` keep the reference's signatures and
 import DIPY / nibabel / trx-python lazily; ``generate_streamlines`` returns
-the packed streamlines without any of them.
+the packed streamlines without any of them.  All three consume the packed
+points and lengths each seed chunk's device compaction returns.
 """
 
 from __future__ import annotations
@@ -11,7 +12,7 @@ import logging
 
 import numpy as np
 
-from .array_sequence import StreamlineArray, make_array_sequence
+from .array_sequence import StreamlineArray
 
 logger = logging.getLogger("GPUStreamlines")
 
@@ -51,83 +52,92 @@ class GenericTracker:
         nchunks = (seeds.shape[0] + global_chunk_sz - 1) // global_chunk_sz
         return global_chunk_sz, nchunks
 
-    def generate_streamlines(self, seeds) -> StreamlineArray:
-        """All kept streamlines of ``seeds`` (voxel coordinates), packed."""
+    def _packed_chunks(self, seeds):
+        """Per seed chunk: (points [n, 3] float32 voxel coordinates, lengths
+        int64) of the streamlines kept by the min/max length filter, packed as
+        the device compaction returns them."""
         seeds = np.asarray(seeds)
         gsz, nchunks = self._divide_chunks(seeds)
-        data, lens = [], []
+        sp = self.seed_propagator
         for idx in range(nchunks):
-            sp = self.seed_propagator
             sp.propagate(seeds[idx * gsz: (idx + 1) * gsz])
             for ii in range(sp.ngpus):
-                L = sp.sline_lens[ii]
-                keep = (L >= sp.minlen) & (L <= sp.maxlen)
+                pts, lens, offs = sp.slines[ii], sp.sline_lens[ii], sp.sline_offs[ii]
+                if lens is None or len(lens) == 0:
+                    continue
+                keep = (lens >= sp.minlen) & (lens <= sp.maxlen)
                 if keep.all():
-                    data.append(sp.slines[ii])
-                    lens.append(L)
-                else:
-                    for jj in np.nonzero(keep)[0]:
-                        o = sp.sline_offs[ii][jj]
-                        data.append(sp.slines[ii][o: o + L[jj]])
-                    lens.append(L[keep])
-        if not data:
+                    yield pts, lens.astype(np.int64)
+                elif keep.any():
+                    idx_pts = np.concatenate([np.arange(offs[j], offs[j] + lens[j]) for j in np.nonzero(keep)[0]])
+                    yield pts[idx_pts], lens[keep].astype(np.int64)
+
+    def generate_streamlines(self, seeds) -> StreamlineArray:
+        """All kept streamlines of ``seeds`` (voxel coordinates), packed."""
+        parts = list(self._packed_chunks(seeds))
+        if not parts:
             return StreamlineArray()
-        return StreamlineArray.from_packed(np.concatenate(data), np.concatenate(lens))
+        return StreamlineArray.from_packed(np.concatenate([p for p, _ in parts]),
+                                           np.concatenate([n for _, n in parts]))
 
     def generate_sft(self, seeds, ref_img):
-        """StatefulTractogram in voxel space (generic_tracker.py:75-93)."""
+        """StatefulTractogram in voxel space (generic_tracker.py:75-93), built
+        from the packed points without a per-streamline copy."""
         from dipy.io.stateful_tractogram import Space, StatefulTractogram
+        from nibabel.streamlines.array_sequence import ArraySequence
 
-        global_chunk_sz, nchunks = self._divide_chunks(seeds)
-        buffer_size = 0
-        generators = []
-        for idx in range(nchunks):
-            self.seed_propagator.propagate(seeds[idx * global_chunk_sz: (idx + 1) * global_chunk_sz])
-            buffer_size += self.seed_propagator.get_buffer_size()
-            generators.append(list(self.seed_propagator.as_generator()))
-        seq = make_array_sequence((s for g in generators for s in g), buffer_size)
+        packed = self.generate_streamlines(seeds)
+        seq = ArraySequence()
+        seq._data = packed._data
+        seq._offsets = packed._offsets
+        seq._lengths = packed._lengths
         return StatefulTractogram(seq, ref_img, Space.VOX)
 
     def generate_trx(self, seeds, ref_img):
-        """Memory-mapped TRX file, grown by doubling (generic_tracker.py:95-172)."""
-        from nibabel.streamlines.array_sequence import ArraySequence
-        from nibabel.streamlines.tractogram import Tractogram
+        """TRX tractogram in world (RAS mm) coordinates (generic_tracker.py:95-172).
+
+        Streams each chunk's packed points into a memory-mapped TrxFile: voxel
+        to world with the image affine, optional linearisation
+        (set_compression_parameters), cast to the position dtype, and the
+        file's capacity grown to 1.5x what the chunk needs when it runs out
+        (first guess: 2 streamlines of 100 points per seed)."""
         from trx.trx_file_memmap import TrxFile
 
-        global_chunk_sz, nchunks = self._divide_chunks(seeds)
-        sl_len_guess = 100
-        sl_per_seed_guess = 2
-        n_sls_guess = sl_per_seed_guess * seeds.shape[0]
-        trx_reference = TrxFile(reference=ref_img)
-        trx_reference.streamlines._data = trx_reference.streamlines._data.astype(self._pos_dtype())
-        trx_reference.streamlines._offsets = trx_reference.streamlines._offsets.astype(np.uint64)
-        trx_file = TrxFile(nb_streamlines=n_sls_guess, nb_vertices=n_sls_guess * sl_len_guess,
-                           init_as=trx_reference)
-        offsets_idx = 0
-        sls_data_idx = 0
-        for idx in range(int(nchunks)):
-            self.seed_propagator.propagate(seeds[idx * global_chunk_sz: (idx + 1) * global_chunk_sz])
-            tractogram = Tractogram(self.seed_propagator.as_array_sequence(), affine_to_rasmm=ref_img.affine)
-            if len(tractogram) == 0:
-                continue
-            tractogram.to_world()
-            sls = tractogram.streamlines
+        dtype = self._pos_dtype()
+        affine = np.asarray(ref_img.affine, dtype=np.float64)
+        template = TrxFile(reference=ref_img)
+        template.streamlines._data = template.streamlines._data.astype(dtype)
+        template.streamlines._offsets = template.streamlines._offsets.astype(np.uint64)
+        cap_sl = 2 * int(np.asarray(seeds).shape[0])
+        cap_pts = 100 * cap_sl
+        trx = TrxFile(nb_streamlines=cap_sl, nb_vertices=cap_pts, init_as=template)
+        n_sl = n_pts = 0
+        for pts, lens in self._packed_chunks(seeds):
+            world = pts.astype(np.float64) @ affine[:3, :3].T + affine[:3, 3]
             if self._linearize():
-                from dipy.tracking.streamlinespeed import compress_streamlines
+                world, lens = _compress(world, lens, self._tol_error(), self._max_segment_length())
+            data = world.astype(dtype)
+            need_sl, need_pts = n_sl + len(lens), n_pts + len(data)
+            if need_sl > cap_sl or need_pts > cap_pts:
+                cap_sl, cap_pts = max(cap_sl, need_sl * 3 // 2), max(cap_pts, need_pts * 3 // 2)
+                logger.info("TRX capacity -> %d streamlines, %d points", cap_sl, cap_pts)
+                trx.resize(nb_streamlines=cap_sl, nb_vertices=cap_pts)
+            offsets = np.zeros(len(lens), dtype=np.uint64)
+            np.cumsum(lens[:-1], out=offsets[1:])
+            trx.streamlines._data[n_pts:need_pts] = data
+            trx.streamlines._offsets[n_sl:need_sl] = offsets + np.uint64(n_pts)
+            trx.streamlines._lengths[n_sl:need_sl] = lens
+            n_sl, n_pts = need_sl, need_pts
+        trx.resize(nb_streamlines=n_sl, nb_vertices=n_pts)
+        return trx
 
-                sls = ArraySequence(compress_streamlines(sls, tol_error=self._tol_error(),
-                                                         max_segment_length=self._max_segment_length()))
-            sls._data = sls._data.astype(self._pos_dtype())
-            new_offsets_idx = offsets_idx + len(sls._offsets)
-            new_sls_data_idx = sls_data_idx + len(sls._data)
-            if (new_offsets_idx > trx_file.header["NB_STREAMLINES"]
-                    or new_sls_data_idx > trx_file.header["NB_VERTICES"]):
-                logger.info("TRX resizing...")
-                trx_file.resize(nb_streamlines=new_offsets_idx * 2, nb_vertices=new_sls_data_idx * 2)
-            trx_file.streamlines._data[sls_data_idx:new_sls_data_idx] = sls._data
-            trx_file.streamlines._offsets[offsets_idx:new_offsets_idx] = sls_data_idx + sls._offsets
-            trx_file.streamlines._lengths[offsets_idx:new_offsets_idx] = sls._lengths
-            offsets_idx = new_offsets_idx
-            sls_data_idx = new_sls_data_idx
-        trx_file.resize()
-        return trx_file
+
+def _compress(points, lens, tol_error, max_segment_length):
+    """DIPY's streamline linearisation of packed streamlines; returns packed."""
+    from dipy.tracking.streamlinespeed import compress_streamlines
+
+    bounds = np.concatenate([[0], np.cumsum(lens)])
+    comp = compress_streamlines([points[bounds[k]:bounds[k + 1]] for k in range(len(lens))], tol_error=tol_error,
+                                max_segment_length=max_segment_length)
+    new_lens = np.array([len(c) for c in comp], dtype=np.int64)
+    return (np.concatenate(comp) if len(comp) else np.zeros((0, 3))), new_lens

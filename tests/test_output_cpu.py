@@ -1,0 +1,157 @@
+"""Tractogram output on the CPU: chunking, the min/max length filter, the
+packed StreamlineArray, and the TRX / StatefulTractogram writers, driven by a
+stand-in seed propagator.  trx-python, nibabel and DIPY are not installed
+here, so minimal stand-ins of the few calls the writers make are injected;
+what is checked is the writers' own arithmetic (affine, offsets, growth)."""
+
+import sys
+import types
+
+import numpy as np
+import pytest
+
+from gpustreamlines_amd.generic_tracker import GenericTracker
+
+
+class FakePropagator:
+    """Streamline j of seed i: (i % 3) + 1 directions, 5 + 97 * ((i + j) % 7)
+    points along x from the seed."""
+
+    def __init__(self, minlen=0, maxlen=np.inf):
+        self.ngpus = 1
+        self.minlen, self.maxlen = minlen, maxlen
+        self.calls = []
+
+    def propagate(self, seeds):
+        self.calls.append(len(seeds))
+        pts, lens = [], []
+        for i, s in enumerate(np.asarray(seeds, np.float32)):
+            for j in range(int(s[1]) % 3 + 1):
+                n = 5 + 97 * ((i + j) % 7)
+                p = np.repeat(s[None], n, 0)
+                p[:, 0] += np.arange(n, dtype=np.float32) * 0.25
+                pts.append(p)
+                lens.append(n)
+        lens = np.array(lens, np.int32)
+        offs = np.zeros(len(lens) + 1, np.int64)
+        np.cumsum(lens, out=offs[1:])
+        self.slines = [np.concatenate(pts) if pts else np.zeros((0, 3), np.float32)]
+        self.sline_lens = [lens]
+        self.sline_offs = [offs]
+
+
+class Tracker(GenericTracker):
+    def __init__(self, chunk_size, **kw):
+        self.chunk_size = chunk_size
+        self.seed_propagator = FakePropagator(**kw)
+
+
+def _seeds(n):
+    i = np.arange(n, dtype=np.float32)
+    return np.stack([i % 11, i % 5, i % 3], axis=1).astype(np.float32)
+
+
+def _expected(seeds, chunk, minlen=0, maxlen=np.inf):
+    fp = FakePropagator()
+    out = []
+    for k in range(0, len(seeds), chunk):
+        fp.propagate(seeds[k:k + chunk])
+        o, L = fp.sline_offs[0], fp.sline_lens[0]
+        for j, n in enumerate(L):
+            if minlen <= n <= maxlen:
+                out.append(fp.slines[0][o[j]:o[j] + n])
+    return out
+
+
+@pytest.mark.parametrize("minlen,maxlen", [(0, np.inf), (100, 400)])
+def test_generate_streamlines_chunks_and_filter(minlen, maxlen):
+    seeds = _seeds(23)
+    tr = Tracker(chunk_size=5, minlen=minlen, maxlen=maxlen)
+    got = tr.generate_streamlines(seeds)
+    want = _expected(seeds, 5, minlen, maxlen)
+    assert tr.seed_propagator.calls == [5, 5, 5, 5, 3]
+    assert len(got) == len(want)
+    for a, b in zip(got, want):
+        np.testing.assert_array_equal(a, b)
+
+
+class FakeTrxFile:
+    """The calls the TRX writer makes on trx.trx_file_memmap.TrxFile."""
+
+    def __init__(self, reference=None, nb_streamlines=None, nb_vertices=None, init_as=None):
+        self.streamlines = types.SimpleNamespace()
+        if init_as is None:
+            self.streamlines._data = np.zeros((0, 3), np.float32)
+            self.streamlines._offsets = np.zeros(0, np.int64)
+            self.streamlines._lengths = np.zeros(0, np.uint32)
+            self.header = {"NB_STREAMLINES": 0, "NB_VERTICES": 0}
+            self.resizes = []
+        else:
+            t = init_as.streamlines
+            self.streamlines._data = np.zeros((nb_vertices, 3), t._data.dtype)
+            self.streamlines._offsets = np.zeros(nb_streamlines, t._offsets.dtype)
+            self.streamlines._lengths = np.zeros(nb_streamlines, t._lengths.dtype)
+            self.header = {"NB_STREAMLINES": nb_streamlines, "NB_VERTICES": nb_vertices}
+            self.resizes = []
+
+    def resize(self, nb_streamlines=None, nb_vertices=None):
+        s = self.streamlines
+        for name, n in (("_data", nb_vertices), ("_offsets", nb_streamlines), ("_lengths", nb_streamlines)):
+            old = getattr(s, name)
+            new = np.zeros((n,) + old.shape[1:], old.dtype)
+            k = min(n, len(old))
+            new[:k] = old[:k]
+            setattr(s, name, new)
+        self.header = {"NB_STREAMLINES": nb_streamlines, "NB_VERTICES": nb_vertices}
+        self.resizes.append((nb_streamlines, nb_vertices))
+
+
+@pytest.fixture
+def fake_trx(monkeypatch):
+    trx = types.ModuleType("trx")
+    mm = types.ModuleType("trx.trx_file_memmap")
+    mm.TrxFile = FakeTrxFile
+    trx.trx_file_memmap = mm
+    monkeypatch.setitem(sys.modules, "trx", trx)
+    monkeypatch.setitem(sys.modules, "trx.trx_file_memmap", mm)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float16])
+def test_generate_trx_world_coordinates_and_growth(fake_trx, dtype):
+    seeds = _seeds(17)
+    tr = Tracker(chunk_size=4)
+    tr.set_compression_parameters(pos_dtype=dtype)
+    ref = types.SimpleNamespace(affine=np.array([[2, 0, 0, -80], [0, 2, 0, -100], [0, 0, 2, -60], [0, 0, 0, 1.0]]))
+    trx = tr.generate_trx(seeds, ref)
+    want = _expected(seeds, 4)
+    lens = np.array([len(w) for w in want])
+    s = trx.streamlines
+    assert len(s._lengths) == len(want) and len(s._data) == lens.sum()
+    np.testing.assert_array_equal(s._lengths, lens)
+    np.testing.assert_array_equal(s._offsets, np.concatenate([[0], np.cumsum(lens)[:-1]]))
+    world = np.concatenate(want).astype(np.float64) * 2 + np.array([-80, -100, -60])
+    np.testing.assert_array_equal(s._data, world.astype(dtype))
+    assert s._data.dtype == dtype
+    assert len(trx.resizes) >= 2  # grew past the 100-point guess, then trimmed
+
+
+def test_generate_sft_uses_packed_arrays(monkeypatch):
+    seeds = _seeds(9)
+    tr = Tracker(chunk_size=4)
+    dipy = types.ModuleType("dipy")
+    io = types.ModuleType("dipy.io")
+    st = types.ModuleType("dipy.io.stateful_tractogram")
+    st.Space = types.SimpleNamespace(VOX="vox")
+    st.StatefulTractogram = lambda seq, ref, space: types.SimpleNamespace(seq=seq, ref=ref, space=space)
+    nib = types.ModuleType("nibabel")
+    nst = types.ModuleType("nibabel.streamlines")
+    asq = types.ModuleType("nibabel.streamlines.array_sequence")
+    asq.ArraySequence = lambda: types.SimpleNamespace()
+    for name, mod in (("dipy", dipy), ("dipy.io", io), ("dipy.io.stateful_tractogram", st), ("nibabel", nib),
+                      ("nibabel.streamlines", nst), ("nibabel.streamlines.array_sequence", asq)):
+        monkeypatch.setitem(sys.modules, name, mod)
+    sft = tr.generate_sft(seeds, "ref")
+    want = _expected(seeds, 4)
+    assert sft.space == "vox" and sft.ref == "ref"
+    np.testing.assert_array_equal(sft.seq._data, np.concatenate(want))
+    np.testing.assert_array_equal(sft.seq._lengths, [len(w) for w in want])
