@@ -355,10 +355,13 @@ __device__ float tiled_sample(const DevParams &P, const float *__restrict__ odfv
     return acc;
 }
 
-// interp4_d (ptt.cu:24-42): nearest vertex of the tangent from the LUT
-// (point sampling, normalised coordinates, clamp), then the tiled sample.
-__device__ __forceinline__ float interp4(const DevParams &P, const DevBufs &B, float px, float py,
-                                         float pz, float f0, float f1, float f2) {
+// interp4_d (ptt.cu:24-42) in two halves, so a caller can issue the LUT
+// read of one tap before it consumes the previous tap's block.
+// lut_index: linear index of the nearest-vertex LUT cell of the tangent
+// (point sampling, normalised coordinates, clamp).  Index products use
+// 24-bit multiplies (full rate; every operand is below 2^24: LUT coordinates,
+// tile indices, cell coordinates).
+__device__ __forceinline__ uint32_t lut_index(const DevParams &P, float f0, float f1, float f2) {
     const int res = P.lut_res;
     const float rf = (float)res, rmax = (float)(res - 1);
     const float u0 = __builtin_fmaf(f0, 0.5f, 0.5f), u1 = __builtin_fmaf(f1, 0.5f, 0.5f),
@@ -366,17 +369,19 @@ __device__ __forceinline__ float interp4(const DevParams &P, const DevBufs &B, f
     const uint32_t i0 = (uint32_t)fminf(fmaxf(floorf(u0 * rf), 0.0f), rmax);
     const uint32_t i1 = (uint32_t)fminf(fmaxf(floorf(u1 * rf), 0.0f), rmax);
     const uint32_t i2 = (uint32_t)fminf(fmaxf(floorf(u2 * rf), 0.0f), rmax);
-    // Index products use 24-bit multiplies (full rate; every operand is
-    // below 2^24: LUT coordinates, tile indices, cell coordinates).
-    uint32_t li;
     if (P.lut_brick) {
         const uint32_t nb = (uint32_t)res >> 2;
-        li = (__umul24(__umul24(i0 >> 2, nb) + (i1 >> 2), nb) + (i2 >> 2)) * 64u + ((i0 & 3u) << 4) +
-             ((i1 & 3u) << 2) + (i2 & 3u);
-    } else {
-        li = __umul24(__umul24(i0, (uint32_t)res) + i1, (uint32_t)res) + i2;
+        return (__umul24(__umul24(i0 >> 2, nb) + (i1 >> 2), nb) + (i2 >> 2)) * 64u + ((i0 & 3u) << 4) +
+               ((i1 & 3u) << 2) + (i2 & 3u);
     }
-    const int idx = (int)B.lut[li];
+    return __umul24(__umul24(i0, (uint32_t)res) + i1, (uint32_t)res) + i2;
+}
+
+// The linear-filtered sample of vertex idx's tile of the ODF texture at
+// (px, py, pz): one 32-byte corner block when the cell is inside the volume
+// (its 8 texels, the tile's clamp already applied), else the 8-texel gather.
+__device__ __forceinline__ float tap_sample(const DevParams &P, const DevBufs &B, int idx, float px, float py,
+                                            float pz) {
     const int tx = idx & (P.nx - 1);
     const int ty = (idx >> P.log2x) & (P.ny - 1);
     const int tz = idx >> (P.log2x + P.log2y);
@@ -411,6 +416,11 @@ __device__ __forceinline__ float interp4(const DevParams &P, const DevBufs &B, f
         }
     }
     return tiled_sample(P, B.data, xq, yq, zq, tx, ty, tz);
+}
+
+__device__ __forceinline__ float interp4(const DevParams &P, const DevBufs &B, float px, float py,
+                                         float pz, float f0, float f1, float f2) {
+    return tap_sample(P, B, (int)B.lut[lut_index(P, f0, f1, f2)], px, py, pz);
 }
 
 // -------------------------------------------------------------------------
@@ -734,14 +744,25 @@ __device__ __forceinline__ void probing_frame(const float *frame, Philox &rng, f
 }
 
 // calculate_data_support_d (ptt.cu:156-192)
+// The frames of successive probes do not depend on the sampled values, so
+// probe i+1's frame and LUT read are issued before probe i's block is
+// consumed: the LUT latency of one tap overlaps the block read of the last.
 __device__ __forceinline__ float data_support(const DevParams &P, const DevBufs &B, float support,
                                               float px, float py, float pz, float k1, float k2, float *pf) {
     float prop[9], direc[3];
     prepare_propagator(k1, k2, P.probe_step, prop);
+    propagate_frame(prop, pf, direc);
+    px += direc[0]; py += direc[1]; pz += direc[2];
+    int idx = (int)B.lut[lut_index(P, pf[0], pf[1], pf[2])];
     for (int i = 0; i < P.probe_quality; ++i) {
-        propagate_frame(prop, pf, direc);
-        px += direc[0]; py += direc[1]; pz += direc[2];
-        support += interp4(P, B, px, py, pz, pf[0], pf[1], pf[2]);
+        const float qx = px, qy = py, qz = pz;
+        const int qidx = idx;
+        if (i + 1 < P.probe_quality) {
+            propagate_frame(prop, pf, direc);
+            px += direc[0]; py += direc[1]; pz += direc[2];
+            idx = (int)B.lut[lut_index(P, pf[0], pf[1], pf[2])];
+        }
+        support += tap_sample(P, B, qidx, qx, qy, qz);
     }
     return support;
 }
