@@ -1261,6 +1261,9 @@ int oracle_peaks(oracle_problem *h, const float *odf, int32_t *out) {
     std::vector<int> scr(2 * (size_t)P.p.nverts + 64);
     return peak_directions(P, odf, out, scr.data());
 }
+int oracle_closest_peak(oracle_problem *h, const float *dir, int npeaks, const int32_t *peaks, float *out) {
+    return closest_peak(*reinterpret_cast<Problem *>(h), dir, npeaks, peaks, out);
+}
 int oracle_trilinear(oracle_problem *h, const float *pt, float *out) {
     const Problem &P = *reinterpret_cast<Problem *>(h);
     return trilinear(P, P.dataf.data(), P.p.dimt, pt, out);

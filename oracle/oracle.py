@@ -45,6 +45,7 @@ def lib():
             ("oracle_phase1", [vp, fp, C.c_int, C.c_int, ip, ip]),
             ("oracle_phase2", [vp, fp, C.c_int, C.c_int, ip, ip, ip, ip, fp]),
             ("oracle_peaks", [vp, fp, ip]),
+            ("oracle_closest_peak", [vp, fp, C.c_int, ip, fp]),
             ("oracle_trilinear", [vp, fp, fp]),
             ("oracle_check_point", [vp, fp]),
             ("oracle_boot_odf", [vp, fp, C.c_int, C.c_int, fp, fp]),
@@ -156,6 +157,15 @@ class Oracle:
         out = np.zeros(self.params["nverts"], dtype=np.int32)
         n = self.L.oracle_peaks(self.h, _f(odf), _i(out))
         return out[:n]
+
+    def closest_peak(self, direction, peaks):
+        """closest_peak_d (boot.cu:115-193): (found, direction) for vertex
+        indices ``peaks`` of the sphere."""
+        d = np.ascontiguousarray(direction, dtype=np.float32)
+        pk = np.ascontiguousarray(peaks, dtype=np.int32)
+        out = np.zeros(3, dtype=np.float32)
+        ok = self.L.oracle_closest_peak(self.h, _f(d), len(pk), _i(pk), _f(out))
+        return ok, out
 
     def trilinear(self, pt):
         pt = np.ascontiguousarray(pt, dtype=np.float32)

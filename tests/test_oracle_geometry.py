@@ -140,6 +140,57 @@ def test_single_fibre_gives_antipodal_pair():
     assert np.dot(verts[pk[0]], verts[pk[1]]) < -0.99
 
 
+@pytest.mark.parametrize("npeaks", [1, 5, 31, 32, 33, 70])
+def test_closest_peak_brute_force(npeaks):
+    """closest_peak_d (boot.cu:115-193): the peak of largest |dot| with the
+    current direction, returned as is when dot >= cos(max_angle), negated
+    when dot <= -cos(max_angle), else none.  Random peaks (distinct |dot|),
+    against an argmax in float64."""
+    rng = np.random.default_rng(npeaks)
+    verts = rng.normal(size=(200, 3))
+    verts = (verts / np.linalg.norm(verts, axis=1, keepdims=True)).astype(np.float32)
+    o, _ = toy_problem(shape=(2, 2, 2), T=200, nverts=200, verts=verts, edges=np.array([[0, 1]], np.int32))
+    cos_max = np.cos(np.float32(np.radians(60)))
+    hits = flips = misses = 0
+    for trial in range(60):
+        d = rng.normal(size=3)
+        d = (d / np.linalg.norm(d)).astype(np.float32)
+        pk = rng.choice(200, npeaks, replace=False).astype(np.int32)
+        dots = verts[pk].astype(np.float64) @ d.astype(np.float64)
+        best = int(np.argmax(np.abs(dots)))
+        ok, out = o.closest_peak(d, pk)
+        if dots[best] >= cos_max + 1e-6:
+            assert ok == 1 and np.array_equal(out, verts[pk[best]]), trial
+            hits += 1
+        elif dots[best] <= -cos_max - 1e-6:
+            assert ok == 1 and np.array_equal(out, -verts[pk[best]]), trial
+            flips += 1
+        elif abs(abs(dots[best]) - cos_max) > 1e-6:
+            assert ok == 0, trial
+            misses += 1
+    assert hits + flips + misses > 50
+    if npeaks >= 31:
+        assert hits > 0 and flips > 0
+
+
+def test_closest_peak_threshold_and_degenerate_cases():
+    """Exactly 60 degrees is accepted (>=), just beyond is not; an empty peak
+    list or dots of exactly 0 (strict > against an initial 0) give none."""
+    c60 = np.float32(np.sqrt(0.75))
+    verts = np.array([[0.5, c60, 0], [-0.5, 0, c60], [0.49, 0.87, 0], [0, 1, 0], [0, 0, 1]], np.float32)
+    o, _ = toy_problem(shape=(2, 2, 2), T=5, nverts=5, verts=verts, edges=np.array([[0, 1]], np.int32))
+    d = np.array([1, 0, 0], np.float32)
+    ok, out = o.closest_peak(d, [0])
+    assert ok == 1 and np.array_equal(out, verts[0])
+    ok, out = o.closest_peak(d, [1])
+    assert ok == 1 and np.array_equal(out, -verts[1])
+    assert o.closest_peak(d, [2])[0] == 0
+    assert o.closest_peak(d, [3, 4])[0] == 0
+    assert o.closest_peak(d, [])[0] == 0
+    ok, out = o.closest_peak(d, [3, 2, 4, 1])  # |-0.5| > 0.49: the flipped vertex 1 wins
+    assert ok == 1 and np.array_equal(out, -verts[1])
+
+
 def test_ptt_tiled_texture_bleed():
     """A constant ODF per vertex, g(v) = v: interior taps return g of the LUT
     vertex; past the last texel centre along x the clamped/tiled texture
