@@ -179,6 +179,39 @@ def measure_learned(args, w, dt, step, plain_lens, sld, sl, nsl, L):
     }
 
 
+def measure_end_to_end(args, w, seeds, kernel_steps):
+    """The user-facing path: GPUTracker.generate_streamlines over chunks of
+    CHUNK seeds (seed upload, phase 1, scan, count read, phase 2, compaction,
+    point download, length filter and packing), timed after one warm-up
+    chunk.  Data preparation (PTT tiling and LUT) happens in the constructor
+    and is reported apart; no file is written."""
+    from gpustreamlines_amd import (BootDirectionGetter, GPUTracker, ProbDirectionGetter, PttDirectionGetter)
+
+    p = w.params
+    if args.getter == "prob":
+        dg, data = ProbDirectionGetter(), w.dataf
+    elif args.getter == "ptt":
+        dg, data = PttDirectionGetter(), w.extra["odf"]
+    else:
+        dg, data = BootDirectionGetter(args.getter.upper(), p["min_signal"], **w.boot), w.dataf
+    t0 = time.perf_counter()
+    tr = GPUTracker(dg, data, w.stop_map, p["tc_threshold"], w.vertices, w.edges, max_angle=p["max_angle"],
+                    step_size=p["step_size"], relative_peak_thresh=p["relative_peak_thresh"],
+                    min_separation_angle=p["min_separation_angle"], chunk_size=CHUNK)
+    with tr:
+        prep = time.perf_counter() - t0
+        tr.generate_streamlines(seeds[:CHUNK])
+        t1 = time.perf_counter()
+        out = tr.generate_streamlines(seeds)
+        el = time.perf_counter() - t1
+    lens = np.asarray(out._lengths, dtype=np.int64)
+    steps = int((lens - 1).clip(min=0).sum())
+    print(f"[bench] end to end: {el:.2f}s, {steps / el / 1e6:.2f} M steps/s (prep {prep:.1f}s)", file=sys.stderr,
+          flush=True)
+    return {"steps_per_s": steps / el, "s": el, "streamlines": int(len(lens)), "points": int(lens.sum()),
+            "chunks": -(-len(seeds) // CHUNK), "prep_s": prep, "same_steps_as_kernel_pass": steps == kernel_steps}
+
+
 def copy_calibration(nbytes):
     """One hipMemcpy device-to-device of nbytes: a plain copy kernel of known
     size, whose FETCH_SIZE/WRITE_SIZE scale the tracking kernels' counters."""
@@ -206,6 +239,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--proposal", default="both", choices=["plain", "learned", "both"],
                     help="PTT: time the plain getter, the learned proposal, or both (value = plain)")
+    ap.add_argument("--end-to-end", action="store_true",
+                    help="also time GPUTracker.generate_streamlines over 25,000-seed chunks (host copies included)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -281,6 +316,12 @@ def main():
             print(f"[bench] learned proposal failed: {exc!r}", file=sys.stderr, flush=True)
             learned = {"error": repr(exc)}
     steps_per_pass = int((lens - 1).clip(min=0).sum())
+    end_to_end = None
+    if args.end_to_end:
+        for b in (ssd, sld, sl):
+            b.free()
+        ssd = sld = sl = None
+        end_to_end = measure_end_to_end(args, w, seeds, steps_per_pass)
     t_step = np.array([t[3] for t in times]) / 1e3
     t_p2 = np.array([t[2] for t in times]) / 1e3
     ms_per_step = float(t_step.mean() * 1e3)
@@ -326,6 +367,8 @@ def main():
             "alg_bytes_per_launch": abytes * steps_per_pass,
         },
     }
+    if end_to_end is not None:
+        res["end_to_end"] = end_to_end
     if learned is not None:
         res["learned"] = learned
         if args.proposal == "learned" and "steps_per_s" in learned:
@@ -337,7 +380,8 @@ def main():
         res["roofline"]["traffic"] = traffic["traffic_bytes"]
         res["roofline"]["traffic_source"] = traffic["source"]
     for b in (sd, cd, dd, od, ssd, sld, sl):
-        b.free()
+        if b is not None:
+            b.free()
     dt.close()
     if not args.no_cpu:
         ns = args.cpu_sample or CPU_SAMPLE[args.getter]

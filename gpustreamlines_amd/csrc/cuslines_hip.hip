@@ -102,7 +102,6 @@ struct DevParams {
     int step_frac, probe_quality, lut_res, nx, ny, nz, log2x, log2y;
     float probe_step, max_curv, min_support_f, ptt_seed_thr_f;
     float ptt_sub_step;  // (float)(STEP_SIZE / STEP_FRAC)
-    uint32_t cells_per_vertex;  // PTT corner blocks: (dimx+1)(dimy+1)(dimz+1)
     int lut_brick;               // PTT LUT stored as 4x4x4 bricks
     uint32_t rec_thresh;  // record a sub-step when its hash is below this
     int rec_cap;          // capacity of the pair buffer
@@ -116,7 +115,7 @@ struct DevBufs {
     const float *H, *R, *db, *dq, *smat;
     const int *b0;
     const int *dwi_idx;
-    const uint16_t *lut;
+    const uint32_t *lut;  // PTT LUT entry: vertex | (direction rank << 16)
     const float *blocks;  // PTT corner blocks (nullptr: gather 8 texels)
     const float *fprop;    // learned proposal: per disc vertex, propagator of 4 probe steps [24][9]
     const uint16_t *mlp;   // proposal network, bf16 (MLP_NPARAM)
@@ -377,11 +376,15 @@ __device__ __forceinline__ uint32_t lut_index(const DevParams &P, float f0, floa
     return __umul24(__umul24(i0, (uint32_t)res) + i1, (uint32_t)res) + i2;
 }
 
-// The linear-filtered sample of vertex idx's tile of the ODF texture at
-// (px, py, pz): one 32-byte corner block when the cell is inside the volume
-// (its 8 texels, the tile's clamp already applied), else the 8-texel gather.
-__device__ __forceinline__ float tap_sample(const DevParams &P, const DevBufs &B, int idx, float px, float py,
+// The linear-filtered sample of a vertex's tile of the ODF texture at
+// (px, py, pz); `lv` is the vertex's LUT entry (vertex | rank << 16).  One
+// 32-byte corner block when the cell is inside the volume (its 8 texels, the
+// tile's clamp already applied), else the 8-texel gather.  Blocks are stored
+// cell-major, the vertices of a cell in the rank order, so one 128-byte line
+// holds a cell's blocks for four neighbouring directions.
+__device__ __forceinline__ float tap_sample(const DevParams &P, const DevBufs &B, uint32_t lv, float px, float py,
                                             float pz) {
+    const int idx = (int)(lv & 0xffffu);
     const int tx = idx & (P.nx - 1);
     const int ty = (idx >> P.log2x) & (P.ny - 1);
     const int tz = idx >> (P.log2x + P.log2y);
@@ -400,7 +403,7 @@ __device__ __forceinline__ float tap_sample(const DevParams &P, const DevBufs &B
             const uint32_t inner =
                 __umul24(__umul24((uint32_t)(cx + 1), (uint32_t)(P.dimy + 1)) + (uint32_t)(cy + 1),
                          (uint32_t)(P.dimz + 1)) + (uint32_t)(cz + 1);
-            const uint64_t cell = (uint64_t)(uint32_t)idx * P.cells_per_vertex + inner;
+            const uint64_t cell = (uint64_t)inner * (uint32_t)P.nverts + (lv >> 16);
             const float4 *blk = reinterpret_cast<const float4 *>(B.blocks + cell * 8);
             const float4 lo = blk[0], hi = blk[1];
             const float a0 = 1.0f - a, b0 = 1.0f - b, g0 = 1.0f - g;
@@ -420,7 +423,7 @@ __device__ __forceinline__ float tap_sample(const DevParams &P, const DevBufs &B
 
 __device__ __forceinline__ float interp4(const DevParams &P, const DevBufs &B, float px, float py,
                                          float pz, float f0, float f1, float f2) {
-    return tap_sample(P, B, (int)B.lut[lut_index(P, f0, f1, f2)], px, py, pz);
+    return tap_sample(P, B, B.lut[lut_index(P, f0, f1, f2)], px, py, pz);
 }
 
 // -------------------------------------------------------------------------
@@ -753,14 +756,14 @@ __device__ __forceinline__ float data_support(const DevParams &P, const DevBufs 
     prepare_propagator(k1, k2, P.probe_step, prop);
     propagate_frame(prop, pf, direc);
     px += direc[0]; py += direc[1]; pz += direc[2];
-    int idx = (int)B.lut[lut_index(P, pf[0], pf[1], pf[2])];
+    uint32_t idx = B.lut[lut_index(P, pf[0], pf[1], pf[2])];
     for (int i = 0; i < P.probe_quality; ++i) {
         const float qx = px, qy = py, qz = pz;
-        const int qidx = idx;
+        const uint32_t qidx = idx;
         if (i + 1 < P.probe_quality) {
             propagate_frame(prop, pf, direc);
             px += direc[0]; py += direc[1]; pz += direc[2];
-            idx = (int)B.lut[lut_index(P, pf[0], pf[1], pf[2])];
+            idx = B.lut[lut_index(P, pf[0], pf[1], pf[2])];
         }
         support += tap_sample(P, B, qidx, qx, qy, qz);
     }
@@ -2020,19 +2023,22 @@ __global__ void feature_propagators(DevParams P, float *__restrict__ out) {
     }
 }
 
-// PTT corner blocks: for vertex v and texel cell (cx,cy,cz) in [-1, D-1]^3 of
-// its tile, the 8 texels one linear-filtered tap reads (x fastest), with the
-// tiling's bleed into neighbouring tiles and the clamp at the texture edge.
-__global__ void build_corner_blocks(DevParams P, const float *__restrict__ odfv, float *__restrict__ blocks,
+// PTT corner blocks: for texel cell (cx,cy,cz) in [-1, D-1]^3 and direction
+// rank r (vertex rank_vertex[r]), the 8 texels one linear-filtered tap of
+// that vertex's tile reads (x fastest), with the tiling's bleed into
+// neighbouring tiles and the clamp at the texture edge.  Block index =
+// cell * nverts + r.
+__global__ void build_corner_blocks(DevParams P, const float *__restrict__ odfv,
+                                    const uint16_t *__restrict__ rank_vertex, float *__restrict__ blocks,
                                     long long ncells) {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= ncells) return;
     const int DX = P.dimx, DY = P.dimy, DZ = P.dimz;
     long long t = idx;
+    const int v = (int)rank_vertex[t % P.nverts]; t /= P.nverts;
     const int cz = (int)(t % (DZ + 1)) - 1; t /= (DZ + 1);
     const int cy = (int)(t % (DY + 1)) - 1; t /= (DY + 1);
-    const int cx = (int)(t % (DX + 1)) - 1; t /= (DX + 1);
-    const int v = (int)t;
+    const int cx = (int)(t % (DX + 1)) - 1;
     const int tx = v & (P.nx - 1), ty = (v >> P.log2x) & (P.ny - 1), tz = v >> (P.log2x + P.log2y);
     int txa, txb, lxa, lxb, tya, tyb, lya, lyb, tza, tzb, lza, lzb;
     axis_split(tx * DX + cx, DX, P.nx, tx, &txa, &lxa);
@@ -2186,7 +2192,8 @@ struct cuslines_tracker {
     uint32_t *d_edges = nullptr;
     float *d_H = nullptr, *d_R = nullptr, *d_db = nullptr, *d_dq = nullptr, *d_smat = nullptr;
     int *d_b0 = nullptr, *d_dwi = nullptr;
-    uint16_t *d_lut = nullptr;
+    uint32_t *d_lut = nullptr;
+    std::vector<uint16_t> rank_vertex;  // PTT: vertex of each direction rank (corner-block order)
     int *d_work = nullptr;      // work-queue counters of the persistent phase-2 kernels (QUEUES)
     float *d_blocks = nullptr;  // PTT corner blocks
     int use_blocks = 1;
@@ -2500,7 +2507,6 @@ int cuslines_tracker_create(const cuslines_params *p, int device, cuslines_track
     d.min_support_f = smallest_float_at_least((double)p->probe_quality * PMF_THRESHOLD_P);
     d.ptt_seed_thr_f = smallest_float_at_least(PMF_THRESHOLD_P);
     d.ptt_sub_step = (float)(d.step_d / (double)d.step_frac);
-    d.cells_per_vertex = (uint32_t)((uint64_t)(p->dimx + 1) * (p->dimy + 1) * (p->dimz + 1));
     *out = t;
     return CUSLINES_OK;
 }
@@ -2515,6 +2521,31 @@ int cuslines_tracker_destroy(cuslines_tracker *t) {
     dfree(t->d_fprop); dfree(t->d_mlp); dfree(t->d_stats);
     if (t->stream) (void)hipStreamDestroy(t->stream);
     delete t;
+    return CUSLINES_OK;
+}
+
+// PTT corner blocks from the transposed ODF and the LUT's direction ranks
+// (both must be present).  Without room for them taps gather their 8 texels.
+static int build_blocks(cuslines_tracker *t) {
+    const DevParams &d = t->dp;
+    dfree(t->d_blocks);
+    t->d_blocks = nullptr;
+    const long long ncells = (long long)d.nverts * (d.dimx + 1) * (d.dimy + 1) * (d.dimz + 1);
+    uint16_t *rv = nullptr;
+    HIPC(hipMalloc(&rv, t->rank_vertex.size() * sizeof(uint16_t)));
+    hipError_t e = hipMemcpy(rv, t->rank_vertex.data(), t->rank_vertex.size() * sizeof(uint16_t),
+                             hipMemcpyHostToDevice);
+    if (e == hipSuccess && hipMalloc(&t->d_blocks, (size_t)ncells * 8 * sizeof(float)) == hipSuccess) {
+        hipLaunchKernelGGL(build_corner_blocks, dim3((unsigned)((ncells + 255) / 256)), dim3(256), 0, t->stream, d,
+                           t->d_data, rv, t->d_blocks, ncells);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+    } else if (e == hipSuccess) {
+        (void)hipGetLastError();
+        t->d_blocks = nullptr;
+    }
+    dfree(rv);
+    if (e != hipSuccess) return fail(CUSLINES_EHIP, std::string("PTT blocks: ") + hipGetErrorString(e));
     return CUSLINES_OK;
 }
 
@@ -2541,19 +2572,9 @@ int cuslines_set_data(cuslines_tracker *t, const float *dataf) {
         if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
         dfree(tmp);
         if (e != hipSuccess) return fail(CUSLINES_EHIP, std::string("PTT data upload: ") + hipGetErrorString(e));
-        dfree(t->d_blocks);
-        t->d_blocks = nullptr;
-        const long long ncells = (long long)d.nverts * (d.dimx + 1) * (d.dimy + 1) * (d.dimz + 1);
-        if (hipMalloc(&t->d_blocks, (size_t)ncells * 8 * sizeof(float)) == hipSuccess) {
-            hipLaunchKernelGGL(build_corner_blocks, dim3((unsigned)((ncells + 255) / 256)), dim3(256), 0, t->stream, d,
-                               t->d_data, t->d_blocks, ncells);
-            e = hipGetLastError();
-            if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
-            if (e != hipSuccess) return fail(CUSLINES_EHIP, std::string("PTT blocks: ") + hipGetErrorString(e));
-        } else {
-            (void)hipGetLastError();  // no room for the blocks: taps gather their 8 texels
-            t->d_blocks = nullptr;
-        }
+        t->has_data = true;
+        if (t->has_lut) return build_blocks(t);
+        return CUSLINES_OK;
     } else {
         const size_t pitch = (size_t)d.t32 * sizeof(float);
         HIPC(hipMalloc(&t->d_data, nvox * pitch));
@@ -2639,7 +2660,31 @@ int cuslines_set_ptt_lut(cuslines_tracker *t, const int32_t *lut) {
     if (t->p.model_type != CUSLINES_PTT) return fail(CUSLINES_ESTATE, "LUT needs a PTT tracker");
     HIPC(hipSetDevice(t->device));
     const size_t r = (size_t)t->dp.lut_res;
-    std::vector<uint16_t> h(r * r * r);
+    const int nv = t->dp.nverts;
+    std::vector<uint32_t> h(r * r * r);
+    // Direction ranks: vertices numbered in the order they first appear along
+    // a Morton traversal of the LUT grid, so consecutive ranks are
+    // neighbouring directions; the corner blocks of a cell are stored in this
+    // order (tap_sample).
+    std::vector<int> rank(nv, -1);
+    std::vector<uint16_t> rank_vertex;
+    rank_vertex.reserve(nv);
+    {
+        int bits = 0;
+        while ((size_t(1) << bits) < r) ++bits;
+        const uint64_t ncode = uint64_t(1) << (3 * bits);
+        for (uint64_t c = 0; c < ncode; ++c) {
+            uint64_t a[3] = {0, 0, 0};
+            for (int b = 0; b < bits; ++b)
+                for (int k = 0; k < 3; ++k) a[k] |= ((c >> (3 * b + 2 - k)) & 1u) << b;
+            if (a[0] >= r || a[1] >= r || a[2] >= r) continue;
+            const int32_t v = lut[(a[0] * r + a[1]) * r + a[2]];
+            if (v < 0 || v >= nv) return fail(CUSLINES_EINVAL, "LUT vertex index out of range");
+            if (rank[v] < 0) { rank[v] = (int)rank_vertex.size(); rank_vertex.push_back((uint16_t)v); }
+        }
+        for (int v = 0; v < nv; ++v)
+            if (rank[v] < 0) { rank[v] = (int)rank_vertex.size(); rank_vertex.push_back((uint16_t)v); }
+    }
     // Unit directions touch a thin shell of the cube: with res % 4 == 0 the
     // table is stored as 4x4x4 bricks of 128 bytes (one cache line), so the
     // shell spans ~10x fewer lines than with rows along the last axis.
@@ -2649,18 +2694,19 @@ int cuslines_set_ptt_lut(cuslines_tracker *t, const int32_t *lut) {
         for (size_t i1 = 0; i1 < r; ++i1)
             for (size_t i2 = 0; i2 < r; ++i2) {
                 const int32_t v = lut[(i0 * r + i1) * r + i2];
-                if (v < 0 || v >= t->dp.nverts) return fail(CUSLINES_EINVAL, "LUT vertex index out of range");
                 const size_t dst = brick ? (((i0 >> 2) * nb + (i1 >> 2)) * nb + (i2 >> 2)) * 64 +
                                                ((i0 & 3) << 4) + ((i1 & 3) << 2) + (i2 & 3)
                                          : (i0 * r + i1) * r + i2;
-                h[dst] = (uint16_t)v;
+                h[dst] = (uint32_t)v | ((uint32_t)rank[v] << 16);
             }
     t->dp.lut_brick = brick ? 1 : 0;
     dfree(t->d_lut);
     t->d_lut = nullptr;
-    HIPC(hipMalloc(&t->d_lut, h.size() * sizeof(uint16_t)));
-    HIPC(hipMemcpy(t->d_lut, h.data(), h.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    HIPC(hipMalloc(&t->d_lut, h.size() * sizeof(uint32_t)));
+    HIPC(hipMemcpy(t->d_lut, h.data(), h.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    t->rank_vertex.swap(rank_vertex);
     t->has_lut = true;
+    if (t->has_data) return build_blocks(t);
     return CUSLINES_OK;
 }
 

@@ -91,7 +91,7 @@ def build(kind: str, shape=(81, 106, 76), step_size=0.25, seed=0) -> Workload:
                  max_curvature=float(8.0 * np.sin(np.float32(p["max_angle"]) / 2.0)),
                  lut_res=dg.odf_lut_res)
         return Workload(kind, p, data, ph.fa, verts, edges, ptt_lut=dg.sphere_vertices_lut_h,
-                        extra=dict(phantom=ph))
+                        extra=dict(phantom=ph, odf=odf))
     raise ValueError(f"unknown workload kind {kind!r}")
 
 
