@@ -447,16 +447,22 @@ __device__ __forceinline__ float gemv_xf(float x) {
 // One row per lane: lane r of the slot sums row i0 + r itself, keeping the
 // 32 partials in registers and adding them with the pair tree, so a row
 // costs M fma + 31 adds (a shuffle reduction across lanes costs ~4 VALU per
-// row-lane pair more).  Rows are read as 16-byte vectors when M % 4 == 0; x
-// comes from LDS as broadcast reads and must be 16-byte aligned, already
-// transformed (gemv_xf).
+// row-lane pair more).  When M % 4 == 0 the matrix is stored lane-interleaved
+// (mat_offset: the 4-column pieces of 32 consecutive rows side by side), so
+// the 16-byte row loads of one instruction are 512 contiguous bytes; else
+// row-major.  x comes from LDS as broadcast reads and must be 16-byte
+// aligned, already transformed (gemv_xf).
+__host__ __device__ inline size_t mat_offset(int M, int i, int j) {
+    return (((size_t)(i >> 5) * (size_t)(M >> 2) + (size_t)(j >> 2)) << 7) + (size_t)((i & 31) << 2) + (size_t)(j & 3);
+}
+
 __device__ void gemv_lanerow(int N, int M, const float *x, const float *__restrict__ mat, float *out) {
     const int l = lane32();
     const bool vec = (M & 3) == 0;
     for (int i0 = 0; i0 < N; i0 += 32) {
         const int i = i0 + l;
         if (i < N) {
-            const float *row = mat + (size_t)i * M;
+            const float *row = vec ? mat + mat_offset(M, i, 0) : mat + (size_t)i * M;
             float p[32];
 #pragma unroll
             for (int q = 0; q < 32; ++q) p[q] = 0.0f;
@@ -465,7 +471,7 @@ __device__ void gemv_lanerow(int N, int M, const float *x, const float *__restri
 #pragma unroll
                     for (int q = 0; q < 32; q += 4) {
                         if (j0 + q < M) {
-                            const float4 m = *reinterpret_cast<const float4 *>(row + j0 + q);
+                            const float4 m = *reinterpret_cast<const float4 *>(row + ((size_t)(j0 + q) << 5));
                             const float4 v = *reinterpret_cast<const float4 *>(x + j0 + q);
                             p[q] = __builtin_fmaf(v.x, m.x, p[q]);
                             p[q + 1] = __builtin_fmaf(v.y, m.y, p[q + 1]);
@@ -1967,21 +1973,37 @@ __global__ void mlp_bwd_input(const float *__restrict__ prm, int w_off, const fl
 }
 
 // grad W[j][k] = sum_s dout[s][j] in[s][k]; column k == K is the bias.
+// Pass 1: blockIdx.y takes samples [y * WGRAD_CHUNK, (y + 1) * WGRAD_CHUNK)
+// and writes its partial sums to part[y][N * (K + 1)]; pass 2 adds the
+// partials in chunk order (deterministic).
+constexpr int WGRAD_CHUNK = 128;
 template <int N, int K>
-__global__ void mlp_wgrad(const float *__restrict__ dout, const float *__restrict__ in, int in_ld, int n,
-                          float *__restrict__ grad, int w_off, int b_off) {
+__global__ void mlp_wgrad_part(const float *__restrict__ dout, const float *__restrict__ in, int in_ld, int n,
+                               float *__restrict__ part) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= N * (K + 1)) return;
     const int j = idx / (K + 1), k = idx % (K + 1);
+    const int s0 = blockIdx.y * WGRAD_CHUNK, s1 = min(n, s0 + WGRAD_CHUNK);
     float acc = 0.0f;
     if (k == K) {
-        for (int smp = 0; smp < n; ++smp) acc += dout[(size_t)smp * N + j];
-        grad[b_off + j] = acc;
+        for (int smp = s0; smp < s1; ++smp) acc += dout[(size_t)smp * N + j];
     } else {
-        for (int smp = 0; smp < n; ++smp)
+        for (int smp = s0; smp < s1; ++smp)
             acc = __builtin_fmaf(dout[(size_t)smp * N + j], in[(size_t)smp * in_ld + k], acc);
-        grad[w_off + (size_t)j * K + k] = acc;
     }
+    part[(size_t)blockIdx.y * (N * (K + 1)) + idx] = acc;
+}
+
+template <int N, int K>
+__global__ void mlp_wgrad_sum(const float *__restrict__ part, int nchunk, float *__restrict__ grad, int w_off,
+                              int b_off) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= N * (K + 1)) return;
+    float acc = 0.0f;
+    for (int c = 0; c < nchunk; ++c) acc += part[(size_t)c * (N * (K + 1)) + idx];
+    const int j = idx / (K + 1), k = idx % (K + 1);
+    if (k == K) grad[b_off + j] = acc;
+    else grad[w_off + (size_t)j * K + k] = acc;
 }
 
 __global__ void __launch_bounds__(256) mean_kernel(const float *__restrict__ x, int n, float *__restrict__ out) {
@@ -2638,12 +2660,23 @@ int cuslines_set_boot_matrices(cuslines_tracker *t, const float *H, const float 
     t->dp.hr32 = ceil32(hr);
     float **dst[5] = {&t->d_H, &t->d_R, &t->d_db, &t->d_dq, &t->d_smat};
     const float *src[5] = {H, R, delta_b, delta_q, sampling_matrix};
-    const size_t cnt[5] = {(size_t)hr * hr, (size_t)hr * hr, (size_t)dn * hr, (size_t)dn * hr, (size_t)nv * dn};
+    const int rows[5] = {hr, hr, dn, dn, nv}, cols[5] = {hr, hr, hr, hr, dn};
     for (int k = 0; k < 5; ++k) {
+        // gemv_lanerow's layout: lane-interleaved when the row length is a
+        // multiple of 4 (rows padded to a multiple of 32 with zeros), else row-major
+        const int N = rows[k], M = cols[k];
+        std::vector<float> h;
+        if ((M & 3) == 0) {
+            h.assign((size_t)ceil32(N) * M, 0.0f);
+            for (int i = 0; i < N; ++i)
+                for (int j = 0; j < M; ++j) h[mat_offset(M, i, j)] = src[k][(size_t)i * M + j];
+        } else {
+            h.assign(src[k], src[k] + (size_t)N * M);
+        }
         dfree(*dst[k]);
         *dst[k] = nullptr;
-        HIPC(hipMalloc(dst[k], cnt[k] * sizeof(float)));
-        HIPC(hipMemcpy(*dst[k], src[k], cnt[k] * sizeof(float), hipMemcpyHostToDevice));
+        HIPC(hipMalloc(dst[k], h.size() * sizeof(float)));
+        HIPC(hipMemcpy(*dst[k], h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
     }
     dfree(t->d_b0); dfree(t->d_dwi);
     t->d_b0 = nullptr; t->d_dwi = nullptr;
@@ -2837,7 +2870,11 @@ int cuslines_proposal_stats(cuslines_tracker *t, int64_t *out3, int reset) {
     return CUSLINES_OK;
 }
 
-static size_t mlp_ws_floats(int n) { return (size_t)n * (4 * MLP_H + 2 * MLP_OUT + 1); }  // H1 H2 dH2 dH1, O dO, loss
+// H1 H2 dH2 dH1, O dO, loss; then the weight-gradient partials of the largest layer
+static size_t mlp_ws_floats(int n) {
+    const size_t nchunk = (size_t)((n + WGRAD_CHUNK - 1) / WGRAD_CHUNK);
+    return (size_t)n * (4 * MLP_H + 2 * MLP_OUT + 1) + nchunk * MLP_H * (MLP_H + 1);
+}
 
 int cuslines_mlp_workspace_bytes(int32_t n, size_t *bytes) {
     if (!bytes || n <= 0) return fail(CUSLINES_EINVAL, "n > 0 and bytes required");
@@ -2857,6 +2894,8 @@ int cuslines_mlp_loss_grad_async(cuslines_tracker *t, const float *params_d, con
     float *dH2 = dO + (size_t)n * MLP_OUT;
     float *dH1 = dH2 + (size_t)n * MLP_H;
     float *ls = dH1 + (size_t)n * MLP_H;
+    float *part = ls + n;
+    const unsigned nchunk = (unsigned)((n + WGRAD_CHUNK - 1) / WGRAD_CHUNK);
     const float *X = pairs_d;
     const int LD = CUSLINES_PAIR_FLOATS;
     auto g1 = [](long long cnt) { return dim3((unsigned)((cnt + 255) / 256)); };
@@ -2868,16 +2907,21 @@ int cuslines_mlp_loss_grad_async(cuslines_tracker *t, const float *params_d, con
     hipLaunchKernelGGL((mlp_fwd_layer<MLP_H, MLP_OUT, false>), g1((long long)n * MLP_OUT), dim3(256), 0, st,
                        params_d, MLP_W3, MLP_B3, H2, MLP_H, n, O);
     hipLaunchKernelGGL(mlp_loss_out, g1(n), dim3(256), 0, st, O, X, n, dO, ls);
-    hipLaunchKernelGGL((mlp_wgrad<MLP_OUT, MLP_H>), g1(MLP_OUT * (MLP_H + 1)), dim3(256), 0, st, dO, H2, MLP_H, n,
-                       grad_d, MLP_W3, MLP_B3);
+    auto wgrad = [&](auto part_k, auto sum_k, int tiles, const float *dout, const float *in, int ld, int w, int b) {
+        hipLaunchKernelGGL(part_k, dim3((unsigned)((tiles + 255) / 256), nchunk), dim3(256), 0, st, dout, in, ld, n,
+                           part);
+        hipLaunchKernelGGL(sum_k, g1(tiles), dim3(256), 0, st, part, (int)nchunk, grad_d, w, b);
+    };
+    wgrad(mlp_wgrad_part<MLP_OUT, MLP_H>, mlp_wgrad_sum<MLP_OUT, MLP_H>, MLP_OUT * (MLP_H + 1), dO, H2, MLP_H, MLP_W3,
+          MLP_B3);
     hipLaunchKernelGGL((mlp_bwd_input<MLP_OUT, MLP_H>), g1((long long)n * MLP_H), dim3(256), 0, st, params_d, MLP_W3,
                        dO, H2, n, dH2);
-    hipLaunchKernelGGL((mlp_wgrad<MLP_H, MLP_H>), g1(MLP_H * (MLP_H + 1)), dim3(256), 0, st, dH2, H1, MLP_H, n, grad_d,
-                       MLP_W2, MLP_B2);
+    wgrad(mlp_wgrad_part<MLP_H, MLP_H>, mlp_wgrad_sum<MLP_H, MLP_H>, MLP_H * (MLP_H + 1), dH2, H1, MLP_H, MLP_W2,
+          MLP_B2);
     hipLaunchKernelGGL((mlp_bwd_input<MLP_H, MLP_H>), g1((long long)n * MLP_H), dim3(256), 0, st, params_d, MLP_W2,
                        dH2, H1, n, dH1);
-    hipLaunchKernelGGL((mlp_wgrad<MLP_H, MLP_IN>), g1(MLP_H * (MLP_IN + 1)), dim3(256), 0, st, dH1, X, LD, n, grad_d,
-                       MLP_W1, MLP_B1);
+    wgrad(mlp_wgrad_part<MLP_H, MLP_IN>, mlp_wgrad_sum<MLP_H, MLP_IN>, MLP_H * (MLP_IN + 1), dH1, X, LD, MLP_W1,
+          MLP_B1);
     hipLaunchKernelGGL(mean_kernel, dim3(1), dim3(256), 0, st, ls, n, loss_d);
     HIPC(hipGetLastError());
     return CUSLINES_OK;

@@ -71,12 +71,12 @@ class GPUDirectionGetter(ABC):
     # Phase launches through the device tracker of GPU ``n`` -------------
     def _phase1(self, n, nseeds_gpu, sp):
         dt = sp.gpu_tracker.device_trackers[n]
-        dt.phase1_async(sp.seeds_d[n].ptr, nseeds_gpu, nseeds_gpu, sp.slinesOffs_d[n].ptr,
+        dt.phase1_async(sp.seeds_d[n].ptr, nseeds_gpu, sp.rng_chunk or nseeds_gpu, sp.slinesOffs_d[n].ptr,
                         sp.shDirTemp0_d[n].ptr)
 
     def _phase2(self, n, nseeds_gpu, sp):
         dt = sp.gpu_tracker.device_trackers[n]
-        dt.phase2_async(sp.seeds_d[n].ptr, nseeds_gpu, nseeds_gpu, sp.slinesScan_d[n].ptr,
+        dt.phase2_async(sp.seeds_d[n].ptr, nseeds_gpu, sp.rng_chunk or nseeds_gpu, sp.slinesScan_d[n].ptr,
                         sp.shDirTemp0_d[n].ptr, sp.slineSeed_d[n].ptr, sp.slineLen_d[n].ptr,
                         sp.sline_d[n].ptr)
 

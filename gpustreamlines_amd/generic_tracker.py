@@ -59,8 +59,16 @@ class GenericTracker:
         seeds = np.asarray(seeds)
         gsz, nchunks = self._divide_chunks(seeds)
         sp = self.seed_propagator
-        for idx in range(nchunks):
-            sp.propagate(seeds[idx * gsz: (idx + 1) * gsz])
+        # On one GPU several chunks go into one launch: each chunk keeps its
+        # own random streams (rng_chunk), so the output is the same as chunk
+        # by chunk, and the persistent kernel's tail of long streamlines is
+        # paid once per launch instead of once per chunk.
+        per = 1
+        if self._ngpus() == 1:
+            per = max(1, int(getattr(self, "launch_seeds", 0) or 0) // gsz)
+        sp.rng_chunk = gsz if per > 1 else None
+        for idx in range(0, nchunks, per):
+            sp.propagate(seeds[idx * gsz: (idx + per) * gsz])
             for ii in range(sp.ngpus):
                 pts, lens, offs = sp.slines[ii], sp.sline_lens[ii], sp.sline_offs[ii]
                 if lens is None or len(lens) == 0:

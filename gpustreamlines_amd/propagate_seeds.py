@@ -38,6 +38,10 @@ class SeedBatchPropagator:
         self.sline_lens = [None] * self.ngpus
         self.sline_offs = [None] * self.ngpus
         self.sline_seeds = [None] * self.ngpus
+        # Random streams restart every rng_chunk seeds of a launch (None: one
+        # chunk per launch, as the reference); set when one launch carries
+        # several chunks (GenericTracker, one GPU).
+        self.rng_chunk = None
         self._reset_buffers()
 
     def _reset_buffers(self):

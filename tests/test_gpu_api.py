@@ -24,8 +24,11 @@ def _oracle_by_chunks(w, seeds, chunk):
     return np.concatenate(pts), np.concatenate(lens)
 
 
+@pytest.mark.parametrize("launch_seeds", [0, 250000])
 @pytest.mark.parametrize("kind", ["prob", "ptt", "csa"])
-def test_gpu_tracker_generate_streamlines(small_workloads, kind):
+def test_gpu_tracker_generate_streamlines(small_workloads, kind, launch_seeds):
+    """launch_seeds 0: one chunk per launch (the reference's protocol);
+    250000: all chunks in one launch with chunk-relative streams."""
     from gpustreamlines_amd import (BootDirectionGetter, GPUTracker, ProbDirectionGetter, PttDirectionGetter)
     from gpustreamlines_amd import workloads as W
 
@@ -39,9 +42,9 @@ def test_gpu_tracker_generate_streamlines(small_workloads, kind):
         dg, data = PttDirectionGetter(), small_workloads["prob"].dataf  # raw ODF; the getter prepares it
     else:
         dg, data = BootDirectionGetter("CSA", 1.0, **{k: v for k, v in w.boot.items()}), w.dataf
-    print(f"[api] {kind}", flush=True)
+    print(f"[api] {kind} launch_seeds={launch_seeds}", flush=True)
     with GPUTracker(dg, data, w.stop_map, p["tc_threshold"], w.vertices, w.edges, step_size=p["step_size"],
-                    chunk_size=chunk) as tr:
+                    chunk_size=chunk, launch_seeds=launch_seeds) as tr:
         out = tr.generate_streamlines(seeds)
     pts, lens = _oracle_by_chunks(w, seeds, chunk)
     assert np.array_equal(out._lengths, lens)
