@@ -60,9 +60,10 @@ __constant__ int c_disc_face[DISC_FACE_CNT * 3] = {
     23, 18, 21, 21, 18, 15, 2, 4, 8,    2, 5, 0,    8, 5, 2,    7, 10, 12,  6, 7, 3,    10, 7, 6,
     3, 1, 6,    1, 5, 6,    11, 10, 6,  6, 5, 11,   14, 19, 15, 15, 13, 14, 14, 13, 9};
 
-// Partner lane of every layer of the 32-key sort network; filled at tracker
-// creation from the generator below.
-__constant__ unsigned char c_sortnet[SORT_LAYERS][32];
+// Partner lane of every layer of the 32-key sort network, lane-major: lane g
+// reads its 16 bytes (byte L = partner in layer L) with one load.  Filled at
+// tracker creation from the generator below.
+__constant__ uint4 c_sortnet4[32];
 
 // Knuth's merge exchange (TAOCP 5.2.2 Algorithm M) for 32 keys with each
 // comparator placed in the earliest layer its inputs allow: the network of
@@ -112,6 +113,7 @@ struct DevBufs {
     const float *stop;
     const float *verts;
     const uint32_t *edges;
+    const uint16_t *nbr;  // [nverts][8] neighbours, padded with the vertex itself (nullptr: edge path)
     const float *H, *R, *db, *dq, *smat;
     const int *b0;
     const int *dwi_idx;
@@ -131,9 +133,11 @@ struct DevBufs {
 // warp_sort<32,32,WSORT_DIR_DEC> (cuwsort.cuh:104-168) on (key, value).
 __device__ __forceinline__ void sort_dec(float &k, int &v) {
     const int g = lane32();
+    const uint4 net = c_sortnet4[g];
 #pragma unroll
     for (int L = 0; L < SORT_LAYERS; ++L) {
-        const int src = c_sortnet[L][g];
+        const uint32_t w = (L < 4) ? net.x : (L < 8) ? net.y : (L < 12) ? net.z : net.w;
+        const int src = (int)((w >> (8 * (L & 3))) & 0xffu);
         const float a = shfl32(k, src);
         const int b = shfl32(v, src);
         if (g >= src) {
@@ -172,6 +176,8 @@ __device__ __forceinline__ uint32_t peak_mark(const uint32_t *marks, int j) {
     return (marks[j >> 4] >> (2 * (j & 15))) & 3u;
 }
 
+__device__ int sort_dedup_peaks(const DevParams &P, const float *odf, uint16_t *cand, const float *sv, int n);
+
 __device__ int peak_directions(const DevParams &P, const DevBufs &B, const float *odf, uint32_t *marks,
                                uint16_t *cand, const float *sv) {
     const int l = lane32();
@@ -180,6 +186,42 @@ __device__ int peak_directions(const DevParams &P, const DevBufs &B, const float
     for (int j = l; j < nv; j += 32) mn = (mn < odf[j]) ? mn : odf[j];
     mn = min32(mn);
     mn = (0.0f > mn) ? 0.0f : mn;
+    if (B.nbr) {
+        // Vertex-centred form of the edge marks below: a vertex is a peak when
+        // some neighbour is strictly smaller and none strictly larger.  Bit r
+        // of pk: vertex l + 32 r (nverts <= 1024).
+        uint32_t pk = 0u;
+        float m = REAL_MIN;
+        for (int r = 0, j = l; j < nv; ++r, j += 32) {
+            const uint4 q = reinterpret_cast<const uint4 *>(B.nbr)[j];
+            const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+            const float x = odf[j];
+            bool smaller = false, larger = false;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const float y = odf[(w[k >> 1] >> (16 * (k & 1))) & 0xffffu];
+                smaller = smaller || (y < x);
+                larger = larger || (x < y);
+            }
+            if (smaller && !larger) {
+                pk |= 1u << r;
+                const float xm = x + (-mn);
+                m = (m > xm) ? m : xm;
+            }
+        }
+        m = max32(m);
+        const float comp = (float)((double)P.rel_peak * (double)m);
+        int n = 0;
+        for (int r = 0, j0 = 0; j0 < nv; ++r, j0 += 32) {
+            const int j = j0 + l;
+            const bool keep = (j < nv) && ((pk >> r) & 1u) && ((odf[j] - mn) >= comp);
+            const uint32_t msk = half_ballot(keep);
+            if (keep) cand[n + __popc(msk & ((1u << l) - 1u))] = (uint16_t)j;
+            n += __popc(msk);
+        }
+        WAVE_SYNC();
+        return sort_dedup_peaks(P, odf, cand, sv, n);
+    }
     for (int w = l; w < (nv + 15) >> 4; w += 32) marks[w] = 0u;
     WAVE_SYNC();
     for (int e = l; e < P.nedges; e += 32) {
@@ -209,6 +251,14 @@ __device__ int peak_directions(const DevParams &P, const DevBufs &B, const float
         n += __popc(msk);
     }
     WAVE_SYNC();
+    return sort_dedup_peaks(P, odf, cand, sv, n);
+}
+
+// The tail of peak_directions_d (tracking_helpers.cu:182-245) on the n
+// candidates in cand: descending sort when n < 32, then the greedy
+// minimum-separation filter on lane 0.
+__device__ int sort_dedup_peaks(const DevParams &P, const float *odf, uint16_t *cand, const float *sv, int n) {
+    const int l = lane32();
     if (n < 32) {
         float k = REAL_MIN;
         int v = 0;
@@ -2212,6 +2262,7 @@ struct cuslines_tracker {
     hipStream_t stream = nullptr;
     float *d_data = nullptr, *d_stop = nullptr, *d_verts = nullptr;
     uint32_t *d_edges = nullptr;
+    uint16_t *d_nbr = nullptr;  // vertex adjacency, 8 per vertex (nullptr: a degree above 8 or > 1024 vertices)
     float *d_H = nullptr, *d_R = nullptr, *d_db = nullptr, *d_dq = nullptr, *d_smat = nullptr;
     int *d_b0 = nullptr, *d_dwi = nullptr;
     uint32_t *d_lut = nullptr;
@@ -2246,6 +2297,7 @@ static DevBufs bufs_of(const cuslines_tracker *t) {
     b.stop = t->d_stop;
     b.verts = t->d_verts;
     b.edges = t->d_edges;
+    b.nbr = t->d_nbr;
     b.H = t->d_H; b.R = t->d_R; b.db = t->d_db; b.dq = t->d_dq; b.smat = t->d_smat;
     b.b0 = t->d_b0;
     b.dwi_idx = t->d_dwi;
@@ -2498,7 +2550,10 @@ int cuslines_tracker_create(const cuslines_params *p, int device, cuslines_track
     unsigned char net[SORT_LAYERS][32];
     int nl = 0;
     make_sortnet(net, &nl);
-    e = hipMemcpyToSymbol(HIP_SYMBOL(c_sortnet), net, sizeof(net));
+    uint32_t packed_net[32][4] = {};
+    for (int L = 0; L < SORT_LAYERS; ++L)
+        for (int g = 0; g < 32; ++g) packed_net[g][L >> 2] |= (uint32_t)net[L][g] << (8 * (L & 3));
+    e = hipMemcpyToSymbol(HIP_SYMBOL(c_sortnet4), packed_net, sizeof(packed_net));
     if (e != hipSuccess) {
         dfree(t->d_work);
         (void)hipStreamDestroy(t->stream);
@@ -2537,7 +2592,7 @@ int cuslines_tracker_destroy(cuslines_tracker *t) {
     if (!t) return CUSLINES_OK;
     (void)hipSetDevice(t->device);
     if (t->stream) (void)hipStreamSynchronize(t->stream);
-    dfree(t->d_data); dfree(t->d_stop); dfree(t->d_verts); dfree(t->d_edges);
+    dfree(t->d_data); dfree(t->d_stop); dfree(t->d_verts); dfree(t->d_edges); dfree(t->d_nbr);
     dfree(t->d_H); dfree(t->d_R); dfree(t->d_db); dfree(t->d_dq); dfree(t->d_smat);
     dfree(t->d_b0); dfree(t->d_dwi); dfree(t->d_lut); dfree(t->d_work); dfree(t->d_blocks);
     dfree(t->d_fprop); dfree(t->d_mlp); dfree(t->d_stats);
@@ -2630,12 +2685,34 @@ int cuslines_set_sphere(cuslines_tracker *t, const float *vertices, const int32_
         if (u < 0 || u >= nv || v < 0 || v >= nv) return fail(CUSLINES_EINVAL, "edge vertex index out of range");
         packed[e] = (uint32_t)u | ((uint32_t)v << 16);
     }
-    dfree(t->d_verts); dfree(t->d_edges);
-    t->d_verts = nullptr; t->d_edges = nullptr;
+    dfree(t->d_verts); dfree(t->d_edges); dfree(t->d_nbr);
+    t->d_verts = nullptr; t->d_edges = nullptr; t->d_nbr = nullptr;
     HIPC(hipMalloc(&t->d_verts, (size_t)nv * 3 * sizeof(float)));
     HIPC(hipMemcpy(t->d_verts, vertices, (size_t)nv * 3 * sizeof(float), hipMemcpyHostToDevice));
     HIPC(hipMalloc(&t->d_edges, (size_t)ne * sizeof(uint32_t)));
     HIPC(hipMemcpy(t->d_edges, packed.data(), (size_t)ne * sizeof(uint32_t), hipMemcpyHostToDevice));
+    // Adjacency for the peak test (peak_directions): up to 8 distinct
+    // neighbours per vertex, the rest of the row the vertex itself.
+    std::vector<uint16_t> nbr((size_t)nv * 8);
+    std::vector<int> deg(nv, 0);
+    bool fits = nv <= 1024;
+    for (int v = 0; v < nv; ++v)
+        for (int k = 0; k < 8; ++k) nbr[(size_t)v * 8 + k] = (uint16_t)v;
+    for (int e = 0; e < ne && fits; ++e) {
+        const int a = edges[2 * e], b = edges[2 * e + 1];
+        for (int side = 0; side < 2 && fits; ++side) {
+            const int x = side ? b : a, y = side ? a : b;
+            bool seen = false;
+            for (int k = 0; k < deg[x]; ++k) seen = seen || nbr[(size_t)x * 8 + k] == y;
+            if (seen) continue;
+            if (deg[x] == 8) { fits = false; break; }
+            nbr[(size_t)x * 8 + deg[x]++] = (uint16_t)y;
+        }
+    }
+    if (fits) {
+        HIPC(hipMalloc(&t->d_nbr, nbr.size() * sizeof(uint16_t)));
+        HIPC(hipMemcpy(t->d_nbr, nbr.data(), nbr.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    }
     t->has_sphere = true;
     return CUSLINES_OK;
 }
