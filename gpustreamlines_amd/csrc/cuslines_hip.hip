@@ -120,7 +120,7 @@ struct DevBufs {
     const uint32_t *lut;  // PTT LUT entry: vertex | (direction rank << 16)
     const float *blocks;  // PTT corner blocks (nullptr: gather 8 texels)
     const float *fprop;    // learned proposal: per disc vertex, propagator of 4 probe steps [24][9]
-    const uint16_t *mlp;   // proposal network, bf16 (MLP_NPARAM)
+    const uint16_t *mlp;   // proposal network, bf16, lane-interleaved (IL_NPARAM)
     float *rec;            // training pairs [rec_cap][PAIR_FLOATS]
     int *rec_count;
     unsigned long long *stats;  // learned sub-steps, fallbacks, plain sub-steps
@@ -1009,6 +1009,17 @@ constexpr int MLP_IN = CUSLINES_MLP_IN, MLP_H = CUSLINES_MLP_HIDDEN, MLP_OUT = C
 constexpr int MLP_W1 = 0, MLP_B1 = MLP_W1 + MLP_H * MLP_IN, MLP_W2 = MLP_B1 + MLP_H,
               MLP_B2 = MLP_W2 + MLP_H * MLP_H, MLP_W3 = MLP_B2 + MLP_H, MLP_B3 = MLP_W3 + MLP_OUT * MLP_H,
               MLP_NPARAM = MLP_B3 + MLP_OUT;
+// The network as the tracking kernel reads it: each weight matrix stored
+// lane-interleaved (mlp_il_offset: the 8-weight pieces of 32 consecutive
+// rows side by side, rows padded to 32), so lane j's 16-byte row loads of
+// one instruction are 512 contiguous bytes; biases after the matrices.
+constexpr int MLP_OUT32 = 32;
+constexpr int IL_W1 = 0, IL_W2 = IL_W1 + MLP_H * MLP_IN, IL_W3 = IL_W2 + MLP_H * MLP_H,
+              IL_B1 = IL_W3 + MLP_OUT32 * MLP_H, IL_B2 = IL_B1 + MLP_H, IL_B3 = IL_B2 + MLP_H,
+              IL_NPARAM = IL_B3 + MLP_OUT32;
+__host__ __device__ inline int mlp_il_offset(int K, int j, int k) {
+    return (((j >> 5) * (K >> 3) + (k >> 3)) << 8) + ((j & 31) << 3) + (k & 7);
+}
 static_assert(MLP_NPARAM == CUSLINES_MLP_NPARAM, "MLP layout");
 static_assert(MLP_OUT == DISC_VERT_CNT, "one output per disc vertex");
 constexpr int ACT_X = 0, ACT_H1 = 64, ACT_H2 = 192, ACT_O = 320, ACT_FLOATS = 352;
@@ -1051,12 +1062,13 @@ __device__ __forceinline__ void mlp_features(const DevParams &P, const DevBufs &
 
 // One neuron: bias + sum_k w[k] in[k] as an fma chain, K a multiple of 8.
 template <int K>
+// w: the neuron's row in the lane-interleaved layout (mlp_il_offset(K, j, 0)).
 __device__ __forceinline__ float mlp_neuron(const uint16_t *__restrict__ w, uint16_t bias, const float *in) {
     float acc = bf16f(bias);
     const uint4 *w8 = reinterpret_cast<const uint4 *>(w);
 #pragma unroll 2
     for (int k8 = 0; k8 < K / 8; ++k8) {
-        const uint4 q = w8[k8];
+        const uint4 q = w8[k8 * 32];
         const float4 a = *reinterpret_cast<const float4 *>(in + 8 * k8);
         const float4 b = *reinterpret_cast<const float4 *>(in + 8 * k8 + 4);
         acc = __builtin_fmaf(bf16f(q.x & 0xffffu), a.x, acc);
@@ -1077,18 +1089,19 @@ __device__ __noinline__ void mlp_forward(const uint16_t *__restrict__ W, float *
 #pragma unroll 1
     for (int r = 0; r < 4; ++r) {
         const int j = l + 32 * r;
-        const float v = mlp_neuron<MLP_IN>(W + MLP_W1 + j * MLP_IN, W[MLP_B1 + j], act + ACT_X);
+        const float v = mlp_neuron<MLP_IN>(W + IL_W1 + mlp_il_offset(MLP_IN, j, 0), W[IL_B1 + j], act + ACT_X);
         act[ACT_H1 + j] = (v > 0.0f) ? v : 0.0f;
     }
     WAVE_SYNC();
 #pragma unroll 1
     for (int r = 0; r < 4; ++r) {
         const int j = l + 32 * r;
-        const float v = mlp_neuron<MLP_H>(W + MLP_W2 + j * MLP_H, W[MLP_B2 + j], act + ACT_H1);
+        const float v = mlp_neuron<MLP_H>(W + IL_W2 + mlp_il_offset(MLP_H, j, 0), W[IL_B2 + j], act + ACT_H1);
         act[ACT_H2 + j] = (v > 0.0f) ? v : 0.0f;
     }
     WAVE_SYNC();
-    if (l < MLP_OUT) act[ACT_O + l] = mlp_neuron<MLP_H>(W + MLP_W3 + l * MLP_H, W[MLP_B3 + l], act + ACT_H2);
+    if (l < MLP_OUT)
+        act[ACT_O + l] = mlp_neuron<MLP_H>(W + IL_W3 + mlp_il_offset(MLP_H, l, 0), W[IL_B3 + l], act + ACT_H2);
     WAVE_SYNC();
 }
 
@@ -2906,8 +2919,18 @@ int cuslines_set_proposal(cuslines_tracker *t, const uint16_t *mlp_bf16) {
     }
     if (t->p.model_type != CUSLINES_PTT) return fail(CUSLINES_EINVAL, "the learned proposal is for PTT");
     if (int rc = ensure_feature_propagators(t)) return rc;
-    if (!t->d_mlp) HIPC(hipMalloc(&t->d_mlp, MLP_NPARAM * sizeof(uint16_t)));
-    HIPC(hipMemcpy(t->d_mlp, mlp_bf16, MLP_NPARAM * sizeof(uint16_t), hipMemcpyHostToDevice));
+    std::vector<uint16_t> il(IL_NPARAM, 0);
+    auto put = [&](int dst, int src, int N, int K) {
+        for (int j = 0; j < N; ++j)
+            for (int k = 0; k < K; ++k) il[dst + mlp_il_offset(K, j, k)] = mlp_bf16[src + j * K + k];
+    };
+    put(IL_W1, MLP_W1, MLP_H, MLP_IN);
+    put(IL_W2, MLP_W2, MLP_H, MLP_H);
+    put(IL_W3, MLP_W3, MLP_OUT, MLP_H);
+    for (int j = 0; j < MLP_H; ++j) { il[IL_B1 + j] = mlp_bf16[MLP_B1 + j]; il[IL_B2 + j] = mlp_bf16[MLP_B2 + j]; }
+    for (int j = 0; j < MLP_OUT; ++j) il[IL_B3 + j] = mlp_bf16[MLP_B3 + j];
+    if (!t->d_mlp) HIPC(hipMalloc(&t->d_mlp, IL_NPARAM * sizeof(uint16_t)));
+    HIPC(hipMemcpy(t->d_mlp, il.data(), IL_NPARAM * sizeof(uint16_t), hipMemcpyHostToDevice));
     if (!t->d_stats) {
         HIPC(hipMalloc(&t->d_stats, 4 * sizeof(unsigned long long)));
         HIPC(hipMemset(t->d_stats, 0, 4 * sizeof(unsigned long long)));

@@ -151,3 +151,24 @@ def test_max_length_truncation_bit_exact(small_workloads, kind):
     assert g["sline_len"].max() <= 2 * 12 - 1
     assert (g["sline_len"] >= 12).any()
     _assert_same(g, o)
+
+
+@pytest.mark.parametrize("kind", ["prob", "csa"])
+def test_peaks_edge_path_bit_exact(small_workloads, kind):
+    """Vertex 0 joined to its 9 nearest other vertices has more than 8
+    neighbours, so the library takes the edge form of the peak test instead
+    of its 8-neighbour table; the oracle sees the same edge list."""
+    import dataclasses
+
+    from gpustreamlines_amd import workloads as W
+
+    w = small_workloads[kind]
+    d = w.vertices.astype(np.float64) @ w.vertices[0].astype(np.float64)
+    near = [int(j) for j in np.argsort(-d) if j != 0][:9]
+    edges = np.concatenate([w.edges, np.array([[0, j] for j in near], np.int32)]).astype(np.int32)
+    params = dict(w.params, nedges=len(edges))
+    w2 = dataclasses.replace(w, edges=edges, params=params)
+    seeds = W.seeds_for(w2, NSEEDS[kind], placement="mixed")
+    g, o = _run_both(w2, seeds)
+    assert len(o["sline_len"]) > 0
+    _assert_same(g, o)
