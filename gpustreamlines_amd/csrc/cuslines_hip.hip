@@ -1061,8 +1061,39 @@ __device__ __forceinline__ void mlp_features(const DevParams &P, const DevBufs &
 }
 
 // One neuron: bias + sum_k w[k] in[k] as an fma chain, K a multiple of 8.
+// Four neurons of one lane (rows j, j + 32, j + 64, j + 96 of the layer),
+// each summed in input order like a single neuron; w: row j in the
+// lane-interleaved layout (mlp_il_offset(K, j, 0)), the other rows 32 rows
+// further, i.e. (K / 8) * 256 weights on.
 template <int K>
+__device__ __forceinline__ void mlp_neuron4(const uint16_t *__restrict__ w, const uint16_t *bias, const float *in,
+                                            float *acc) {
+    const uint4 *w8 = reinterpret_cast<const uint4 *>(w);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[r] = bf16f(bias[32 * r]);
+#pragma unroll 2
+    for (int k8 = 0; k8 < K / 8; ++k8) {
+        const float4 a = *reinterpret_cast<const float4 *>(in + 8 * k8);
+        const float4 b = *reinterpret_cast<const float4 *>(in + 8 * k8 + 4);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const uint4 q = w8[k8 * 32 + r * (K / 8) * 32];
+            float c = acc[r];
+            c = __builtin_fmaf(bf16f(q.x & 0xffffu), a.x, c);
+            c = __builtin_fmaf(bf16f(q.x >> 16), a.y, c);
+            c = __builtin_fmaf(bf16f(q.y & 0xffffu), a.z, c);
+            c = __builtin_fmaf(bf16f(q.y >> 16), a.w, c);
+            c = __builtin_fmaf(bf16f(q.z & 0xffffu), b.x, c);
+            c = __builtin_fmaf(bf16f(q.z >> 16), b.y, c);
+            c = __builtin_fmaf(bf16f(q.w & 0xffffu), b.z, c);
+            c = __builtin_fmaf(bf16f(q.w >> 16), b.w, c);
+            acc[r] = c;
+        }
+    }
+}
+
 // w: the neuron's row in the lane-interleaved layout (mlp_il_offset(K, j, 0)).
+template <int K>
 __device__ __forceinline__ float mlp_neuron(const uint16_t *__restrict__ w, uint16_t bias, const float *in) {
     float acc = bf16f(bias);
     const uint4 *w8 = reinterpret_cast<const uint4 *>(w);
@@ -1086,19 +1117,14 @@ __device__ __forceinline__ float mlp_neuron(const uint16_t *__restrict__ w, uint
 // Kept out of line so the tracking loop keeps its register budget.
 __device__ __noinline__ void mlp_forward(const uint16_t *__restrict__ W, float *act) {
     const int l = lane32();
-#pragma unroll 1
-    for (int r = 0; r < 4; ++r) {
-        const int j = l + 32 * r;
-        const float v = mlp_neuron<MLP_IN>(W + IL_W1 + mlp_il_offset(MLP_IN, j, 0), W[IL_B1 + j], act + ACT_X);
-        act[ACT_H1 + j] = (v > 0.0f) ? v : 0.0f;
-    }
+    float v[4];
+    mlp_neuron4<MLP_IN>(W + IL_W1 + mlp_il_offset(MLP_IN, l, 0), W + IL_B1 + l, act + ACT_X, v);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) act[ACT_H1 + l + 32 * r] = (v[r] > 0.0f) ? v[r] : 0.0f;
     WAVE_SYNC();
-#pragma unroll 1
-    for (int r = 0; r < 4; ++r) {
-        const int j = l + 32 * r;
-        const float v = mlp_neuron<MLP_H>(W + IL_W2 + mlp_il_offset(MLP_H, j, 0), W[IL_B2 + j], act + ACT_H1);
-        act[ACT_H2 + j] = (v > 0.0f) ? v : 0.0f;
-    }
+    mlp_neuron4<MLP_H>(W + IL_W2 + mlp_il_offset(MLP_H, l, 0), W + IL_B2 + l, act + ACT_H1, v);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) act[ACT_H2 + l + 32 * r] = (v[r] > 0.0f) ? v[r] : 0.0f;
     WAVE_SYNC();
     if (l < MLP_OUT)
         act[ACT_O + l] = mlp_neuron<MLP_H>(W + IL_W3 + mlp_il_offset(MLP_H, l, 0), W[IL_B3 + l], act + ACT_H2);
