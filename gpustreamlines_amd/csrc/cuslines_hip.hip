@@ -1017,6 +1017,7 @@ constexpr int MLP_OUT32 = 32;
 constexpr int IL_W1 = 0, IL_W2 = IL_W1 + MLP_H * MLP_IN, IL_W3 = IL_W2 + MLP_H * MLP_H,
               IL_B1 = IL_W3 + MLP_OUT32 * MLP_H, IL_B2 = IL_B1 + MLP_H, IL_B3 = IL_B2 + MLP_H,
               IL_NPARAM = IL_B3 + MLP_OUT32;
+constexpr int MLP_LDS_FLOATS = ((IL_NPARAM + 1) / 2 + 3) & ~3;  // the network staged in LDS (learned kernels)
 __host__ __device__ inline int mlp_il_offset(int K, int j, int k) {
     return (((j >> 5) * (K >> 3) + (k >> 3)) << 8) + ((j & 31) << 3) + (k & 7);
 }
@@ -1061,23 +1062,32 @@ __device__ __forceinline__ void mlp_features(const DevParams &P, const DevBufs &
 }
 
 // One neuron: bias + sum_k w[k] in[k] as an fma chain, K a multiple of 8.
+// Address-space-qualified pointers for the out-of-line network: its
+// arguments arrive as generic pointers, and generic (flat) loads are slower
+// than ds loads.  The learned kernels stage the network in LDS
+// (stage_network), so weights and activations are both LDS reads.
+#define AS_LDS __attribute__((address_space(3)))
+typedef const AS_LDS uint16_t GWeight;
+typedef AS_LDS float LAct;
+typedef float vf4 __attribute__((ext_vector_type(4)));
+typedef uint32_t vu4 __attribute__((ext_vector_type(4)));
+
 // Four neurons of one lane (rows j, j + 32, j + 64, j + 96 of the layer),
 // each summed in input order like a single neuron; w: row j in the
 // lane-interleaved layout (mlp_il_offset(K, j, 0)), the other rows 32 rows
 // further, i.e. (K / 8) * 256 weights on.
 template <int K>
-__device__ __forceinline__ void mlp_neuron4(const uint16_t *__restrict__ w, const uint16_t *bias, const float *in,
-                                            float *acc) {
-    const uint4 *w8 = reinterpret_cast<const uint4 *>(w);
+__device__ __forceinline__ void mlp_neuron4(GWeight *__restrict__ w, GWeight *bias, const LAct *in, float *acc) {
+    const AS_LDS vu4 *w8 = reinterpret_cast<const AS_LDS vu4 *>(w);
 #pragma unroll
     for (int r = 0; r < 4; ++r) acc[r] = bf16f(bias[32 * r]);
 #pragma unroll 2
     for (int k8 = 0; k8 < K / 8; ++k8) {
-        const float4 a = *reinterpret_cast<const float4 *>(in + 8 * k8);
-        const float4 b = *reinterpret_cast<const float4 *>(in + 8 * k8 + 4);
+        const vf4 a = *reinterpret_cast<const AS_LDS vf4 *>(in + 8 * k8);
+        const vf4 b = *reinterpret_cast<const AS_LDS vf4 *>(in + 8 * k8 + 4);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const uint4 q = w8[k8 * 32 + r * (K / 8) * 32];
+            const vu4 q = w8[k8 * 32 + r * (K / 8) * 32];
             float c = acc[r];
             c = __builtin_fmaf(bf16f(q.x & 0xffffu), a.x, c);
             c = __builtin_fmaf(bf16f(q.x >> 16), a.y, c);
@@ -1094,14 +1104,14 @@ __device__ __forceinline__ void mlp_neuron4(const uint16_t *__restrict__ w, cons
 
 // w: the neuron's row in the lane-interleaved layout (mlp_il_offset(K, j, 0)).
 template <int K>
-__device__ __forceinline__ float mlp_neuron(const uint16_t *__restrict__ w, uint16_t bias, const float *in) {
+__device__ __forceinline__ float mlp_neuron(GWeight *__restrict__ w, uint16_t bias, const LAct *in) {
     float acc = bf16f(bias);
-    const uint4 *w8 = reinterpret_cast<const uint4 *>(w);
+    const AS_LDS vu4 *w8 = reinterpret_cast<const AS_LDS vu4 *>(w);
 #pragma unroll 2
     for (int k8 = 0; k8 < K / 8; ++k8) {
-        const uint4 q = w8[k8 * 32];
-        const float4 a = *reinterpret_cast<const float4 *>(in + 8 * k8);
-        const float4 b = *reinterpret_cast<const float4 *>(in + 8 * k8 + 4);
+        const vu4 q = w8[k8 * 32];
+        const vf4 a = *reinterpret_cast<const AS_LDS vf4 *>(in + 8 * k8);
+        const vf4 b = *reinterpret_cast<const AS_LDS vf4 *>(in + 8 * k8 + 4);
         acc = __builtin_fmaf(bf16f(q.x & 0xffffu), a.x, acc);
         acc = __builtin_fmaf(bf16f(q.x >> 16), a.y, acc);
         acc = __builtin_fmaf(bf16f(q.y & 0xffffu), a.z, acc);
@@ -1114,8 +1124,12 @@ __device__ __forceinline__ float mlp_neuron(const uint16_t *__restrict__ w, uint
     return acc;
 }
 
-// Kept out of line so the tracking loop keeps its register budget.
-__device__ __noinline__ void mlp_forward(const uint16_t *__restrict__ W, float *act) {
+// Kept out of line so the tracking loop keeps its register budget.  Both
+// W_generic (the staged network) and act_generic (the slot's activations)
+// point into LDS.
+__device__ __noinline__ void mlp_forward(const uint16_t *__restrict__ W_generic, float *act_generic) {
+    GWeight *W = (GWeight *)W_generic;
+    LAct *act = (LAct *)act_generic;
     const int l = lane32();
     float v[4];
     mlp_neuron4<MLP_IN>(W + IL_W1 + mlp_il_offset(MLP_IN, l, 0), W + IL_B1 + l, act + ACT_X, v);
@@ -1891,6 +1905,21 @@ __device__ void track_seed(const DevParams &P, const DevBufs &B, const float *sv
     }
 }
 
+// The learned kernels (V == 1) copy the proposal network (IL_NPARAM bf16,
+// lane-interleaved) into the first MLP_LDS_FLOATS floats of `lds` and point
+// Bl.mlp there; every slot of the workgroup reads it from LDS.  Returns the
+// floats used.  The caller synchronises the workgroup before use.
+template <int V>
+__device__ __forceinline__ int stage_network(const DevBufs &B, float *lds, DevBufs &Bl) {
+    Bl = B;
+    if (V != 1) return 0;
+    const uint4 *src = reinterpret_cast<const uint4 *>(B.mlp);
+    uint4 *dst = reinterpret_cast<uint4 *>(lds);
+    for (int i = threadIdx.x; i < IL_NPARAM / 8; i += blockDim.x) dst[i] = src[i];
+    Bl.mlp = reinterpret_cast<const uint16_t *>(lds);
+    return MLP_LDS_FLOATS;
+}
+
 // One seed per 32-lane slot, static assignment.
 template <int MODEL, int V = 0>
 __global__ void __launch_bounds__(256) phase2_kernel(DevParams P, DevBufs B, const float *__restrict__ seeds,
@@ -1905,6 +1934,8 @@ __global__ void __launch_bounds__(256) phase2_kernel(DevParams P, DevBufs B, con
         sv = smem;
         svf = verts_floats(P);
     }
+    DevBufs BL;
+    svf += stage_network<V>(B, smem + svf, BL);
     __syncthreads();
     const int nslots = blockDim.x >> 5;
     const int slot = threadIdx.x >> 5;
@@ -1912,7 +1943,7 @@ __global__ void __launch_bounds__(256) phase2_kernel(DevParams P, DevBufs B, con
     if (s >= nseeds) return;
     SlotLds S;
     bind_slot<MODEL>(P, false, smem + svf + slot * slot_floats(MODEL, false, P.nv32, P.t32, P.hr32), S);
-    track_seed<MODEL, V>(P, B, sv, S, s, seeds, rng_chunk, offsets, dirs, sline_seed, sline_len, sline);
+    track_seed<MODEL, V>(P, BL, sv, S, s, seeds, rng_chunk, offsets, dirs, sline_seed, sline_len, sline);
 }
 
 // Persistent work queue: a fixed grid of slots; each 32-lane slot takes the
@@ -1925,7 +1956,8 @@ __global__ void __launch_bounds__(256) phase2_kernel(DevParams P, DevBufs B, con
 // than the latency hiding gained (+8% at 200k seeds).  Bootstrap likewise
 // (4 waves per SIMD, which its 5 KB of LDS per slot allows: +13% CSA).
 template <int MODEL, int V = 0>
-__global__ void __launch_bounds__(256, (MODEL == CUSLINES_PTT && V != 2) ? 4 : (MODEL <= CUSLINES_CSA ? 4 : 1))
+__global__ void __launch_bounds__((MODEL == CUSLINES_PTT && V == 1) ? 512 : 256,
+                                  (MODEL == CUSLINES_PTT && V != 2) ? 4 : (MODEL <= CUSLINES_CSA ? 4 : 1))
     phase2_queue_kernel(DevParams P, DevBufs B, const float *__restrict__ seeds,
                                                            int nseeds, int rng_chunk, const int *__restrict__ offsets,
                                                            const int *__restrict__ dirs, int *__restrict__ sline_seed,
@@ -1939,6 +1971,8 @@ __global__ void __launch_bounds__(256, (MODEL == CUSLINES_PTT && V != 2) ? 4 : (
         sv = smem;
         svf = verts_floats(P);
     }
+    DevBufs BL;
+    svf += stage_network<V>(B, smem + svf, BL);
     __syncthreads();
     const int slot = threadIdx.x >> 5;
     const int l = lane32();
@@ -1958,7 +1992,7 @@ __global__ void __launch_bounds__(256, (MODEL == CUSLINES_PTT && V != 2) ? 4 : (
             q = shfl32(q, 0);
             if (lo + q >= hi) break;
             const int s = order ? order[lo + q] : lo + q;
-            track_seed<MODEL, V>(P, B, sv, S, s, seeds, rng_chunk, offsets, dirs, sline_seed, sline_len, sline);
+            track_seed<MODEL, V>(P, BL, sv, S, s, seeds, rng_chunk, offsets, dirs, sline_seed, sline_len, sline);
         }
     }
 }
@@ -2454,8 +2488,12 @@ static int launch_phase2_v(cuslines_tracker *t, const float *seeds, int n, int c
     const DevParams &d = t->dp;
     int threads = t->threads;
     if (threads <= 0) threads = (MODEL == CUSLINES_OPDT || MODEL == CUSLINES_CSA) ? 128 : 256;
+    // learned: 512-thread workgroups of the queue kernel share one staged
+    // copy of the network (50 KB of LDS) among 16 slots
+    if (V == 1 && t->threads <= 0 && t->schedule != CUSLINES_SCHED_STATIC) threads = 512;
     const int nslots = threads / 32;
-    const size_t vf = (MODEL == CUSLINES_PROB) ? (size_t)((3 * d.nverts + 3) & ~3) : 0;
+    const size_t vf = ((MODEL == CUSLINES_PROB) ? (size_t)((3 * d.nverts + 3) & ~3) : 0) +
+                      ((V == 1) ? (size_t)MLP_LDS_FLOATS : 0);
     const size_t lds = sizeof(float) * (vf + (size_t)nslots * slot_floats(MODEL, false, d.nv32, d.t32, d.hr32));
     if (lds > 160 * 1024) return fail(CUSLINES_EINVAL, "sphere too large for LDS");
     if (t->schedule == CUSLINES_SCHED_STATIC) {
