@@ -103,34 +103,40 @@ def cpu_baseline(w, seeds, n_sample):
                        f"streamlines) in {dt:.1f} s")
 
 
-DENSITY_SLINES = 20000
 
 
-def _density(slots, lens, shape):
-    """Voxel visit counts of the points of the given padded slots."""
-    pts = np.concatenate([slots[i, :n] for i, n in enumerate(lens)])
-    idx = np.clip(np.rint(pts).astype(np.int64), 0, np.array(shape) - 1)
-    d = np.zeros(shape)
-    np.add.at(d, tuple(idx.T), 1)
-    return d.ravel()
+def _density(dt, sld, sl, nsl, lens, shape):
+    """Voxel visit counts of every point of the pass held in the padded slots
+    (compacted on the device, then counted on the host)."""
+    npts = int(lens.sum())
+    pod = dt.alloc(8 * (nsl + 1))
+    ptd = dt.alloc(12 * max(npts, 1))
+    try:
+        dt.compact_async(sld.ptr, nsl, sl.ptr, pod.ptr, ptd.ptr)
+        dt.synchronize()
+        pts = ptd.download(np.zeros((npts, 3), np.float32))
+    finally:
+        pod.free()
+        ptd.free()
+    shp = np.array(shape)
+    idx = np.clip(np.rint(pts).astype(np.int64), 0, shp - 1)
+    lin = (idx[:, 0] * shp[1] + idx[:, 1]) * shp[2] + idx[:, 2]
+    return np.bincount(lin, minlength=int(np.prod(shp))).astype(np.float64)
 
 
 def measure_learned(args, w, dt, step, plain_lens, sld, sl, nsl, L):
     """Train the BF16 proposal network on the GPU from pairs the plain getter
     records (20,000 seeds in the mask, 5% of its sub-steps, 4 epochs of Adam),
     then time the same pass with the learned proposal (no warmup pass of its
-    own).  Compares the streamline lengths (two-sample KS) and, on the first
-    20,000 streamlines, the voxel visit density with the plain pass."""
+    own).  Compares the streamline lengths (two-sample KS) and the voxel
+    visit density of all points with the plain pass."""
     from scipy.stats import ks_2samp
 
     from gpustreamlines_amd import learned as LP
     from gpustreamlines_amd import workloads as W
 
     shape = tuple(w.params[k] for k in ("dimx", "dimy", "dimz"))
-    k = min(DENSITY_SLINES, nsl)
-    plain_slots = sl.download(np.zeros((k, 2 * L, 3), np.float32))
-    d_plain = _density(plain_slots, plain_lens[:k], shape)
-    del plain_slots
+    d_plain = _density(dt, sld, sl, nsl, plain_lens, shape)
     t0 = time.perf_counter()
     pairs, npairs = LP.record_pairs(dt, W.seeds_for(w, 20000, placement="mask", seed=7), capacity=3_000_000,
                                     rate=0.05, rng_chunk=CHUNK)
@@ -153,9 +159,7 @@ def measure_learned(args, w, dt, step, plain_lens, sld, sl, nsl, L):
     stats = dt.proposal_stats(reset=True) // max(1, args.steps)
     dt.set_proposal(None)
     lens = sld.download(np.zeros(nsl, dtype=np.int32)).astype(np.int64)
-    slots = sl.download(np.zeros((k, 2 * L, 3), np.float32))
-    d_learned = _density(slots, lens[:k], shape)
-    del slots
+    d_learned = _density(dt, sld, sl, nsl, lens, shape)
     t_step = float(np.mean([t[3] for t in times])) / 1e3
     steps = int((lens - 1).clip(min=0).sum())
     ks = ks_2samp(plain_lens, lens)
@@ -172,7 +176,7 @@ def measure_learned(args, w, dt, step, plain_lens, sld, sl, nsl, L):
         "mean_length_learned": float(lens.mean()),
         "ks_length_statistic": float(ks.statistic),
         "ks_length_p": float(ks.pvalue),
-        "density_corr_first_20k": float(np.corrcoef(d_plain, d_learned)[0, 1]),
+        "density_corr": float(np.corrcoef(d_plain, d_learned)[0, 1]),
         "train_pairs": int(npairs),
         "train_loss": [float(loss0)] + [float(h) for h in hist],
         "train_s": t_train,

@@ -140,41 +140,66 @@ def test_training_lowers_loss_and_raises_acceptance(small_workloads, trained):
     assert share["untrained"] < 0.5
 
 
-@pytest.mark.xfail(strict=False, reason="the learned proposal shortens streamlines: on this workload mean "
-                   "length 52 vs 68 points, KS p ~ 1e-91, density r ~ 0.95 (bar: KS p > 0.05, r > 0.98)")
-def test_learned_statistics_match_plain(small_workloads, trained):
-    """Same seeds, plain getter vs learned proposal: streamline count, the
-    length distribution (two-sample KS) and voxel visit density, against the
-    bar of p > 0.05 and r > 0.98."""
-    from scipy.stats import ks_2samp
+def _density(r, shape):
+    idx = np.clip(np.rint(r["points"]).astype(int), 0, np.array(shape) - 1)
+    d = np.zeros(shape)
+    np.add.at(d, tuple(idx.T), 1)
+    return d.ravel()
+
+
+@pytest.fixture(scope="module")
+def stats_runs(small_workloads, trained):
+    """Same seeds: the plain getter, the plain getter under another rng_seed
+    (the noise floor of every statistic), and the learned proposal."""
+    import dataclasses
 
     from gpustreamlines_amd import learned as LP
     from gpustreamlines_amd import workloads as W
 
     w = small_workloads["ptt"]
-    seeds = W.seeds_for(w, 3000, placement="mixed", seed=13)
+    seeds = W.seeds_for(w, 30000, placement="mixed", seed=13)
     res = {}
     dt = _tracker(w)
     try:
-        for name, p in (("plain", None), ("learned", trained["params"])):
-            dt.set_proposal(None if p is None else LP.to_bf16(p))
-            res[name] = dt.track(seeds)
+        res["plain"] = dt.track(seeds)
+        dt.set_proposal(LP.to_bf16(trained["params"]))
+        res["learned"] = dt.track(seeds)
     finally:
         dt.close()
-    a, b = res["plain"], res["learned"]
+    dt = _tracker(dataclasses.replace(w, params=dict(w.params, rng_seed=int(w.params["rng_seed"]) + 1)))
+    try:
+        res["plain_other_seed"] = dt.track(seeds)
+    finally:
+        dt.close()
+    return res
+
+
+def test_learned_count_and_density_match_plain(small_workloads, stats_runs):
+    """Streamline count within 2 % and voxel visit density of all points
+    correlated > 0.98 with the plain getter's (the plain getter's own
+    correlation under another random seed is printed beside it)."""
+    shape = tuple(small_workloads["ptt"].params[k] for k in ("dimx", "dimy", "dimz"))
+    a, b, c = stats_runs["plain"], stats_runs["learned"], stats_runs["plain_other_seed"]
     na, nb = len(a["sline_len"]), len(b["sline_len"])
+    da = _density(a, shape)
+    r_learned = np.corrcoef(da, _density(b, shape))[0, 1]
+    r_floor = np.corrcoef(da, _density(c, shape))[0, 1]
+    print(f"[learned] streamlines {na} vs {nb}; density r learned {r_learned:.5f}, plain under another seed "
+          f"{r_floor:.5f}", flush=True)
     assert abs(na - nb) <= 0.02 * na
+    assert r_learned > 0.98
+
+
+@pytest.mark.xfail(strict=False, reason="the learned proposal shortens streamlines (1M seeds: mean 108 vs 120 "
+                   "points, KS statistic 0.044 against 0.0007 between two plain seeds); bar: KS p > 0.05")
+def test_learned_length_distribution_matches_plain(stats_runs):
+    """Two-sample KS of the streamline lengths, plain vs learned, against
+    the bar of p > 0.05 (plain vs plain under another seed printed beside)."""
+    from scipy.stats import ks_2samp
+
+    a, b, c = stats_runs["plain"], stats_runs["learned"], stats_runs["plain_other_seed"]
     ks = ks_2samp(a["sline_len"], b["sline_len"])
-    shape = tuple(w.params[k] for k in ("dimx", "dimy", "dimz"))
-
-    def density(r):
-        idx = np.clip(np.rint(r["points"]).astype(int), 0, np.array(shape) - 1)
-        d = np.zeros(shape)
-        np.add.at(d, tuple(idx.T), 1)
-        return d.ravel()
-
-    corr = np.corrcoef(density(a), density(b))[0, 1]
-    print(f"[learned] streamlines {na} vs {nb}, mean length {a['sline_len'].mean():.1f} vs "
-          f"{b['sline_len'].mean():.1f}, KS p={ks.pvalue:.3g}, density r={corr:.4f}", flush=True)
-    assert corr > 0.98
+    ks0 = ks_2samp(a["sline_len"], c["sline_len"])
+    print(f"[learned] mean length {a['sline_len'].mean():.1f} vs {b['sline_len'].mean():.1f}; KS p={ks.pvalue:.3g} "
+          f"(plain vs plain under another seed: p={ks0.pvalue:.3g})", flush=True)
     assert ks.pvalue > 0.05
