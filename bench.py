@@ -126,7 +126,8 @@ def _density(dt, sld, sl, nsl, lens, shape):
 
 def measure_learned(args, w, dt, step, plain_lens, sld, sl, nsl, L):
     """Train the BF16 proposal network on the GPU from pairs the plain getter
-    records (20,000 seeds in the mask, 5% of its sub-steps, 4 epochs of Adam),
+    records (100,000 seeds in the mask, 6% of its sub-steps: 20 M pairs,
+    4 epochs of Adam),
     then time the same pass with the learned proposal (no warmup pass of its
     own).  Compares the streamline lengths (two-sample KS) and the voxel
     visit density of all points with the plain pass."""
@@ -138,8 +139,8 @@ def measure_learned(args, w, dt, step, plain_lens, sld, sl, nsl, L):
     shape = tuple(w.params[k] for k in ("dimx", "dimy", "dimz"))
     d_plain = _density(dt, sld, sl, nsl, plain_lens, shape)
     t0 = time.perf_counter()
-    pairs, npairs = LP.record_pairs(dt, W.seeds_for(w, 20000, placement="mask", seed=7), capacity=3_000_000,
-                                    rate=0.05, rng_chunk=CHUNK)
+    pairs, npairs = LP.record_pairs(dt, W.seeds_for(w, 100000, placement="mask", seed=7), capacity=20_000_000,
+                                    rate=0.06, rng_chunk=CHUNK)
     tr = LP.Trainer(dt, batch=4096)
     loss0 = tr.loss_grad(pairs.ptr, 4096)[0]
     hist = tr.fit(pairs, npairs, epochs=4)

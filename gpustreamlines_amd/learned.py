@@ -131,7 +131,7 @@ class Trainer:
             b.free()
 
 
-def train(dt, seeds: np.ndarray, rate: float = 0.05, capacity: int = 3_000_000, epochs: int = 4,
+def train(dt, seeds: np.ndarray, rate: float = 0.06, capacity: int = 20_000_000, epochs: int = 4,
           batch: int = 4096, rng_chunk: int = 25000, verbose: bool = False):
     """Record pairs with the plain getter on ``seeds`` and train a network on
     the GPU.  Returns (float32 parameters, info dict)."""
