@@ -806,14 +806,21 @@ __device__ __forceinline__ void probing_frame(const float *frame, Philox &rng, f
 // The frames of successive probes do not depend on the sampled values, so
 // probe i+1's frame and LUT read are issued before probe i's block is
 // consumed: the LUT latency of one tap overlaps the block read of the last.
+// With ALLOW_WEAK_LINK (ptt.cuh:8) the support is first_val plus the taps,
+// all >= 0 (the ODF is clipped at 0 and linear filtering keeps the sign),
+// so the running sum never decreases: once it reaches `stop_at` the
+// candidate passes whatever the remaining taps add, and the lane stops
+// probing.  Callers that need the value itself pass +infinity.
 __device__ __forceinline__ float data_support(const DevParams &P, const DevBufs &B, float support,
-                                              float px, float py, float pz, float k1, float k2, float *pf) {
+                                              float px, float py, float pz, float k1, float k2, float *pf,
+                                              float stop_at) {
     float prop[9], direc[3];
     prepare_propagator(k1, k2, P.probe_step, prop);
     propagate_frame(prop, pf, direc);
     px += direc[0]; py += direc[1]; pz += direc[2];
     uint32_t idx = B.lut[lut_index(P, pf[0], pf[1], pf[2])];
     for (int i = 0; i < P.probe_quality; ++i) {
+        if (support >= stop_at) break;
         const float qx = px, qy = py, qz = pz;
         const uint32_t qidx = idx;
         if (i + 1 < P.probe_quality) {
@@ -907,7 +914,9 @@ __device__ int ptt_substep(const DevParams &P, const DevBufs &B, PttLds &S, floa
         float sup = 0.0f;
         if (active) {
             probing_frame<INIT>(fr, rng, pf);
-            sup = data_support(P, B, first_val, px, py, pz, k1, k2, pf);
+            // rejection rounds of a non-initial sub-step only test sup >= min_support_f
+            const float stop_at = (ph >= 1 && !INIT) ? P.min_support_f : __builtin_inff();
+            sup = data_support(P, B, first_val, px, py, pz, k1, k2, pf, stop_at);
         }
         if (ph == 0) {
             const bool ok = active && !(sup < P.min_support_f);
