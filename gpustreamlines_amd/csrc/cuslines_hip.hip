@@ -104,6 +104,7 @@ struct DevParams {
     float probe_step, max_curv, min_support_f, ptt_seed_thr_f;
     float ptt_sub_step;  // (float)(STEP_SIZE / STEP_FRAC)
     int lut_brick;               // PTT LUT stored as 4x4x4 bricks
+    int ptt_nonneg;              // PTT ODF has no negative value: rejection rounds may stop early
     uint32_t rec_thresh;  // record a sub-step when its hash is below this
     int rec_cap;          // capacity of the pair buffer
 };
@@ -915,7 +916,7 @@ __device__ int ptt_substep(const DevParams &P, const DevBufs &B, PttLds &S, floa
         if (active) {
             probing_frame<INIT>(fr, rng, pf);
             // rejection rounds of a non-initial sub-step only test sup >= min_support_f
-            const float stop_at = (ph >= 1 && !INIT) ? P.min_support_f : __builtin_inff();
+            const float stop_at = (ph >= 1 && !INIT && P.ptt_nonneg) ? P.min_support_f : __builtin_inff();
             sup = data_support(P, B, first_val, px, py, pz, k1, k2, pf, stop_at);
         }
         if (ph == 0) {
@@ -2721,6 +2722,10 @@ int cuslines_set_data(cuslines_tracker *t, const float *dataf) {
     t->d_data = nullptr;
     t->has_data = false;
     if (t->p.model_type == CUSLINES_PTT) {
+        // The early stop of rejection-round probes (data_support) relies on
+        // taps >= 0; the reference's prepare_data clips the ODF at 0.
+        const size_t nval = nvox * (size_t)d.dimt;
+        t->dp.ptt_nonneg = std::none_of(dataf, dataf + nval, [](float v) { return v < 0.0f; }) ? 1 : 0;
         float *tmp = nullptr;
         const size_t bytes = nvox * d.dimt * sizeof(float);
         HIPC(hipMalloc(&tmp, bytes));

@@ -172,3 +172,22 @@ def test_peaks_edge_path_bit_exact(small_workloads, kind):
     g, o = _run_both(w2, seeds)
     assert len(o["sline_len"]) > 0
     _assert_same(g, o)
+
+
+def test_ptt_with_negative_odf_values_bit_exact(small_workloads):
+    """ODF data with negative values (not prepared by PttDirectionGetter,
+    which clips at 0): the library then measures every candidate's full
+    support instead of stopping at the acceptance threshold, and must still
+    match the oracle."""
+    import dataclasses
+
+    from gpustreamlines_amd import workloads as W
+
+    w = small_workloads["ptt"]
+    data = w.dataf.copy()
+    data[data == 0] = -0.02
+    w2 = dataclasses.replace(w, dataf=data)
+    seeds = W.seeds_for(w2, NSEEDS["ptt"], placement="mixed")
+    g, o = _run_both(w2, seeds)
+    assert len(o["sline_len"]) > 0
+    _assert_same(g, o)

@@ -136,3 +136,26 @@ def test_propagator_arc_geometry():
         assert np.allclose(F @ F.T, np.eye(3), atol=2e-6)
         assert np.allclose(np.cross(F[0], F[1]), F[2], atol=2e-6)
         assert np.linalg.norm(d) <= 0.25 + 1e-6
+
+
+def test_ptt_taps_are_non_negative(small_workloads):
+    """The premise of the early stop of rejection-round probes: on the
+    prepared (clipped) ODF every tap, inside the volume, in the border cells
+    and beyond them (clamp and tile bleed), is >= 0, so a candidate's running
+    support never decreases (ptt.cu:156-192 with ALLOW_WEAK_LINK, ptt.cuh:8)."""
+    w = small_workloads["ptt"]
+    o = O.Oracle(w.params, w.dataf, w.stop_map, w.vertices, w.edges, ptt_lut=w.ptt_lut)
+    dims = np.array([w.params[k] for k in ("dimx", "dimy", "dimz")], np.float64)
+    rng = np.random.default_rng(4)
+    assert w.dataf.min() >= 0
+    vals = []
+    for _ in range(3000):
+        pos = rng.uniform(-3.0, dims + 2.0).astype(np.float32)
+        f = rng.normal(size=3)
+        f /= np.linalg.norm(f)
+        frame = np.zeros(9, np.float32)
+        frame[:3] = f
+        vals.append(o.interp4(pos, frame))
+    vals = np.array(vals)
+    assert vals.min() >= 0.0
+    assert (vals > 0).mean() > 0.1
