@@ -518,22 +518,39 @@ __device__ void gemv_lanerow(int N, int M, const float *x, const float *__restri
 #pragma unroll
             for (int q = 0; q < 32; ++q) p[q] = 0.0f;
             for (int j0 = 0; j0 < M; j0 += 32) {
+                // All loads of the block are issued before the first use: a
+                // column past M re-reads the block's last valid one, and its
+                // product is dropped by the select, so no load sits behind a
+                // branch (and none leaves the matrix).
+                const int nj = min(32, M - j0);
                 if (vec) {  // M % 4 == 0: a vector is all in or all out
+                    const int nq = nj >> 2;
+                    float4 m[8];
 #pragma unroll
-                    for (int q = 0; q < 32; q += 4) {
-                        if (j0 + q < M) {
-                            const float4 m = *reinterpret_cast<const float4 *>(row + ((size_t)(j0 + q) << 5));
-                            const float4 v = *reinterpret_cast<const float4 *>(x + j0 + q);
-                            p[q] = __builtin_fmaf(v.x, m.x, p[q]);
-                            p[q + 1] = __builtin_fmaf(v.y, m.y, p[q + 1]);
-                            p[q + 2] = __builtin_fmaf(v.z, m.z, p[q + 2]);
-                            p[q + 3] = __builtin_fmaf(v.w, m.w, p[q + 3]);
-                        }
+                    for (int q = 0; q < 8; ++q)
+                        m[q] = *reinterpret_cast<const float4 *>(row + ((size_t)(j0 + 4 * min(q, nq - 1)) << 5));
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        const float4 v = *reinterpret_cast<const float4 *>(x + j0 + 4 * q);
+                        const bool in = q < nq;
+                        const float a0 = __builtin_fmaf(v.x, m[q].x, p[4 * q]);
+                        const float a1 = __builtin_fmaf(v.y, m[q].y, p[4 * q + 1]);
+                        const float a2 = __builtin_fmaf(v.z, m[q].z, p[4 * q + 2]);
+                        const float a3 = __builtin_fmaf(v.w, m[q].w, p[4 * q + 3]);
+                        p[4 * q] = in ? a0 : p[4 * q];
+                        p[4 * q + 1] = in ? a1 : p[4 * q + 1];
+                        p[4 * q + 2] = in ? a2 : p[4 * q + 2];
+                        p[4 * q + 3] = in ? a3 : p[4 * q + 3];
                     }
                 } else {
+                    float m[32];
 #pragma unroll
-                    for (int q = 0; q < 32; ++q)
-                        if (j0 + q < M) p[q] = __builtin_fmaf(x[j0 + q], row[j0 + q], p[q]);
+                    for (int q = 0; q < 32; ++q) m[q] = row[j0 + min(q, nj - 1)];
+#pragma unroll
+                    for (int q = 0; q < 32; ++q) {
+                        const float a = __builtin_fmaf(x[j0 + q], m[q], p[q]);
+                        p[q] = (q < nj) ? a : p[q];
+                    }
                 }
             }
 #pragma unroll
