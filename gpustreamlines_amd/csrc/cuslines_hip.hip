@@ -296,7 +296,10 @@ __device__ int sort_dedup_peaks(const DevParams &P, const float *odf, uint16_t *
 // -------------------------------------------------------------------------
 
 // trilinear_interp_d (tracking_helpers.cu:3-61) over `dimt` channels with
-// channel stride `ld`; writes out[0..dimt).
+// channel stride `ld`; writes out[0..dimt).  UNROLL channel iterations are
+// in flight together (8 corner loads each): the probabilistic getter, whose
+// step is this 23 KB gather, uses 3.
+template <int UNROLL = 1>
 __device__ int trilinear(const DevParams &P, const float *__restrict__ data, int ld, int dimt,
                          float px, float py, float pz, float *out) {
     if (px < -0.5f || px + 0.5f >= (float)P.dimx || py < -0.5f || py + 0.5f >= (float)P.dimy ||
@@ -317,6 +320,7 @@ __device__ int trilinear(const DevParams &P, const float *__restrict__ data, int
         data + (((size_t)x1 * DY + y0) * DZ + z0) * ld, data + (((size_t)x1 * DY + y0) * DZ + z1) * ld,
         data + (((size_t)x1 * DY + y1) * DZ + z0) * ld, data + (((size_t)x1 * DY + y1) * DZ + z1) * ld};
     const int l = lane32();
+#pragma unroll UNROLL
     for (int t = l; t < dimt; t += 32) {
         float v[8];
 #pragma unroll
@@ -579,7 +583,7 @@ __device__ int get_direction_prob(const DevParams &P, const DevBufs &B, const fl
     const int l = lane32();
     const int nv = P.nverts;
     float *pmf = S.pmf;
-    if (trilinear(P, B.data, P.t32, P.dimt, px, py, pz, pmf) != 0) return 0;
+    if (trilinear<3>(P, B.data, P.t32, P.dimt, px, py, pz, pmf) != 0) return 0;
     WAVE_SYNC();
     float mx = REAL_MIN;
     for (int i = l; i < nv; i += 32) mx = (mx > pmf[i]) ? mx : pmf[i];
@@ -1984,7 +1988,7 @@ __global__ void __launch_bounds__(256) phase2_kernel(DevParams P, DevBufs B, con
 // (4 waves per SIMD, which its 5 KB of LDS per slot allows: +13% CSA).
 template <int MODEL, int V = 0>
 __global__ void __launch_bounds__((MODEL == CUSLINES_PTT && V == 1) ? 512 : 256,
-                                  (MODEL == CUSLINES_PTT && V != 2) ? 4 : (MODEL <= CUSLINES_CSA ? 4 : 1))
+                                  (MODEL == CUSLINES_PTT && V != 2) ? 4 : (MODEL <= CUSLINES_PROB ? 4 : 1))
     phase2_queue_kernel(DevParams P, DevBufs B, const float *__restrict__ seeds,
                                                            int nseeds, int rng_chunk, const int *__restrict__ offsets,
                                                            const int *__restrict__ dirs, int *__restrict__ sline_seed,
