@@ -481,6 +481,73 @@ __device__ __forceinline__ float interp4(const DevParams &P, const DevBufs &B, f
     return tap_sample(P, B, B.lut[lut_index(P, f0, f1, f2)], px, py, pz);
 }
 
+// tap_sample in two halves, so that a probe loop can issue one tap's block
+// read, do other work, and blend afterwards: tap_begin computes the weights
+// and issues the read (fast = the cell has a corner block), tap_end blends
+// (or gathers the 8 texels when there is no block).
+struct Tap {
+    float a, b, g;
+    float4 lo, hi;
+    float xq, yq, zq;
+    uint32_t lv;
+    bool fast;
+};
+
+__device__ __forceinline__ Tap tap_begin(const DevParams &P, const DevBufs &B, uint32_t lv, float px, float py,
+                                         float pz) {
+    Tap t;
+    t.lv = lv;
+    const int idx = (int)(lv & 0xffffu);
+    const int tx = idx & (P.nx - 1);
+    const int ty = (idx >> P.log2x) & (P.ny - 1);
+    const int tz = idx >> (P.log2x + P.log2y);
+    const int ox = (int)__umul24((uint32_t)tx, (uint32_t)P.dimx);
+    const int oy = (int)__umul24((uint32_t)ty, (uint32_t)P.dimy);
+    const int oz = (int)__umul24((uint32_t)tz, (uint32_t)P.dimz);
+    t.xq = (float)ox + px;
+    t.yq = (float)oy + py;
+    t.zq = (float)oz + pz;
+    t.fast = false;
+    if (B.blocks) {
+        int i, j, k;
+        t.a = tex_weight(t.xq, &i);
+        t.b = tex_weight(t.yq, &j);
+        t.g = tex_weight(t.zq, &k);
+        const int cx = i - ox, cy = j - oy, cz = k - oz;
+        if ((unsigned)(cx + 1) <= (unsigned)P.dimx && (unsigned)(cy + 1) <= (unsigned)P.dimy &&
+            (unsigned)(cz + 1) <= (unsigned)P.dimz) {
+            const uint32_t inner =
+                __umul24(__umul24((uint32_t)(cx + 1), (uint32_t)(P.dimy + 1)) + (uint32_t)(cy + 1),
+                         (uint32_t)(P.dimz + 1)) + (uint32_t)(cz + 1);
+            const uint64_t cell = (uint64_t)inner * (uint32_t)P.nverts + (lv >> 16);
+            const float4 *blk = reinterpret_cast<const float4 *>(B.blocks + cell * 8);
+            t.lo = blk[0];
+            t.hi = blk[1];
+            t.fast = true;
+        }
+    }
+    return t;
+}
+
+__device__ __forceinline__ float tap_end(const DevParams &P, const DevBufs &B, const Tap &t) {
+    if (t.fast) {
+        const float a = t.a, b = t.b, g = t.g;
+        const float a0 = 1.0f - a, b0 = 1.0f - b, g0 = 1.0f - g;
+        float acc = ((a0 * b0) * g0) * t.lo.x;
+        acc = __builtin_fmaf((a * b0) * g0, t.lo.y, acc);
+        acc = __builtin_fmaf((a0 * b) * g0, t.lo.z, acc);
+        acc = __builtin_fmaf((a * b) * g0, t.lo.w, acc);
+        acc = __builtin_fmaf((a0 * b0) * g, t.hi.x, acc);
+        acc = __builtin_fmaf((a * b0) * g, t.hi.y, acc);
+        acc = __builtin_fmaf((a0 * b) * g, t.hi.z, acc);
+        acc = __builtin_fmaf((a * b) * g, t.hi.w, acc);
+        return acc;
+    }
+    const int idx = (int)(t.lv & 0xffffu);
+    return tiled_sample(P, B.data, t.xq, t.yq, t.zq, idx & (P.nx - 1), (idx >> P.log2x) & (P.ny - 1),
+                        idx >> (P.log2x + P.log2y));
+}
+
 // -------------------------------------------------------------------------
 // Half-wave matrix-vector products with the reference's per-row summation
 // (ndotp_d, boot.cu:195-235): per row, 32 lane partials over j = l, l+32, ...
@@ -826,8 +893,9 @@ __device__ __forceinline__ void probing_frame(const float *frame, Philox &rng, f
 
 // calculate_data_support_d (ptt.cu:156-192)
 // The frames of successive probes do not depend on the sampled values, so
-// probe i+1's frame and LUT read are issued before probe i's block is
-// consumed: the LUT latency of one tap overlaps the block read of the last.
+// probe i's block read is issued, then probe i+1's frame is propagated and
+// its LUT read issued, and only then is probe i's block blended: each load's
+// latency overlaps the other probe's work.
 // With ALLOW_WEAK_LINK (ptt.cuh:8) the support is first_val plus the taps,
 // all >= 0 (the ODF is clipped at 0 and linear filtering keeps the sign),
 // so the running sum never decreases: once it reaches `stop_at` the
@@ -843,14 +911,13 @@ __device__ __forceinline__ float data_support(const DevParams &P, const DevBufs 
     uint32_t idx = B.lut[lut_index(P, pf[0], pf[1], pf[2])];
     for (int i = 0; i < P.probe_quality; ++i) {
         if (support >= stop_at) break;
-        const float qx = px, qy = py, qz = pz;
-        const uint32_t qidx = idx;
+        const Tap t = tap_begin(P, B, idx, px, py, pz);
         if (i + 1 < P.probe_quality) {
             propagate_frame(prop, pf, direc);
             px += direc[0]; py += direc[1]; pz += direc[2];
             idx = B.lut[lut_index(P, pf[0], pf[1], pf[2])];
         }
-        support += tap_sample(P, B, qidx, qx, qy, qz);
+        support += tap_end(P, B, t);
     }
     return support;
 }
