@@ -437,54 +437,11 @@ __device__ __forceinline__ uint32_t lut_index(const DevParams &P, float f0, floa
 // tile's clamp already applied), else the 8-texel gather.  Blocks are stored
 // cell-major, the vertices of a cell in the rank order, so one 128-byte line
 // holds a cell's blocks for four neighbouring directions.
-__device__ __forceinline__ float tap_sample(const DevParams &P, const DevBufs &B, uint32_t lv, float px, float py,
-                                            float pz) {
-    const int idx = (int)(lv & 0xffffu);
-    const int tx = idx & (P.nx - 1);
-    const int ty = (idx >> P.log2x) & (P.ny - 1);
-    const int tz = idx >> (P.log2x + P.log2y);
-    const int ox = (int)__umul24((uint32_t)tx, (uint32_t)P.dimx);
-    const int oy = (int)__umul24((uint32_t)ty, (uint32_t)P.dimy);
-    const int oz = (int)__umul24((uint32_t)tz, (uint32_t)P.dimz);
-    const float xq = (float)ox + px;
-    const float yq = (float)oy + py;
-    const float zq = (float)oz + pz;
-    if (B.blocks) {
-        int i, j, k;
-        const float a = tex_weight(xq, &i), b = tex_weight(yq, &j), g = tex_weight(zq, &k);
-        const int cx = i - ox, cy = j - oy, cz = k - oz;
-        if ((unsigned)(cx + 1) <= (unsigned)P.dimx && (unsigned)(cy + 1) <= (unsigned)P.dimy &&
-            (unsigned)(cz + 1) <= (unsigned)P.dimz) {
-            const uint32_t inner =
-                __umul24(__umul24((uint32_t)(cx + 1), (uint32_t)(P.dimy + 1)) + (uint32_t)(cy + 1),
-                         (uint32_t)(P.dimz + 1)) + (uint32_t)(cz + 1);
-            const uint64_t cell = (uint64_t)inner * (uint32_t)P.nverts + (lv >> 16);
-            const float4 *blk = reinterpret_cast<const float4 *>(B.blocks + cell * 8);
-            const float4 lo = blk[0], hi = blk[1];
-            const float a0 = 1.0f - a, b0 = 1.0f - b, g0 = 1.0f - g;
-            float acc = ((a0 * b0) * g0) * lo.x;
-            acc = __builtin_fmaf((a * b0) * g0, lo.y, acc);
-            acc = __builtin_fmaf((a0 * b) * g0, lo.z, acc);
-            acc = __builtin_fmaf((a * b) * g0, lo.w, acc);
-            acc = __builtin_fmaf((a0 * b0) * g, hi.x, acc);
-            acc = __builtin_fmaf((a * b0) * g, hi.y, acc);
-            acc = __builtin_fmaf((a0 * b) * g, hi.z, acc);
-            acc = __builtin_fmaf((a * b) * g, hi.w, acc);
-            return acc;
-        }
-    }
-    return tiled_sample(P, B.data, xq, yq, zq, tx, ty, tz);
-}
-
-__device__ __forceinline__ float interp4(const DevParams &P, const DevBufs &B, float px, float py,
-                                         float pz, float f0, float f1, float f2) {
-    return tap_sample(P, B, B.lut[lut_index(P, f0, f1, f2)], px, py, pz);
-}
-
-// tap_sample in two halves, so that a probe loop can issue one tap's block
-// read, do other work, and blend afterwards: tap_begin computes the weights
-// and issues the read (fast = the cell has a corner block), tap_end blends
-// (or gathers the 8 texels when there is no block).
+//
+// In two halves, so that a probe loop can issue one tap's block read, do
+// other work and blend afterwards: tap_begin computes the weights and issues
+// the read (fast = the cell has a corner block), tap_end blends (or gathers
+// the 8 texels when there is no block).
 struct Tap {
     float a, b, g;
     float4 lo, hi;
@@ -547,6 +504,17 @@ __device__ __forceinline__ float tap_end(const DevParams &P, const DevBufs &B, c
     return tiled_sample(P, B.data, t.xq, t.yq, t.zq, idx & (P.nx - 1), (idx >> P.log2x) & (P.ny - 1),
                         idx >> (P.log2x + P.log2y));
 }
+
+__device__ __forceinline__ float tap_sample(const DevParams &P, const DevBufs &B, uint32_t lv, float px, float py,
+                                            float pz) {
+    return tap_end(P, B, tap_begin(P, B, lv, px, py, pz));
+}
+
+__device__ __forceinline__ float interp4(const DevParams &P, const DevBufs &B, float px, float py,
+                                         float pz, float f0, float f1, float f2) {
+    return tap_sample(P, B, B.lut[lut_index(P, f0, f1, f2)], px, py, pz);
+}
+
 
 // -------------------------------------------------------------------------
 // Half-wave matrix-vector products with the reference's per-row summation
