@@ -1494,10 +1494,12 @@ __device__ __forceinline__ void probe_frame_init(const float *frame, VLane &vl, 
     crossnorm3<2>(pf + 6, pf, pf + 3);
 }
 
-// Support of one candidate with a precomputed propagator.
+// Support of one candidate with a precomputed propagator; probing stops
+// once the running support reaches `stop_at` (see data_support).
 __device__ __forceinline__ float support_prop(const DevParams &P, const DevBufs &B, float support, float px,
-                                              float py, float pz, const float *prop, float *pf) {
+                                              float py, float pz, const float *prop, float *pf, float stop_at) {
     for (int i = 0; i < P.probe_quality; ++i) {
+        if (support >= stop_at) break;
         float direc[3];
         propagate_frame(prop, pf, direc);
         px += direc[0]; py += direc[1]; pz += direc[2];
@@ -1533,7 +1535,7 @@ __device__ int get_direction_ptt_g(const DevParams &P, const DevBufs &B, const P
 #pragma unroll
             for (int i = 0; i < 9; ++i) pf[i] = fr[i];
         }
-        const float sup = support_prop(P, B, first_val, px, py, pz, K.vprop + 9 * v, pf);
+        const float sup = support_prop(P, B, first_val, px, py, pz, K.vprop + 9 * v, pf, __builtin_inff());
         const bool ok = !(sup < P.min_support_f);
         S.vert_pdf[v] = ok ? sup : 0.0f;
         found |= ok;
@@ -1591,7 +1593,8 @@ __device__ int get_direction_ptt_g(const DevParams &P, const DevBufs &B, const P
             }
             float prop[9];
             prepare_propagator(k1, k2, P.probe_step, prop);
-            const float sup = support_prop(P, B, first_val, px, py, pz, prop, pf);
+            const float sup = support_prop(P, B, first_val, px, py, pz, prop, pf,
+                                           (!INIT && P.ptt_nonneg) ? P.min_support_f : __builtin_inff());
             if (INIT) {
                 // batches visit this lane's virtual lanes in increasing order, so
                 // keeping the first of equal supports keeps the lowest one
