@@ -129,7 +129,9 @@ int cuslines_tracker_destroy(cuslines_tracker *t);
 
 /* Upload the data volume: DWI for bootstrap, ODF amplitudes for prob, the
  * prepared ODF (clip >= 0, zero outside the stop mask, max-normalised) for
- * PTT.  Host float32 [dimx][dimy][dimz][dimt] (cu_tractography.py:204-221). */
+ * PTT.  Host float32 [dimx][dimy][dimz][dimt] (cu_tractography.py:204-221).
+ * PTT data with a negative value is accepted; rejection rounds then measure
+ * every candidate's full support (the results are the same). */
 int cuslines_set_data(cuslines_tracker *t, const float *dataf);
 
 /* Upload the stop map, host float32 [dimx][dimy][dimz]
@@ -174,9 +176,10 @@ int cuslines_set_launch(cuslines_tracker *t, int schedule, int threads, int bloc
 int cuslines_set_lanes_per_seed(cuslines_tracker *t, int lanes);
 
 /* PTT probe taps: 1 (default) reads each tap's 8 texels as one 32-byte
- * corner block, an expanded copy of the ODF built by cuslines_set_data when
- * HBM allows (nverts x (X+1)(Y+1)(Z+1) x 32 B); 0 gathers the 8 texels from
- * the vertex-major ODF.  Results are identical either way. */
+ * corner block, an expanded copy of the ODF built once both
+ * cuslines_set_data and cuslines_set_ptt_lut have been called, when HBM
+ * allows ((X+1)(Y+1)(Z+1) x nverts x 32 B, cell-major); 0 gathers the 8
+ * texels from the vertex-major ODF.  Results are identical either way. */
 int cuslines_set_ptt_layout(cuslines_tracker *t, int corner_blocks);
 
 /* Learned proposal for PTT phase 2: host bf16 parameters (uint16 bit
