@@ -191,3 +191,30 @@ def test_ptt_with_negative_odf_values_bit_exact(small_workloads):
     g, o = _run_both(w2, seeds)
     assert len(o["sline_len"]) > 0
     _assert_same(g, o)
+
+
+@pytest.mark.parametrize("kind", ["csa", "opdt"])
+def test_boot_odd_matrix_sizes_bit_exact(kind):
+    """90 gradient directions and SH order 8 (45 coefficients): no matrix row
+    length is a multiple of 4, so every matrix-vector product takes the
+    row-major path instead of the lane-interleaved one."""
+    from gpustreamlines_amd import workloads as W
+    from gpustreamlines_amd.boot_utils import prepare_csa, prepare_opdt
+    from gpustreamlines_amd.phantom import make_phantom
+    from gpustreamlines_amd.sphere import single_shell_gtab, symmetric_sphere
+
+    shape = (24, 26, 22)
+    ph = make_phantom(shape, gtab=single_shell_gtab(n_b0=6, n_dirs=90), seed=3)
+    verts, edges = symmetric_sphere(724)
+    prep = (prepare_csa if kind == "csa" else prepare_opdt)(ph.gtab, verts, sh_order_max=8)
+    boot = {k: np.ascontiguousarray(prep[k], dtype=np.float32)
+            for k in ("H", "R", "delta_b", "delta_q", "sampling_matrix")}
+    boot["b0s_mask"] = np.asarray(prep["b0s_mask"], dtype=np.int32)
+    assert boot["H"].shape[0] % 4 != 0 and boot["delta_b"].shape[0] % 4 != 0
+    p = W.base_params(shape, ph.dwi.shape[3], len(verts), len(edges), 0.25, kind.upper())
+    p["delta_nr"] = boot["delta_b"].shape[0]
+    w = W.Workload(kind, p, ph.dwi, ph.fa, verts, edges, boot=boot, extra=dict(phantom=ph))
+    seeds = W.seeds_for(w, 64, placement="mixed")
+    g, o = _run_both(w, seeds)
+    assert len(o["sline_len"]) > 0
+    _assert_same(g, o)
