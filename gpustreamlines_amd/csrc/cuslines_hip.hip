@@ -1108,13 +1108,18 @@ __device__ __forceinline__ void mlp_features(const DevParams &P, const DevBufs &
         for (int i = 0; i < 9; ++i) pf[i] = fr[i];
 #pragma unroll
         for (int i = 0; i < 9; ++i) prop[i] = B.fprop[9 * l + i];
-        float qx = px, qy = py, qz = pz;
+        // both frames first (they do not depend on the taps), then both LUT
+        // reads, then both block reads, so the two taps' loads overlap
         propagate_frame(prop, pf, d);
-        qx += d[0]; qy += d[1]; qz += d[2];
-        const float t1 = interp4(P, B, qx, qy, qz, pf[0], pf[1], pf[2]);
+        const float q1x = px + d[0], q1y = py + d[1], q1z = pz + d[2];
+        const uint32_t lv1 = B.lut[lut_index(P, pf[0], pf[1], pf[2])];
         propagate_frame(prop, pf, d);
-        qx += d[0]; qy += d[1]; qz += d[2];
-        const float t2 = interp4(P, B, qx, qy, qz, pf[0], pf[1], pf[2]);
+        const float q2x = q1x + d[0], q2y = q1y + d[1], q2z = q1z + d[2];
+        const uint32_t lv2 = B.lut[lut_index(P, pf[0], pf[1], pf[2])];
+        const Tap a = tap_begin(P, B, lv1, q1x, q1y, q1z);
+        const Tap b = tap_begin(P, B, lv2, q2x, q2y, q2z);
+        const float t1 = tap_end(P, B, a);
+        const float t2 = tap_end(P, B, b);
         x[l] = t1 + t2;
     } else if (l == DISC_VERT_CNT) {
         x[l] = first_val;
