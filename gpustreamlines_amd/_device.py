@@ -57,11 +57,13 @@ class DeviceTracker:
 
     # -- setup ---------------------------------------------------------------
     def upload(self, dataf, stop_map, vertices, edges, boot=None, ptt_lut=None):
-        d = np.ascontiguousarray(dataf, dtype=np.float32)
+        """``dataf`` None: the data come later (``ptt_prepare``)."""
         s = np.ascontiguousarray(stop_map, dtype=np.float32)
         v = np.ascontiguousarray(vertices, dtype=np.float32)
         e = np.ascontiguousarray(edges, dtype=np.int32)
-        _lib.check(self.L.cuslines_set_data(self.h, d.ctypes.data), "set_data")
+        if dataf is not None:
+            d = np.ascontiguousarray(dataf, dtype=np.float32)
+            _lib.check(self.L.cuslines_set_data(self.h, d.ctypes.data), "set_data")
         _lib.check(self.L.cuslines_set_stop_map(self.h, s.ctypes.data), "set_stop_map")
         _lib.check(self.L.cuslines_set_sphere(self.h, v.ctypes.data, e.ctypes.data), "set_sphere")
         if boot is not None:
@@ -78,6 +80,18 @@ class DeviceTracker:
     def set_lut(self, lut):
         lut = np.ascontiguousarray(lut, dtype=np.int32)
         _lib.check(self.L.cuslines_set_ptt_lut(self.h, lut.ctypes.data), "set_ptt_lut")
+
+    def ptt_prepare(self, odf, stop_map, stop_threshold, sphere_symm=False):
+        """PTT data preparation on the device (cuslines_ptt_prepare): raw ODF
+        in, prepared volume, LUT and corner blocks on the device; returns the
+        LUT (int32 [lut_res]^3)."""
+        o = np.ascontiguousarray(odf, dtype=np.float32)
+        s = np.ascontiguousarray(stop_map, dtype=np.float32)
+        r = int(self.params["lut_res"])
+        lut = np.zeros((r, r, r), dtype=np.int32)
+        _lib.check(self.L.cuslines_ptt_prepare(self.h, o.ctypes.data, s.ctypes.data, float(stop_threshold),
+                                               int(bool(sphere_symm)), lut.ctypes.data), "ptt_prepare")
+        return lut
 
     def close(self):
         if getattr(self, "h", None):

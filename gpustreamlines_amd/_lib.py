@@ -90,6 +90,7 @@ def load():
         "cuslines_set_sphere": [vp, vp, vp],
         "cuslines_set_boot_matrices": [vp, vp, vp, vp, vp, vp, vp],
         "cuslines_set_ptt_lut": [vp, vp],
+        "cuslines_ptt_prepare": [vp, vp, vp, C.c_float, C.c_int, vp],
         "cuslines_device_alloc": [vp, sz, C.POINTER(vp)],
         "cuslines_device_free": [vp, vp],
         "cuslines_memcpy_htod": [vp, vp, vp, sz],

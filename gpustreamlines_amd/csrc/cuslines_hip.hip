@@ -2286,6 +2286,62 @@ __global__ void build_corner_blocks(DevParams P, const float *__restrict__ odfv,
     out[1] = hi;
 }
 
+// PttDirectionGetter.prepare_data (cu_direction_getters.py:531-625) on the
+// device, one thread per voxel, in the host recipe's order: clip at 0 (NaN
+// kept and -0.0 made +0.0, as numpy's clip does), zero the voxel where the stop value is
+// below the threshold, then divide by the voxel's maximum when it is > 0 (a
+// NaN maximum, as numpy's max propagates it, leaves the voxel unchanged).
+__global__ void ptt_normalise(float *__restrict__ odf, const float *__restrict__ stop, float thr, long long nvox,
+                              int T) {
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvox) return;
+    float *o = odf + (size_t)v * T;
+    const bool zero = stop[v] < thr;
+    float mx = 0.0f;
+    bool first = true;
+    for (int t = 0; t < T; ++t) {
+        float x = o[t];
+        x = zero ? 0.0f : (x > 0.0f || x != x) ? x : 0.0f;  // -0.0 becomes +0.0, as in numpy
+        o[t] = x;
+        if (first || x != x || x > mx) { if (!(mx != mx)) mx = x; }
+        first = false;
+    }
+    if (mx > 0.0f)
+        for (int t = 0; t < T; ++t) o[t] = o[t] / mx;
+}
+
+// Nearest sphere vertex of every point of the linspace(-1, 1, res)^3 grid
+// ([i0][i1][i2], i2 fastest), as PttDirectionGetter.prepare_data's KDTree
+// query (cu_direction_getters.py:601-623): grid values i * (2 / (res - 1)) - 1
+// in double with the last one exactly 1 (numpy's linspace), squared
+// Euclidean distances in double, the first of equal minima; with sphere_symm
+// the antipodes are candidates too, after all the vertices.
+__global__ void ptt_lut_kernel(const float *__restrict__ verts, int nv, int res, int symm, int32_t *__restrict__ lut) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = (long long)res * res * res;
+    if (idx >= n) return;
+    const int i2 = (int)(idx % res), i1 = (int)((idx / res) % res), i0 = (int)(idx / ((long long)res * res));
+    const double step = 2.0 / (double)(res - 1);
+    auto coord = [&](int i) -> double {
+        if (i == res - 1) return 1.0;
+        const double y = (double)i * step;
+        return y + (-1.0);
+    };
+    const double px = coord(i0), py = coord(i1), pz = coord(i2);
+    double best = 0.0;
+    int bi = -1;
+    for (int pass = 0; pass < (symm ? 2 : 1); ++pass) {
+        const double sg = pass ? -1.0 : 1.0;
+        for (int v = 0; v < nv; ++v) {
+            const double dx = px - sg * (double)verts[3 * v], dy = py - sg * (double)verts[3 * v + 1],
+                         dz = pz - sg * (double)verts[3 * v + 2];
+            const double d = (dx * dx + dy * dy) + dz * dz;
+            if (bi < 0 || d < best) { best = d; bi = v; }
+        }
+    }
+    lut[idx] = bi;
+}
+
 // PTT data: [X][Y][Z][T] -> vertex-major [T][X][Y][Z] (32x32 LDS tiles).
 __global__ void transpose_vox_t(const float *__restrict__ src, float *__restrict__ dst, long long nvox, int T) {
     __shared__ float tile[32][33];
@@ -2777,6 +2833,31 @@ static int build_blocks(cuslines_tracker *t) {
     return CUSLINES_OK;
 }
 
+// PTT: the [X][Y][Z][T] ODF in device buffer `src` (left unchanged) becomes
+// the tracker's vertex-major copy; corner blocks follow when the LUT is set.
+static int ptt_take_data(cuslines_tracker *t, const float *src, int nonneg) {
+    const DevParams &d = t->dp;
+    const size_t nvox = (size_t)d.dimx * d.dimy * d.dimz;
+    const size_t bytes = nvox * d.dimt * sizeof(float);
+    dfree(t->d_data);
+    t->d_data = nullptr;
+    t->has_data = false;
+    t->dp.ptt_nonneg = nonneg;
+    if (hipMalloc(&t->d_data, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        t->d_data = nullptr;
+        return fail(CUSLINES_ENOMEM, "PTT data allocation failed");
+    }
+    dim3 grid((unsigned)((nvox + 31) / 32), (unsigned)((d.dimt + 31) / 32));
+    hipLaunchKernelGGL(transpose_vox_t, grid, dim3(256), 0, t->stream, src, t->d_data, (long long)nvox, d.dimt);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+    if (e != hipSuccess) return fail(CUSLINES_EHIP, std::string("PTT data transpose: ") + hipGetErrorString(e));
+    t->has_data = true;
+    if (t->has_lut) return build_blocks(t);
+    return CUSLINES_OK;
+}
+
 int cuslines_set_data(cuslines_tracker *t, const float *dataf) {
     if (!t || !dataf) return fail(CUSLINES_EINVAL, "null argument");
     HIPC(hipSetDevice(t->device));
@@ -2789,24 +2870,14 @@ int cuslines_set_data(cuslines_tracker *t, const float *dataf) {
         // The early stop of rejection-round probes (data_support) relies on
         // taps >= 0; the reference's prepare_data clips the ODF at 0.
         const size_t nval = nvox * (size_t)d.dimt;
-        t->dp.ptt_nonneg = std::none_of(dataf, dataf + nval, [](float v) { return v < 0.0f; }) ? 1 : 0;
+        const int nonneg = std::none_of(dataf, dataf + nval, [](float v) { return v < 0.0f; }) ? 1 : 0;
         float *tmp = nullptr;
-        const size_t bytes = nvox * d.dimt * sizeof(float);
-        HIPC(hipMalloc(&tmp, bytes));
-        hipError_t e = hipMalloc(&t->d_data, bytes);
-        if (e != hipSuccess) { dfree(tmp); return fail(CUSLINES_ENOMEM, "PTT data allocation failed"); }
-        e = hipMemcpy(tmp, dataf, bytes, hipMemcpyHostToDevice);
-        if (e == hipSuccess) {
-            dim3 grid((unsigned)((nvox + 31) / 32), (unsigned)((d.dimt + 31) / 32));
-            hipLaunchKernelGGL(transpose_vox_t, grid, dim3(256), 0, t->stream, tmp, t->d_data, (long long)nvox, d.dimt);
-            e = hipGetLastError();
-        }
-        if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+        HIPC(hipMalloc(&tmp, nval * sizeof(float)));
+        hipError_t e = hipMemcpy(tmp, dataf, nval * sizeof(float), hipMemcpyHostToDevice);
+        if (e != hipSuccess) { dfree(tmp); return fail(CUSLINES_EHIP, std::string("PTT data upload: ") + hipGetErrorString(e)); }
+        const int rc = ptt_take_data(t, tmp, nonneg);
         dfree(tmp);
-        if (e != hipSuccess) return fail(CUSLINES_EHIP, std::string("PTT data upload: ") + hipGetErrorString(e));
-        t->has_data = true;
-        if (t->has_lut) return build_blocks(t);
-        return CUSLINES_OK;
+        return rc;
     } else {
         const size_t pitch = (size_t)d.t32 * sizeof(float);
         HIPC(hipMalloc(&t->d_data, nvox * pitch));
@@ -3033,6 +3104,50 @@ int cuslines_set_lanes_per_seed(cuslines_tracker *t, int lanes) {
     if (lanes != 0 && lanes != 8 && lanes != 16 && lanes != 32)
         return fail(CUSLINES_EINVAL, "lanes per seed must be 0, 8, 16 or 32");
     t->ptt_lanes = lanes;
+    return CUSLINES_OK;
+}
+
+int cuslines_ptt_prepare(cuslines_tracker *t, const float *odf, const float *stop_map, float stop_threshold,
+                         int sphere_symm, int32_t *lut_out) {
+    if (!t || !odf || !stop_map) return fail(CUSLINES_EINVAL, "null argument");
+    if (t->p.model_type != CUSLINES_PTT) return fail(CUSLINES_ESTATE, "PTT preparation needs a PTT tracker");
+    if (!t->has_sphere) return fail(CUSLINES_ESTATE, "upload the sphere before preparing PTT data");
+    if (t->dp.lut_res < 2) return fail(CUSLINES_EINVAL, "lut_res must be at least 2");
+    HIPC(hipSetDevice(t->device));
+    const DevParams &d = t->dp;
+    const size_t nvox = (size_t)d.dimx * d.dimy * d.dimz;
+    const size_t nval = nvox * (size_t)d.dimt;
+    const size_t nlut = (size_t)d.lut_res * d.lut_res * d.lut_res;
+    float *raw = nullptr, *stop = nullptr;
+    int32_t *lut = nullptr;
+    hipError_t e = hipMalloc(&raw, nval * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&stop, nvox * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&lut, nlut * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMemcpy(raw, odf, nval * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(stop, stop_map, nvox * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(ptt_normalise, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, t->stream, raw, stop,
+                           stop_threshold, (long long)nvox, d.dimt);
+        hipLaunchKernelGGL(ptt_lut_kernel, dim3((unsigned)((nlut + 255) / 256)), dim3(256), 0, t->stream, t->d_verts,
+                           d.nverts, d.lut_res, sphere_symm ? 1 : 0, lut);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+    std::vector<int32_t> hl(nlut);
+    if (e == hipSuccess) e = hipMemcpy(hl.data(), lut, nlut * sizeof(int32_t), hipMemcpyDeviceToHost);
+    dfree(stop);
+    dfree(lut);
+    if (e != hipSuccess) {
+        dfree(raw);
+        return fail(CUSLINES_EHIP, std::string("PTT preparation: ") + hipGetErrorString(e));
+    }
+    // the clipped ODF has no negative value
+    int rc = ptt_take_data(t, raw, 1);
+    dfree(raw);
+    if (rc) return rc;
+    rc = cuslines_set_ptt_lut(t, hl.data());
+    if (rc) return rc;
+    if (lut_out) std::memcpy(lut_out, hl.data(), nlut * sizeof(int32_t));
     return CUSLINES_OK;
 }
 

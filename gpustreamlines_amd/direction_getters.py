@@ -183,11 +183,15 @@ class PttDirectionGetter(ProbDirectionGetter):
     model_type = MODEL_TYPES["PTT"]
 
     def __init__(self, odf_lut_res: int = 128, probe_length: float = 0.25,
-                 target_short_step: float = 0.025, probe_quality: int = 8, proposal=None):
+                 target_short_step: float = 0.025, probe_quality: int = 8, proposal=None,
+                 prepare_on_device: bool = False):
         """``proposal`` (not in the reference): parameters of a trained
         learned-proposal network (float32, or bf16 bit patterns as uint16;
         see gpustreamlines_amd.learned).  None keeps the reference's face-CDF
-        proposal."""
+        proposal.  ``prepare_on_device`` (not in the reference): clip,
+        mask, normalise and build the LUT on the GPU (cuslines_ptt_prepare)
+        instead of on the host; same results."""
+        self.prepare_on_device = bool(prepare_on_device)
         self.odf_lut_res = int(odf_lut_res)
         self.probe_length = probe_length
         self.probe_quality = int(probe_quality)
@@ -211,19 +215,27 @@ class PttDirectionGetter(ProbDirectionGetter):
         nearest-vertex LUT (cu_direction_getters.py:531-625).  The tiling is
         kept as index arithmetic; the data stay [X][Y][Z][T] on the host."""
         dimx, dimy, dimz, dimt = dataf.shape
+        nx, ny, nz = calculate_tight_fit(dimt)
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.log2_x, self.log2_y, self.log2_z = (int(math.log2(nx)), int(math.log2(ny)), int(math.log2(nz)))
+        if self.prepare_on_device:
+            # the raw ODF goes to the device in allocate_on_gpu
+            self._device_inputs = (np.asarray(stop_map, dtype=np.float32), float(stop_threshold), bool(sphere_symm))
+            return np.ascontiguousarray(dataf, dtype=np.float32)
         dataf = np.asarray(dataf, dtype=np.float32).clip(min=0)
         dataf[np.asarray(stop_map) < stop_threshold, :] = 0
         mx = dataf.max(axis=3, keepdims=True)
         np.divide(dataf, mx, out=dataf, where=mx > 0)
-        nx, ny, nz = calculate_tight_fit(dimt)
-        self.nx, self.ny, self.nz = nx, ny, nz
-        self.log2_x, self.log2_y, self.log2_z = (int(math.log2(nx)), int(math.log2(ny)), int(math.log2(nz)))
         self.sphere_vertices_lut_h = nearest_vertex_lut(sphere_vertices, self.odf_lut_res, sphere_symm)
         return np.ascontiguousarray(dataf, dtype=np.float32)
 
     def allocate_on_gpu(self, n):
         dt = self.gpu_tracker.device_trackers[n]
-        dt.set_lut(self.sphere_vertices_lut_h)
+        if self.prepare_on_device:
+            stop, thr, symm = self._device_inputs
+            self.sphere_vertices_lut_h = dt.ptt_prepare(self.gpu_tracker.dataf, stop, thr, symm)
+        else:
+            dt.set_lut(self.sphere_vertices_lut_h)
         if self.proposal is not None:
             from .learned import to_bf16
 

@@ -91,7 +91,8 @@ class GPUTracker(GenericTracker):
         for ii in range(self.ngpus):
             dt = DeviceTracker(params, device=ii)
             self.device_trackers.append(dt)
-            dt.upload(self.dataf, self.metric_map, self.sphere_vertices, self.sphere_edges)
+            on_device = getattr(self.dg, "prepare_on_device", False)
+            dt.upload(None if on_device else self.dataf, self.metric_map, self.sphere_vertices, self.sphere_edges)
             self.dg.allocate_on_gpu(ii)
         self._allocated = True
 

@@ -155,6 +155,19 @@ int cuslines_set_boot_matrices(cuslines_tracker *t, const float *H, const float 
  * cu_direction_getters.py:601-623, uploaded as a point-sampled texture). */
 int cuslines_set_ptt_lut(cuslines_tracker *t, const int32_t *lut);
 
+/* PTT data preparation on the device, in place of cuslines_set_data plus
+ * cuslines_set_ptt_lut with host-prepared inputs: PttDirectionGetter.prepare_data
+ * (cu_direction_getters.py:531-625) clips the raw ODF `odf` (host float32
+ * [dimx][dimy][dimz][dimt]) at 0, zeroes voxels whose `stop_map` value is
+ * below `stop_threshold`, divides each voxel by its maximum when positive,
+ * and builds the nearest-vertex LUT of the uploaded sphere (antipodes too
+ * when `sphere_symm`) over linspace(-1, 1, lut_res)^3.  The sphere must be
+ * uploaded first; the tiling comes from the tracker's parameters.  `lut_out`
+ * (optional, int32 [lut_res]^3) receives the LUT.  Same results as the host
+ * recipe. */
+int cuslines_ptt_prepare(cuslines_tracker *t, const float *odf, const float *stop_map, float stop_threshold,
+                         int sphere_symm, int32_t *lut_out);
+
 /* Phase-2 schedules. */
 enum {
     CUSLINES_SCHED_QUEUE = 0,  /* persistent grid, slots pull seeds from a counter */

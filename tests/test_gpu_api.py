@@ -49,3 +49,34 @@ def test_gpu_tracker_generate_streamlines(small_workloads, kind, launch_seeds):
     pts, lens = _oracle_by_chunks(w, seeds, chunk)
     assert np.array_equal(out._lengths, lens)
     assert np.array_equal(out._data, pts)
+
+
+def test_ptt_prepare_on_device_matches_host(small_workloads):
+    """PttDirectionGetter(prepare_on_device=True): clip, stop-mask zeroing,
+    per-voxel normalisation and the nearest-vertex LUT on the GPU
+    (cuslines_ptt_prepare) give the host recipe's LUT and the same
+    streamlines, on a raw ODF with negative values."""
+    import time
+
+    from gpustreamlines_amd import GPUTracker, PttDirectionGetter
+    from gpustreamlines_amd import workloads as W
+
+    w = small_workloads["prob"]
+    raw = (w.dataf - np.float32(0.05)).astype(np.float32)  # negative lobes, for the clip
+    seeds = W.seeds_for(w, 80, placement="mixed", seed=5)
+    p = w.params
+    out, luts, prep = {}, {}, {}
+    for name, dev in (("host", False), ("device", True)):
+        dg = PttDirectionGetter(prepare_on_device=dev)
+        t0 = time.perf_counter()
+        tr = GPUTracker(dg, raw, w.stop_map, p["tc_threshold"], w.vertices, w.edges, step_size=p["step_size"],
+                        chunk_size=40)
+        with tr:
+            prep[name] = time.perf_counter() - t0
+            out[name] = tr.generate_streamlines(seeds)
+        luts[name] = dg.sphere_vertices_lut_h
+    print(f"[api] PTT preparation + upload: host {prep['host']:.2f}s, device {prep['device']:.2f}s", flush=True)
+    assert np.array_equal(luts["host"], luts["device"])
+    assert len(out["host"]._lengths) > 0
+    assert np.array_equal(out["host"]._lengths, out["device"]._lengths)
+    assert np.array_equal(out["host"]._data, out["device"]._data)
