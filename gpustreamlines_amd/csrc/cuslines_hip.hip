@@ -954,9 +954,15 @@ __device__ int ptt_substep(const DevParams &P, const DevBufs &B, PttLds &S, floa
             float r1 = rng.uniform(), r2 = rng.uniform();
             if (r1 + r2 > 1.0f) { r1 = 1.0f - r1; r2 = 1.0f - r2; }
             const float tmp = rng.uniform() * last;
-            int jj = 0;
-            for (; jj < DISC_FACE_CNT - 1; ++jj)
-                if (S.face_cdf[jj] >= tmp) break;
+            // first face jj < 30 with cdf >= tmp, else the last (the
+            // reference's linear scan): the cdf is a prefix sum of values
+            // >= 0, so it never decreases and a binary search finds the same
+            // face in 5 reads (a NaN tmp fails every test in both)
+            int jj = 0, hi = DISC_FACE_CNT - 1;
+            while (jj < hi) {
+                const int mid = (jj + hi) >> 1;
+                if (S.face_cdf[mid] >= tmp) hi = mid; else jj = mid + 1;
+            }
             const int f0 = c_disc_face[jj * 3], f1 = c_disc_face[jj * 3 + 1], f2 = c_disc_face[jj * 3 + 2];
             const float vx0 = (float)(mc * (double)c_disc_vert[f0 * 2]);
             const float vx1 = (float)(mc * (double)c_disc_vert[f1 * 2]);
