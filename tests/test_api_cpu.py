@@ -69,3 +69,24 @@ def test_streamline_array():
     assert len(a) == 2 and a[1].shape == (2, 3) and a._data.shape == (5, 3)
     b = StreamlineArray.from_packed(a._data, a._lengths)
     assert [x.shape for x in b] == [(3, 3), (2, 3)]
+
+
+def test_ptt_prepare_on_device_defers_to_the_gpu():
+    """prepare_on_device: prepare_data keeps the raw ODF (the GPU clips,
+    masks and normalises it) and only chooses the tiling, like the host
+    recipe does."""
+    from gpustreamlines_amd.sphere import symmetric_sphere
+
+    v, _ = symmetric_sphere(724)
+    rng = np.random.default_rng(1)
+    data = rng.normal(size=(3, 4, 5, 724)).astype(np.float32)
+    stop = rng.random((3, 4, 5)).astype(np.float32)
+    dev = D.PttDirectionGetter(odf_lut_res=16, prepare_on_device=True)
+    out = dev.prepare_data(data, stop, 0.3, v, False)
+    assert np.array_equal(out, data)
+    host = D.PttDirectionGetter(odf_lut_res=16)
+    host.prepare_data(data, stop, 0.3, v, False)
+    assert (dev.log2_x, dev.log2_y, dev.log2_z) == (host.log2_x, host.log2_y, host.log2_z)
+    assert dev.sphere_vertices_lut_h is None
+    s, thr, symm = dev._device_inputs
+    assert np.array_equal(s, stop) and thr == 0.3 and symm is False

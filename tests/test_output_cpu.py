@@ -155,3 +155,33 @@ def test_generate_sft_uses_packed_arrays(monkeypatch):
     assert sft.space == "vox" and sft.ref == "ref"
     np.testing.assert_array_equal(sft.seq._data, np.concatenate(want))
     np.testing.assert_array_equal(sft.seq._lengths, [len(w) for w in want])
+
+
+class RecordingPropagator(FakePropagator):
+    def __init__(self, **kw):
+        super().__init__(**kw)
+        self.rng_chunk = None
+        self.rng_chunks = []
+
+    def propagate(self, seeds):
+        self.rng_chunks.append(self.rng_chunk)
+        super().propagate(seeds)
+
+
+def test_chunks_grouped_into_launches():
+    """On one GPU, launch_seeds groups whole chunks into one propagate call
+    and sets rng_chunk to the chunk size, so each chunk keeps its own random
+    streams; launch_seeds 0 keeps one chunk per call."""
+    seeds = _seeds(23)
+    tr = Tracker(chunk_size=5)
+    tr.seed_propagator = RecordingPropagator()
+    tr.launch_seeds = 10
+    tr.generate_streamlines(seeds)
+    assert tr.seed_propagator.calls == [10, 10, 3]
+    assert tr.seed_propagator.rng_chunks == [5, 5, 5]
+    tr2 = Tracker(chunk_size=5)
+    tr2.seed_propagator = RecordingPropagator()
+    tr2.launch_seeds = 0
+    tr2.generate_streamlines(seeds)
+    assert tr2.seed_propagator.calls == [5, 5, 5, 5, 3]
+    assert tr2.seed_propagator.rng_chunks == [None] * 5
