@@ -43,7 +43,8 @@ def alg_bytes_per_step(kind: str, p: dict) -> float:
 
 
 MODEL_IDS = {"opdt": 0, "csa": 1, "prob": 2, "ptt": 3}
-CPU_SAMPLE = {"ptt": 2000, "prob": 150000, "csa": 4000, "opdt": 4000}
+# Seeds of the CPU baseline sample: about 10-20 s of oracle work on 16 host threads.
+CPU_SAMPLE = {"ptt": 2000, "prob": 250000, "csa": 8000, "opdt": 14000}
 
 
 def measured_traffic(workload: str, kernel: str):
