@@ -2899,6 +2899,8 @@ int cuslines_set_stop_map(cuslines_tracker *t, const float *stop_map) {
     if (!t || !stop_map) return fail(CUSLINES_EINVAL, "null argument");
     HIPC(hipSetDevice(t->device));
     const size_t n = (size_t)t->dp.dimx * t->dp.dimy * t->dp.dimz;
+    // A failed re-upload must not leave the tracker ready with freed buffers.
+    t->has_stop = false;
     dfree(t->d_stop);
     t->d_stop = nullptr;
     HIPC(hipMalloc(&t->d_stop, n * sizeof(float)));
@@ -2917,6 +2919,7 @@ int cuslines_set_sphere(cuslines_tracker *t, const float *vertices, const int32_
         if (u < 0 || u >= nv || v < 0 || v >= nv) return fail(CUSLINES_EINVAL, "edge vertex index out of range");
         packed[e] = (uint32_t)u | ((uint32_t)v << 16);
     }
+    t->has_sphere = false;
     dfree(t->d_verts); dfree(t->d_edges); dfree(t->d_nbr);
     t->d_verts = nullptr; t->d_edges = nullptr; t->d_nbr = nullptr;
     HIPC(hipMalloc(&t->d_verts, (size_t)nv * 3 * sizeof(float)));
@@ -2965,6 +2968,7 @@ int cuslines_set_boot_matrices(cuslines_tracker *t, const float *H, const float 
     }
     const int hr = (int)dwi.size();
     if (hr == 0 || nb0 == 0) return fail(CUSLINES_EINVAL, "bootstrap needs at least one b0 and one DWI volume");
+    t->has_boot = false;
     t->dp.hr = hr;
     t->dp.hr32 = ceil32(hr);
     float **dst[5] = {&t->d_H, &t->d_R, &t->d_db, &t->d_dq, &t->d_smat};
@@ -3041,6 +3045,7 @@ int cuslines_set_ptt_lut(cuslines_tracker *t, const int32_t *lut) {
                                          : (i0 * r + i1) * r + i2;
                 h[dst] = (uint32_t)v | ((uint32_t)rank[v] << 16);
             }
+    t->has_lut = false;
     t->dp.lut_brick = brick ? 1 : 0;
     dfree(t->d_lut);
     t->d_lut = nullptr;
@@ -3239,8 +3244,13 @@ static size_t mlp_ws_floats(int n) {
     return (size_t)n * (4 * MLP_H + 2 * MLP_OUT + 1) + nchunk * MLP_H * (MLP_H + 1);
 }
 
+// The weight-gradient partials take one grid row per WGRAD_CHUNK pairs, and a
+// grid has at most 65535 rows.
+constexpr int MLP_MAX_PAIRS = 65535 * WGRAD_CHUNK;
+
 int cuslines_mlp_workspace_bytes(int32_t n, size_t *bytes) {
     if (!bytes || n <= 0) return fail(CUSLINES_EINVAL, "n > 0 and bytes required");
+    if (n > MLP_MAX_PAIRS) return fail(CUSLINES_EINVAL, "at most 8,388,480 pairs per training call");
     *bytes = mlp_ws_floats(n) * sizeof(float);
     return CUSLINES_OK;
 }
@@ -3249,6 +3259,7 @@ int cuslines_mlp_loss_grad_async(cuslines_tracker *t, const float *params_d, con
                                  float *loss_d, float *grad_d, void *work_d) {
     if (!t || !params_d || !pairs_d || !loss_d || !grad_d || !work_d || n <= 0)
         return fail(CUSLINES_EINVAL, "null argument or n <= 0");
+    if (n > MLP_MAX_PAIRS) return fail(CUSLINES_EINVAL, "at most 8,388,480 pairs per training call");
     HIPC(hipSetDevice(t->device));
     float *H1 = static_cast<float *>(work_d);
     float *H2 = H1 + (size_t)n * MLP_H;

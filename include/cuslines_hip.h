@@ -224,7 +224,8 @@ int cuslines_proposal_stats(cuslines_tracker *t, int64_t *out3, int reset);
  * cuslines_mlp_loss_grad_async: mean over n pairs of the squared error of
  * the 24 predicted supports, averaged over the 24, into loss_d[0], and its
  * gradient into grad_d [CUSLINES_MLP_NPARAM]; work_d is a caller buffer of
- * cuslines_mlp_workspace_bytes(n) bytes.
+ * cuslines_mlp_workspace_bytes(n) bytes; n <= 8,388,480 (65535 chunks of
+ * 128 pairs), larger n is rejected with CUSLINES_EINVAL.
  * cuslines_mlp_adam_async: one Adam step (Kingma and Ba, 2015) with step
  * count `step` (>= 1); m_d and v_d start at zero. */
 int cuslines_mlp_workspace_bytes(int32_t n, size_t *bytes);

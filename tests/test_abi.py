@@ -67,3 +67,17 @@ def test_errors_are_reported_without_gpu():
     bad = _lib.params_struct(dict(dimx=0, dimy=1, dimz=1, dimt=1, model_type=2, nverts=1, nedges=1))
     assert L.cuslines_tracker_create(ctypes.byref(bad), 0, ctypes.byref(h)) == 1  # CUSLINES_EINVAL
     assert "positive" in _lib.last_error()
+
+
+def test_training_batch_limit_is_checked_without_gpu():
+    from gpustreamlines_amd import _lib
+
+    L = _lib.load()
+    nbytes = ctypes.c_size_t()
+    L.cuslines_mlp_workspace_bytes.argtypes = [ctypes.c_int32, ctypes.POINTER(ctypes.c_size_t)]
+    assert L.cuslines_mlp_workspace_bytes(8_388_480, ctypes.byref(nbytes)) == 0
+    # 65535 chunks of 128 pairs, each pair's activations plus the partials
+    assert nbytes.value == 4 * (8_388_480 * (4 * 128 + 2 * 24 + 1) + 65535 * 128 * 129)
+    assert L.cuslines_mlp_workspace_bytes(8_388_481, ctypes.byref(nbytes)) == 1  # CUSLINES_EINVAL
+    assert "8,388,480" in _lib.last_error()
+    assert L.cuslines_mlp_workspace_bytes(0, ctypes.byref(nbytes)) == 1
