@@ -70,15 +70,15 @@ class GenericTracker:
         for idx in range(0, nchunks, per):
             sp.propagate(seeds[idx * gsz: (idx + per) * gsz])
             for ii in range(sp.ngpus):
-                pts, lens, offs = sp.slines[ii], sp.sline_lens[ii], sp.sline_offs[ii]
+                pts, lens = sp.slines[ii], sp.sline_lens[ii]
                 if lens is None or len(lens) == 0:
                     continue
                 keep = (lens >= sp.minlen) & (lens <= sp.maxlen)
                 if keep.all():
                     yield pts, lens.astype(np.int64)
                 elif keep.any():
-                    idx_pts = np.concatenate([np.arange(offs[j], offs[j] + lens[j]) for j in np.nonzero(keep)[0]])
-                    yield pts[idx_pts], lens[keep].astype(np.int64)
+                    # points are packed in streamline order: one mask over all of them
+                    yield pts[np.repeat(keep, lens)], lens[keep].astype(np.int64)
 
     def generate_streamlines(self, seeds) -> StreamlineArray:
         """All kept streamlines of ``seeds`` (voxel coordinates), packed."""
