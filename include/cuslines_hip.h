@@ -234,7 +234,10 @@ int cuslines_mlp_loss_grad_async(cuslines_tracker *t, const float *params_d, con
 int cuslines_mlp_adam_async(cuslines_tracker *t, float *params_d, const float *grad_d, float *m_d, float *v_d,
                             int32_t step, float lr, float beta1, float beta2, float eps);
 
-/* Device memory helpers for callers that keep seeds and outputs resident. */
+/* Device memory helpers for callers that keep seeds and outputs resident:
+ * the reference's cudaMalloc / cudaMemcpy / cudaFree / cudaStreamSynchronize
+ * calls on its per-GPU streams (cuslines/cuda_python/cu_propagate_seeds.py:56-201,220-231).
+ * The copies are synchronous on the handle's stream. */
 int cuslines_device_alloc(cuslines_tracker *t, size_t bytes, void **dptr);
 int cuslines_device_free(cuslines_tracker *t, void *dptr);
 int cuslines_memcpy_htod(cuslines_tracker *t, void *dst, const void *src, size_t bytes);
