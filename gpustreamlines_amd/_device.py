@@ -39,6 +39,8 @@ class DeviceBuffer:
         _lib.check(self.dt.L.cuslines_memcpy_htod(self.dt.h, C.c_void_p(self.ptr), a.ctypes.data, a.nbytes), "htod")
 
     def download(self, out: np.ndarray):
+        if not out.flags.c_contiguous or not out.flags.writeable:
+            raise ValueError("download needs a writeable C-contiguous host array")
         if out.nbytes > self.nbytes:
             raise ValueError("host array larger than the device buffer")
         _lib.check(self.dt.L.cuslines_memcpy_dtoh(self.dt.h, out.ctypes.data, C.c_void_p(self.ptr), out.nbytes), "dtoh")
