@@ -59,15 +59,20 @@ def record_pairs(dt, seeds: np.ndarray, capacity: int, rate: float, rng_chunk: i
     count)."""
     pairs = dt.alloc(4 * PAIR * capacity)
     cnt = dt.alloc(4)
-    cnt.upload(np.zeros(1, np.int32))
-    dt.set_proposal(None)
-    dt.set_recording(pairs.ptr, capacity, cnt.ptr, rate)
     try:
-        dt.track(seeds, rng_chunk=rng_chunk)
+        cnt.upload(np.zeros(1, np.int32))
+        dt.set_proposal(None)
+        dt.set_recording(pairs.ptr, capacity, cnt.ptr, rate)
+        try:
+            dt.track(seeds, rng_chunk=rng_chunk)
+        finally:
+            dt.set_recording(None)
+        n = int(cnt.download(np.zeros(1, np.int32))[0])
+    except BaseException:
+        pairs.free()
+        raise
     finally:
-        dt.set_recording(None)
-    n = int(cnt.download(np.zeros(1, np.int32))[0])
-    cnt.free()
+        cnt.free()
     return pairs, min(n, capacity)
 
 
@@ -103,6 +108,8 @@ class Trainer:
         ``lr_final``; the last ``holdout`` pairs are not trained on.  Returns
         the mean loss of each epoch's last 20 batches."""
         n_train = n_pairs - holdout
+        if n_train <= 0:
+            raise ValueError(f"no pairs left to train on ({n_pairs} pairs, holdout {holdout})")
         nb = max(1, n_train // self.batch)
         total = epochs * nb
         hist = []
@@ -137,11 +144,15 @@ def train(dt, seeds: np.ndarray, rate: float = 0.06, capacity: int = 20_000_000,
     the GPU.  Returns (float32 parameters, info dict)."""
     pairs, n = record_pairs(dt, seeds, capacity, rate, rng_chunk)
     try:
+        if n == 0:
+            raise RuntimeError("no training pairs were recorded (no seed took a non-initial sub-step)")
         tr = Trainer(dt, batch=batch)
-        loss0 = tr.loss_grad(pairs.ptr, min(batch, n))[0]
-        hist = tr.fit(pairs, n, epochs=epochs)
-        params = tr.params()
-        tr.free()
+        try:
+            loss0 = tr.loss_grad(pairs.ptr, min(batch, n))[0]
+            hist = tr.fit(pairs, n, epochs=epochs)
+            params = tr.params()
+        finally:
+            tr.free()
     finally:
         pairs.free()
     if verbose:
