@@ -35,6 +35,10 @@ class _Events:
         self.hip.hipEventCreate(C.byref(b))
         return a, b
 
+    def destroy(self, *events):
+        for e in events:
+            self.hip.hipEventDestroy(e)
+
     def ms(self, a, b):
         self.hip.hipEventSynchronize(b)
         t = C.c_float()
@@ -77,21 +81,25 @@ def tune(dt, seeds: np.ndarray, rng_chunk: int = 0, verbose: bool = False):
         nsl = int(od.download(np.zeros(n + 1, dtype=np.int32))[n])
         ssd = dt.alloc(4 * max(nsl, 1)); bufs.append(ssd)
         sld = dt.alloc(4 * max(nsl, 1)); bufs.append(sld)
-        sl = dt.alloc(4 * 3 * 2 * MAX_SLINE_LEN * max(nsl, 1)); bufs.append(sl)
+        L = int(dt.params.get("max_sline_len", MAX_SLINE_LEN))  # slot = 2 L points
+        sl = dt.alloc(4 * 3 * 2 * L * max(nsl, 1)); bufs.append(sl)
         ev = _Events()
         a, b = ev.pair()
         stream = C.c_void_p(dt.stream())
         times = {}
-        for cfg in candidates(dt.params["model_type"]):
-            apply(dt, cfg)
-            ev.hip.hipEventRecord(a, stream)
-            dt.phase2_async(sd.ptr, n, rng_chunk, od.ptr, dd.ptr, ssd.ptr, sld.ptr, sl.ptr)
-            ev.hip.hipEventRecord(b, stream)
-            dt.synchronize()
-            key = tuple(sorted(cfg.items()))
-            times[key] = ev.ms(a, b)
-            if verbose:
-                print(f"[autotune] {cfg} {times[key]:.1f} ms", flush=True)
+        try:
+            for cfg in candidates(dt.params["model_type"]):
+                apply(dt, cfg)
+                ev.hip.hipEventRecord(a, stream)
+                dt.phase2_async(sd.ptr, n, rng_chunk, od.ptr, dd.ptr, ssd.ptr, sld.ptr, sl.ptr)
+                ev.hip.hipEventRecord(b, stream)
+                dt.synchronize()
+                key = tuple(sorted(cfg.items()))
+                times[key] = ev.ms(a, b)
+                if verbose:
+                    print(f"[autotune] {cfg} {times[key]:.1f} ms", flush=True)
+        finally:
+            ev.destroy(a, b)
         best = dict(min(times, key=times.get))
         apply(dt, best)
         return best, times

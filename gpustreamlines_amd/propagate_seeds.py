@@ -106,7 +106,8 @@ class SeedBatchPropagator:
                 nsl = int(self.nSlines[ii])
                 self.slineSeed_d[ii] = dt.alloc(4 * nsl)
                 self.slineLen_d[ii] = dt.alloc(4 * nsl)
-                self.sline_d[ii] = dt.alloc(REAL_SIZE * 2 * 3 * MAX_SLINE_LEN * nsl)
+                slot_len = int(dt.params.get("max_sline_len", MAX_SLINE_LEN))  # slot = 2 * slot_len points
+                self.sline_d[ii] = dt.alloc(REAL_SIZE * 2 * 3 * slot_len * nsl)
                 if nseeds_gpu and nsl:
                     dg.generateStreamlines(ii, nseeds_gpu, block, grid, self)
             self._cleanup()
