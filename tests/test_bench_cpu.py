@@ -1,0 +1,32 @@
+"""bench.py's host-side contract, checked without a GPU: the one-GPU guard and
+the algorithmic bytes per step that its roofline figure divides by time."""
+
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_more_than_one_gpu_is_refused_with_a_message():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert r.stdout == ""
+    assert "one GPU only" in r.stderr
+
+
+def test_algorithmic_bytes_per_step():
+    sys.path.insert(0, ROOT)
+    import bench
+
+    # 724-channel ODF: 8 corners x 724 x 4 B, 8-texel stop tap, 12-byte point
+    assert bench.alg_bytes_per_step("prob", {"dimt": 724}) == 23_212
+    # 170 DWI channels (the bootstrap getters gather the signal)
+    assert bench.alg_bytes_per_step("csa", {"dimt": 170}) == 5_484
+    # PTT at 10 sub-steps per output point: per sub-step one initial tap, 24
+    # vertices x 8 probes and 32 candidates x 8 probes, each 8 texels + a LUT word
+    assert bench.alg_bytes_per_step("ptt", {"step_frac": 10, "dimt": 724}) == 161_972
+    # per sub-step: 449 taps of 32 + 4 bytes and the stop-map tap
+    per_sub = (bench.alg_bytes_per_step("ptt", {"step_frac": 10}) - 12) / 10
+    assert per_sub == 449 * 36 + 32
