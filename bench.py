@@ -33,8 +33,11 @@ CHUNK = 25000
 def alg_bytes_per_step(kind: str, p: dict) -> float:
     """Compulsory HBM bytes per output step with no cross-step reuse: the
     8-corner gather of the data channels the getter reads, the 8-texel stop
-    map tap and the 12-byte point store.  PTT counts every probe tap of one
-    rejection round (1 + 24*8 + 32*8 taps of 8 texels + LUT) per sub-step."""
+    map tap and the 12-byte point store.  For PTT this is the reference's
+    work per step (every probe tap of one full rejection round, 1 + 24*8 +
+    32*8 taps of one 32-byte block + a 4-byte LUT entry, per sub-step); the
+    bytes the kernel must move are counted by an instrumented pass instead
+    (ptt_alg_bytes)."""
     stop_tap = 8 * 4
     if kind == "ptt":
         taps = 1 + 24 * 8 + 32 * 8
@@ -42,9 +45,21 @@ def alg_bytes_per_step(kind: str, p: dict) -> float:
     return 8 * p["dimt"] * 4 + stop_tap + 12
 
 
+def ptt_alg_bytes(taps: int, substeps: int, points: int) -> int:
+    """Bytes a PTT pass must read and write: each tap the getter reads (one
+    32-byte corner block + its 4-byte LUT entry), one 8-texel stop-map tap per
+    sub-step and the 12-byte store of each output point."""
+    return 36 * taps + 32 * substeps + 12 * points
+
+
 MODEL_IDS = {"opdt": 0, "csa": 1, "prob": 2, "ptt": 3}
-# Seeds of the CPU baseline sample: about 10-20 s of oracle work on 16 host threads.
-CPU_SAMPLE = {"ptt": 2000, "prob": 250000, "csa": 8000, "opdt": 14000}
+# Seeds of the CPU baseline samples: about 10 s of oracle work each, on all
+# host threads (16 on the GPU host) and on one thread.
+CPU_SAMPLE = {"ptt": 1500, "prob": 200000, "csa": 6000, "opdt": 10000}
+CPU_SAMPLE_1T = {"ptt": 80, "prob": 15000, "csa": 450, "opdt": 800}
+# Wall-clock seconds held back for the legs after the plain passes.
+CPU_RESERVE_S = 45.0
+LEARNED_PASSES = 3
 
 
 def measured_traffic(workload: str, kernel: str):
@@ -88,22 +103,44 @@ class HipEvents:
         return float(t.value)
 
 
-def cpu_baseline(w, seeds, n_sample):
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, seeds, n_sample, n_sample_1t):
     """The CPU oracle (C++/OpenMP restatement of the reference path) on the
-    first n_sample seeds of the same workload, all host threads."""
-    from oracle.oracle import Oracle, num_threads
+    first n_sample seeds of the same workload on all host threads, and on the
+    first n_sample_1t seeds on one thread."""
+    from oracle.oracle import Oracle, num_threads, set_num_threads
 
     o = Oracle(w.params, w.dataf, w.stop_map, w.vertices, w.edges, boot=w.boot, ptt_lut=w.ptt_lut)
-    sample = seeds[:n_sample]
-    t0 = time.perf_counter()
-    r = o.track(sample, rng_chunk=CHUNK)
-    dt = time.perf_counter() - t0
-    steps = int((r["sline_len"].astype(np.int64) - 1).clip(min=0).sum())
-    return dict(value=steps / dt, unit="steps/s", cores=num_threads(), kind="port",
-                sample=f"{n_sample} seeds of the same workload ({steps} steps, {len(r['sline_len'])} "
-                       f"streamlines) in {dt:.1f} s")
 
+    def run(ns):
+        t0 = time.perf_counter()
+        r = o.track(seeds[:ns], rng_chunk=CHUNK)
+        dt = time.perf_counter() - t0
+        steps = int((r["sline_len"].astype(np.int64) - 1).clip(min=0).sum())
+        return steps, len(r["sline_len"]), dt
 
+    cores = num_threads()
+    steps, nsl, dt = run(n_sample)
+    set_num_threads(1)
+    steps1, nsl1, dt1 = run(n_sample_1t)
+    set_num_threads(cores)
+    return dict(value=steps / dt, unit="steps/s", cores=cores, kind="port",
+                sample=f"first {n_sample} seeds of the same workload ({steps} steps, {nsl} streamlines) in "
+                       f"{dt:.1f} s on {cores} threads",
+                streamlines_per_s=nsl / dt,
+                one_thread={"value": steps1 / dt1, "unit": "steps/s", "cores": 1,
+                            "sample": f"first {n_sample_1t} seeds ({steps1} steps, {nsl1} streamlines) in {dt1:.1f} s"},
+                cpu_model=cpu_model())
 
 
 def _density(dt, sld, sl, nsl, lens, shape):
@@ -154,21 +191,23 @@ def measure_learned(args, w, dt, step, plain_lens, sld, sl, nsl, L):
     dt.set_proposal(LP.to_bf16(params))
     dt.proposal_stats(reset=True)  # no warmup pass: the plain warmup already loaded the code object
     times = []
-    for i in range(args.steps):
+    npass = max(1, min(args.steps, LEARNED_PASSES))
+    for i in range(npass):
         r = step()
         times.append(r)
         print(f"[bench] learned step {i}: phase2 {r[2]:.1f} ms", file=sys.stderr, flush=True)
-    stats = dt.proposal_stats(reset=True) // max(1, args.steps)
+    stats = dt.proposal_stats(reset=True) // npass
     dt.set_proposal(None)
     lens = sld.download(np.zeros(nsl, dtype=np.int32)).astype(np.int64)
     d_learned = _density(dt, sld, sl, nsl, lens, shape)
-    t_step = float(np.mean([t[3] for t in times])) / 1e3
+    t_step = float(np.median([t[3] for t in times])) / 1e3
     steps = int((lens - 1).clip(min=0).sum())
     ks = ks_2samp(plain_lens, lens)
     return {
+        "steps": npass,
         "steps_per_s": steps / t_step,
         "ms_per_step": t_step * 1e3,
-        "phase2_ms": float(np.mean([t[2] for t in times])),
+        "phase2_ms": float(np.median([t[2] for t in times])),
         "points": int(lens.sum()),
         "first_round_accepted_substeps": int(stats[0]),
         "fallback_substeps": int(stats[1]),
@@ -234,10 +273,13 @@ def copy_calibration(nbytes):
 
 
 def main():
+    t_start = time.perf_counter()
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5, help="timed passes (capped by --budget, at least 3 kept)")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed passes (capped by --budget, at least 1 kept)")
+    ap.add_argument("--budget", type=float, default=360.0,
+                    help="wall-clock seconds for the whole run; the pass counts are cut to fit")
     ap.add_argument("--getter", default="ptt", choices=["ptt", "prob", "csa", "opdt"])
     ap.add_argument("--seeds", type=int, default=1_000_000)
     ap.add_argument("--placement", default="mixed", choices=["mixed", "mask"])
@@ -261,7 +303,10 @@ def main():
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
-    t_setup = time.perf_counter()
+
+    def elapsed():
+        return time.perf_counter() - t_start
+
     w = W.build(args.getter)
     seeds = W.seeds_for(w, args.seeds, placement=args.placement)
     n = len(seeds)
@@ -277,8 +322,8 @@ def main():
     stream = dt.stream()
     ev = HipEvents()
     e = [ev.create() for _ in range(4)]
-    print(f"[bench] setup {time.perf_counter() - t_setup:.1f}s getter={args.getter} seeds={n}", file=sys.stderr,
-          flush=True)
+    setup_s = elapsed()
+    print(f"[bench] setup {setup_s:.1f}s getter={args.getter} seeds={n}", file=sys.stderr, flush=True)
 
     # Phase 1 + scan once to size the streamline buffers (same seeds every step).
     dt.phase1_async(sd.ptr, n, CHUNK, cd.ptr, dd.ptr)
@@ -301,48 +346,83 @@ def main():
         dt.synchronize()
         return ev.ms(e[0], e[1]), ev.ms(e[1], e[2]), ev.ms(e[2], e[3]), ev.ms(e[0], e[3])
 
-    for i in range(args.warmup):
+    learned_leg = args.getter == "ptt" and args.proposal != "plain"
+    cpu_reserve = 0.0 if args.no_cpu else CPU_RESERVE_S
+    # Warm-up: the first pass loads the code objects; more only while they
+    # take under a tenth of the budget.
+    n_warm = 0
+    t_pass = None
+    while n_warm < max(1, args.warmup):
+        t0 = time.perf_counter()
         r = step()
-        print(f"[bench] warmup {i}: {r[3] / 1e3:.2f}s", file=sys.stderr, flush=True)
+        t_pass = time.perf_counter() - t0
+        n_warm += 1
+        print(f"[bench] warmup {n_warm - 1}: {r[3] / 1e3:.2f}s", file=sys.stderr, flush=True)
+        if n_warm >= args.warmup or elapsed() - setup_s + t_pass > 0.1 * args.budget:
+            break
     dt.synchronize()
+    # Passes that fit: the plain passes, then (PTT) one counting pass and the
+    # learned leg (training ~30 s + LEARNED_PASSES passes), then the CPU legs.
+    after = cpu_reserve + 10.0 + (t_pass if args.getter == "ptt" else 0.0)
+    if learned_leg:
+        after += 45.0 + LEARNED_PASSES * t_pass
+    fit = int((args.budget - elapsed() - after) // max(t_pass, 1e-3))
+    n_steps = max(min(args.steps, 3), min(args.steps, fit))
     times = []
     t0 = time.perf_counter()
-    for i in range(args.steps):
+    for i in range(n_steps):
         r = step()
         times.append(r)
         print(f"[bench] step {i}: phase1 {r[0]:.1f} ms scan {r[1]:.2f} ms phase2 {r[2]:.1f} ms", file=sys.stderr,
               flush=True)
     wall = time.perf_counter() - t0
     lens = sld.download(np.zeros(nsl, dtype=np.int32)).astype(np.int64)
+    steps_per_pass = int((lens - 1).clip(min=0).sum())
+    t_step = np.array([t[3] for t in times]) / 1e3
+    t_p2 = np.array([t[2] for t in times]) / 1e3
+    med_step = float(np.median(t_step))
+    med_p2 = float(np.median(t_p2))
+    ref_bytes_step = alg_bytes_per_step(args.getter, w.params)
+    counted = None
+    if args.getter == "ptt":
+        # one untimed pass of the instrumented getter (same streamlines)
+        dt.set_tap_count(True)
+        dt.tap_count(reset=True)
+        step()
+        taps, subs = (int(x) for x in dt.tap_count(reset=True))
+        dt.set_tap_count(False)
+        lens_c = sld.download(np.zeros(nsl, dtype=np.int32)).astype(np.int64)
+        counted = {"taps": taps, "substeps": subs, "same_streamlines": bool(np.array_equal(lens_c, lens)),
+                   "taps_per_substep": taps / max(subs, 1),
+                   "reference_taps_per_substep": 1 + 24 * 8 + 32 * 8}
+        alg_launch = ptt_alg_bytes(taps, subs, int(lens.sum()))
+    else:
+        alg_launch = ref_bytes_step * steps_per_pass
+    achieved = alg_launch / med_p2 / 1e9
+    print(f"[bench] plain: {steps_per_pass / med_step / 1e6:.2f} M steps/s, {achieved:.0f} GB/s algorithmic",
+          file=sys.stderr, flush=True)
     learned = None
-    if args.getter == "ptt" and args.proposal != "plain":
+    if learned_leg:
         try:
             learned = measure_learned(args, w, dt, step, lens, sld, sl, nsl, L)
         except Exception as exc:  # the plain line is still reported
             print(f"[bench] learned proposal failed: {exc!r}", file=sys.stderr, flush=True)
             learned = {"error": repr(exc)}
-    steps_per_pass = int((lens - 1).clip(min=0).sum())
     end_to_end = None
     if args.end_to_end:
         for b in (ssd, sld, sl):
             b.free()
         ssd = sld = sl = None
         end_to_end = measure_end_to_end(args, w, seeds, steps_per_pass)
-    t_step = np.array([t[3] for t in times]) / 1e3
-    t_p2 = np.array([t[2] for t in times]) / 1e3
-    ms_per_step = float(t_step.mean() * 1e3)
-    value = steps_per_pass / float(t_step.mean())
-    abytes = alg_bytes_per_step(args.getter, w.params)
-    achieved = abytes * steps_per_pass / float(t_p2.mean()) / 1e9
 
     res = {
         "metric": "streamline steps/sec at 1M seeds, plain vs learned-proposal PTT, 1 MI355X",
-        "value": value,
+        "value": steps_per_pass / med_step,
         "unit": "steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "steps": n_steps,
+        "warmup": n_warm,
+        "ms_per_step": med_step * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -354,10 +434,16 @@ def main():
             "seeds": n,
             "streamlines": nsl,
             "points": int(lens.sum()),
-            "streamlines_per_s": nsl / float(t_step.mean()),
-            "phase1_ms": float(np.mean([t[0] for t in times])),
-            "scan_ms": float(np.mean([t[1] for t in times])),
-            "phase2_ms": float(t_p2.mean() * 1e3),
+            "streamlines_per_s": nsl / med_step,
+            "phase1_ms": float(np.median([t[0] for t in times])),
+            "scan_ms": float(np.median([t[1] for t in times])),
+            "phase2_ms": med_p2 * 1e3,
+            "pass_ms_all": [float(x * 1e3) for x in t_step],
+            "timing": "median of the timed passes (HIP events on the tracker's stream)",
+            "requested_steps": args.steps,
+            "requested_warmup": args.warmup,
+            "budget_s": args.budget,
+            "setup_s": setup_s,
             "wall_s": wall,
             "proposal": "plain",
         },
@@ -369,10 +455,15 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
             "kernel": f"phase2_queue_kernel<{MODEL_IDS[args.getter]}, 0>",
-            "alg_bytes_per_step": abytes,
-            "alg_bytes_per_launch": abytes * steps_per_pass,
+            "alg_bytes_per_step": alg_launch / max(steps_per_pass, 1),
+            "alg_bytes_per_launch": alg_launch,
         },
     }
+    if counted is not None:
+        res["roofline"]["alg_bytes_source"] = "taps counted by an instrumented pass of the same seeds"
+        res["roofline"]["counted"] = counted
+        res["roofline"]["reference_work_bytes_per_step"] = ref_bytes_step
+        res["roofline"]["reference_work_frac"] = ref_bytes_step * steps_per_pass / med_p2 / 1e9 / HBM_PEAK_GBS
     if end_to_end is not None:
         res["end_to_end"] = end_to_end
     if learned is not None:
@@ -389,12 +480,16 @@ def main():
         if b is not None:
             b.free()
     dt.close()
+    print("[bench] preliminary: " + json.dumps(res), file=sys.stderr, flush=True)
     if not args.no_cpu:
         ns = args.cpu_sample or CPU_SAMPLE[args.getter]
-        print(f"[bench] cpu baseline on {ns} seeds", file=sys.stderr, flush=True)
-        res["cpu_baseline"] = cpu_baseline(w, seeds, ns)
+        ns1 = max(1, ns * CPU_SAMPLE_1T[args.getter] // CPU_SAMPLE[args.getter]) if args.cpu_sample \
+            else CPU_SAMPLE_1T[args.getter]
+        print(f"[bench] cpu baseline on {ns} seeds (all threads) and {ns1} (one thread)", file=sys.stderr, flush=True)
+        res["cpu_baseline"] = cpu_baseline(w, seeds, ns, ns1)
     else:
         res["cpu_baseline"] = None
+    res["config"]["total_s"] = elapsed()
     print(json.dumps(res), flush=True)
 
 

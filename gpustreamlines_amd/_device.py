@@ -145,6 +145,16 @@ class DeviceTracker:
         _lib.check(self.L.cuslines_proposal_stats(self.h, out, int(reset)), "proposal_stats")
         return np.array(list(out), dtype=np.int64)
 
+    def set_tap_count(self, enable: bool = True):
+        """PTT phase 2 counts its ODF taps and sub-steps (same results)."""
+        _lib.check(self.L.cuslines_set_tap_count(self.h, int(bool(enable))), "set_tap_count")
+
+    def tap_count(self, reset: bool = True) -> np.ndarray:
+        """[taps, sub-steps] counted since the last reset."""
+        out = (C.c_int64 * 2)()
+        _lib.check(self.L.cuslines_tap_count(self.h, out, int(reset)), "tap_count")
+        return np.array(list(out), dtype=np.int64)
+
     def mlp_workspace_bytes(self, n: int) -> int:
         b = C.c_size_t(0)
         _lib.check(self.L.cuslines_mlp_workspace_bytes(int(n), C.byref(b)), "mlp_workspace_bytes")

@@ -103,6 +103,8 @@ def load():
         "cuslines_set_proposal": [vp, vp],
         "cuslines_set_recording": [vp, vp, i32, vp, C.c_float],
         "cuslines_proposal_stats": [vp, C.POINTER(C.c_int64), C.c_int],
+        "cuslines_set_tap_count": [vp, C.c_int],
+        "cuslines_tap_count": [vp, C.POINTER(C.c_int64), C.c_int],
         "cuslines_mlp_workspace_bytes": [i32, C.POINTER(sz)],
         "cuslines_mlp_loss_grad_async": [vp, vp, vp, i32, vp, vp, vp],
         "cuslines_mlp_adam_async": [vp, vp, vp, vp, vp, i32, C.c_float, C.c_float, C.c_float, C.c_float],

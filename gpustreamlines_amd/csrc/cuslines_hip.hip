@@ -868,10 +868,11 @@ __device__ __forceinline__ void probing_frame(const float *frame, Philox &rng, f
 // all >= 0 (the ODF is clipped at 0 and linear filtering keeps the sign),
 // so the running sum never decreases: once it reaches `stop_at` the
 // candidate passes whatever the remaining taps add, and the lane stops
-// probing.  Callers that need the value itself pass +infinity.
+// probing.  Callers that need the value itself pass +infinity.  `ntap`
+// (instrumented kernels only; nullptr otherwise) counts the taps read.
 __device__ __forceinline__ float data_support(const DevParams &P, const DevBufs &B, float support,
                                               float px, float py, float pz, float k1, float k2, float *pf,
-                                              float stop_at) {
+                                              float stop_at, uint32_t *ntap = nullptr) {
     float prop[9], direc[3];
     prepare_propagator(k1, k2, P.probe_step, prop);
     propagate_frame(prop, pf, direc);
@@ -879,6 +880,7 @@ __device__ __forceinline__ float data_support(const DevParams &P, const DevBufs 
     uint32_t idx = B.lut[lut_index(P, pf[0], pf[1], pf[2])];
     for (int i = 0; i < P.probe_quality; ++i) {
         if (support >= stop_at) break;
+        if (ntap) ++*ntap;
         const Tap t = tap_begin(P, B, idx, px, py, pz);
         if (i + 1 < P.probe_quality) {
             propagate_frame(prop, pf, direc);
@@ -930,10 +932,11 @@ __device__ __forceinline__ float ptt_face_cdf(PttLds &S) {
 // direction, 0 when no vertex passes, the CDF is empty or no round has a
 // winner.  `wk` (optional) receives the winner's curvature, `raw` (optional,
 // LDS [24]) the vertex supports before the 0.4 cut.
-template <bool INIT>
+template <bool INIT, bool CNT = false>
 __device__ int ptt_substep(const DevParams &P, const DevBufs &B, PttLds &S, float *frame, const float *fr,
                            Philox &rng, float first_val, int first_phase, int rounds, float dx, float dy, float dz,
-                           float px, float py, float pz, float *ndx, float *ndy, float *ndz, float *wk, float *raw) {
+                           float px, float py, float pz, float *ndx, float *ndy, float *ndz, float *wk, float *raw,
+                           uint32_t *ntap = nullptr) {
     const int l = lane32();
     const double mc = (double)P.max_curv;
     float last = 0.0f;
@@ -979,7 +982,7 @@ __device__ int ptt_substep(const DevParams &P, const DevBufs &B, PttLds &S, floa
             probing_frame<INIT>(fr, rng, pf);
             // rejection rounds of a non-initial sub-step only test sup >= min_support_f
             const float stop_at = (ph >= 1 && !INIT && P.ptt_nonneg) ? P.min_support_f : __builtin_inff();
-            sup = data_support(P, B, first_val, px, py, pz, k1, k2, pf, stop_at);
+            sup = data_support(P, B, first_val, px, py, pz, k1, k2, pf, stop_at, CNT ? ntap : nullptr);
         }
         if (ph == 0) {
             const bool ok = active && !(sup < P.min_support_f);
@@ -995,9 +998,11 @@ __device__ int ptt_substep(const DevParams &P, const DevBufs &B, PttLds &S, floa
         if (INIT) {
             float mx = sup;
             for (int m = 16; m; m >>= 1) mx = fmaxf(mx, shfl_xor32(mx, m));
-            msk = half_ballot(sup == mx && !(mx < P.min_support_f));
+            msk = half_ballot(sup == mx && mx >= P.min_support_f);
         } else {
-            msk = half_ballot(!(sup < P.min_support_f));
+            // >= as ptt.cu:384: a NaN support never wins a round (the vertex
+            // phase above keeps the reference's `<` test, which lets it in)
+            msk = half_ballot(sup >= P.min_support_f);
         }
         if (msk != 0u) {
             const int win = __ffs(msk) - 1;
@@ -1027,10 +1032,12 @@ __device__ int ptt_substep(const DevParams &P, const DevBufs &B, PttLds &S, floa
 }
 
 // get_direction_ptt_d (ptt.cu:194-412); `frame` is the 9-float frame in use.
-template <bool INIT>
+// CNT (instrumented kernels): every tap read is counted in *ntap.
+template <bool INIT, bool CNT = false>
 __device__ int get_direction_ptt(const DevParams &P, const DevBufs &B, PttLds &S, float *frame,
                                  Philox &rng, float dx, float dy, float dz, float px, float py, float pz,
-                                 float *ndx, float *ndy, float *ndz, float *wk = nullptr, float *raw = nullptr) {
+                                 float *ndx, float *ndy, float *ndz, float *wk = nullptr, float *raw = nullptr,
+                                 uint32_t *ntap = nullptr) {
     const int l = lane32();
     if (INIT) {
         if (l == 0) { frame[0] = dx; frame[1] = dy; frame[2] = dz; }
@@ -1040,20 +1047,23 @@ __device__ int get_direction_ptt(const DevParams &P, const DevBufs &B, PttLds &S
 #pragma unroll
     for (int i = 0; i < 9; ++i) fr[i] = frame[i];
     const float first_val = interp4(P, B, px, py, pz, fr[0], fr[1], fr[2]);
-    return ptt_substep<INIT>(P, B, S, frame, fr, rng, first_val, 0, 1024 / 32, dx, dy, dz, px, py, pz, ndx, ndy, ndz,
-                             wk, raw);
+    if (CNT) ++*ntap;
+    return ptt_substep<INIT, CNT>(P, B, S, frame, fr, rng, first_val, 0, 1024 / 32, dx, dy, dz, px, py, pz, ndx, ndy,
+                                  ndz, wk, raw, ntap);
 }
 
 // init_frame_ptt_d (ptt.cu:415-474); frame[15..17] is set to the binormal.
+template <bool CNT = false>
 __device__ bool init_frame_ptt(const DevParams &P, const DevBufs &B, PttLds &S, Philox &rng,
-                               float fx, float fy, float fz, float sx, float sy, float sz) {
+                               float fx, float fy, float fz, float sx, float sy, float sz, uint32_t *ntap = nullptr) {
     const int l = lane32();
     float tx, ty, tz;
     int att = 0;
     bool ok = false;
     for (; att < 2; ++att) {  // +dir first, then -dir
         const float sg = att ? -1.0f : 1.0f;
-        ok = get_direction_ptt<true>(P, B, S, S.frame, rng, sg * fx, sg * fy, sg * fz, sx, sy, sz, &tx, &ty, &tz);
+        ok = get_direction_ptt<true, CNT>(P, B, S, S.frame, rng, sg * fx, sg * fy, sg * fz, sx, sy, sz, &tx, &ty, &tz,
+                                          nullptr, nullptr, ntap);
         if (ok) break;
     }
     if (!ok) return false;
@@ -1291,11 +1301,13 @@ struct SlotLds {
 };
 
 // V (PTT only): 0 plain getter, 1 learned proposal, 2 plain getter recording
-// training pairs (tag identifies seed, direction and pass for the hash).
+// training pairs (tag identifies seed, direction and pass for the hash), 3
+// plain getter counting its taps and sub-steps into *ntap and *nsub.
 template <int MODEL, int V = 0>
 __device__ int tracker(const DevParams &P, const DevBufs &B, const float *sv, SlotLds &S, Philox &rng,
                        float sx, float sy, float sz, float dx, float dy, float dz, float *frame,
-                       float *__restrict__ sl, uint32_t tag = 0) {
+                       float *__restrict__ sl, uint32_t tag = 0, uint32_t *ntap = nullptr,
+                       uint32_t *nsub = nullptr) {
     const int l = lane32();
     const int step_frac = (MODEL == CUSLINES_PTT) ? P.step_frac : 1;
     const int MAXL = P.max_sline_len;
@@ -1329,6 +1341,10 @@ __device__ int tracker(const DevParams &P, const DevBufs &B, const float *sv, Sl
             ndir = get_direction_ptt<false>(P, B, S.ptt, frame, rng, dx, dy, dz, px, py, pz, &nx, &ny, &nz, nullptr,
                                             rec ? S.ptt.act + ACT_O : nullptr);
             if (rec) record_pair(P, B, S.ptt, px, py, pz, dx, dy, dz, fr0, S.ptt.act + ACT_O);
+        } else if (MODEL == CUSLINES_PTT && V == 3) {
+            ++*nsub;
+            ndir = get_direction_ptt<false, true>(P, B, S.ptt, frame, rng, dx, dy, dz, px, py, pz, &nx, &ny, &nz,
+                                                  nullptr, nullptr, ntap);
         } else if (MODEL == CUSLINES_PTT)
             ndir = get_direction_ptt<false>(P, B, S.ptt, frame, rng, dx, dy, dz, px, py, pz, &nx, &ny, &nz);
         else
@@ -1616,7 +1632,7 @@ __device__ int get_direction_ptt_g(const DevParams &P, const DevBufs &B, const P
                     for (int i = 0; i < 9; ++i) wpf[i] = pf[i];
                 }
             } else {
-                const uint32_t m = G::ballot(!(sup < P.min_support_f));
+                const uint32_t m = G::ballot(sup >= P.min_support_f);
                 if (m != 0u) {
                     win = b * L + (__ffs(m) - 1);
                     if (v == win) {
@@ -1636,7 +1652,7 @@ __device__ int get_direction_ptt_g(const DevParams &P, const DevBufs &B, const P
                 if (om > m || (om == m && ov < mv)) { m = om; mv = ov; }
             }
             WAVE_SYNC();
-            if (!(m < P.min_support_f)) {
+            if (m >= P.min_support_f) {
                 if (bestv == mv) {
 #pragma unroll
                     for (int i = 0; i < 9; ++i) frame[i] = wpf[i];
@@ -1946,13 +1962,14 @@ __device__ void track_seed(const DevParams &P, const DevBufs &B, const float *sv
     const int ndir = offsets[s + 1] - offsets[s];
     int off = offsets[s];
     const size_t slot_f = (size_t)2 * P.max_sline_len * 3;
+    uint32_t ntap = 0, nsub = 0;  // V == 3: this lane's taps, the seed's sub-steps
     for (int i = 0; i < ndir; ++i, ++off) {
         const int vi = dirs[(size_t)s * P.nverts + i];
         const float fx = B.verts[3 * vi], fy = B.verts[3 * vi + 1], fz = B.verts[3 * vi + 2];
         float *cur = sline + (size_t)off * slot_f;
         if (l == 0) sline_seed[off] = s;
         if (MODEL == CUSLINES_PTT) {
-            if (!init_frame_ptt(P, B, S.ptt, rng, fx, fy, fz, sx, sy, sz)) {
+            if (!init_frame_ptt<V == 3>(P, B, S.ptt, rng, fx, fy, fz, sx, sy, sz, &ntap)) {
                 if (l == 0) {
                     sline_len[off] = 1;
                     cur[0] = sx; cur[1] = sy; cur[2] = sz;
@@ -1966,7 +1983,7 @@ __device__ void track_seed(const DevParams &P, const DevBufs &B, const float *sv
             float *frame = (MODEL == CUSLINES_PTT) ? S.ptt.frame + 9 * pass : nullptr;
             float *out = pass ? cur + 3 * (size_t)(nb - 1) : cur;
             const int ns = tracker<MODEL, V>(P, B, sv, S, rng, sx, sy, sz, sg * fx, sg * fy, sg * fz, frame, out,
-                                             ((uint32_t)s * 64u + (uint32_t)i) * 2u + (uint32_t)pass);
+                                             ((uint32_t)s * 64u + (uint32_t)i) * 2u + (uint32_t)pass, &ntap, &nsub);
             if (pass == 0) {
                 nb = ns;
                 MEM_SYNC();
@@ -1982,6 +1999,14 @@ __device__ void track_seed(const DevParams &P, const DevBufs &B, const float *sv
             }
         }
         if (l == 0) sline_len[off] = nb - 1 + nf;
+    }
+    if (V == 3 && B.stats) {
+        unsigned long long taps = ntap;
+        for (int m = 16; m; m >>= 1) taps += __shfl_xor(taps, m, 32);
+        if (l == 0) {
+            atomicAdd(B.stats, taps);
+            atomicAdd(B.stats + 1, (unsigned long long)nsub);
+        }
     }
 }
 
@@ -2316,6 +2341,14 @@ __global__ void ptt_normalise(float *__restrict__ odf, const float *__restrict__
         for (int t = 0; t < T; ++t) o[t] = o[t] / mx;
 }
 
+// Sets *flag when some value of x[0..n) is not >= 0 (negative or NaN).
+__global__ void flag_not_nonneg(const float *__restrict__ x, long long n, int *__restrict__ flag) {
+    bool bad = false;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        bad |= !(x[i] >= 0.0f);
+    if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+}
+
 // Nearest sphere vertex of every point of the linspace(-1, 1, res)^3 grid
 // ([i0][i1][i2], i2 fastest), as PttDirectionGetter.prepare_data's KDTree
 // query (cu_direction_getters.py:601-623): grid values i * (2 / (res - 1)) - 1
@@ -2487,6 +2520,7 @@ struct cuslines_tracker {
     float *d_fprop = nullptr;            // learned proposal: feature propagators [24][9]
     uint16_t *d_mlp = nullptr;           // learned proposal (nullptr: plain getter)
     unsigned long long *d_stats = nullptr;
+    int count_taps = 0;  // PTT phase 2 counts its taps and sub-steps into d_stats
     float *rec = nullptr;                // caller's training-pair buffer (nullptr: no recording)
     int *rec_count = nullptr;
     bool has_data = false, has_stop = false, has_sphere = false, has_boot = false, has_lut = false;
@@ -2670,6 +2704,8 @@ static int launch_phase2(cuslines_tracker *t, const float *seeds, int n, int chu
             return launch_phase2_v<MODEL, 1>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl);
         if (t->rec)
             return launch_phase2_v<MODEL, 2>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl);
+        if (t->count_taps)
+            return launch_phase2_v<MODEL, 3>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl);
         if (t->ptt_lanes > 0 && t->schedule != CUSLINES_SCHED_STATIC)
             return launch_ptt_groups(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl);
     }
@@ -2876,7 +2912,9 @@ int cuslines_set_data(cuslines_tracker *t, const float *dataf) {
         // The early stop of rejection-round probes (data_support) relies on
         // taps >= 0; the reference's prepare_data clips the ODF at 0.
         const size_t nval = nvox * (size_t)d.dimt;
-        const int nonneg = std::none_of(dataf, dataf + nval, [](float v) { return v < 0.0f; }) ? 1 : 0;
+        // a NaN counts as negative: a NaN tap makes the full support NaN,
+        // which fails the acceptance test, so no early stop may skip it
+        const int nonneg = std::none_of(dataf, dataf + nval, [](float v) { return !(v >= 0.0f); }) ? 1 : 0;
         float *tmp = nullptr;
         HIPC(hipMalloc(&tmp, nval * sizeof(float)));
         hipError_t e = hipMemcpy(tmp, dataf, nval * sizeof(float), hipMemcpyHostToDevice);
@@ -3152,8 +3190,28 @@ int cuslines_ptt_prepare(cuslines_tracker *t, const float *odf, const float *sto
         dfree(raw);
         return fail(CUSLINES_EHIP, std::string("PTT preparation: ") + hipGetErrorString(e));
     }
-    // the clipped ODF has no negative value
-    int rc = ptt_take_data(t, raw, 1);
+    // the clipped ODF has no negative value, but NaN inputs stay NaN
+    int nonneg = 0;
+    {
+        int *flag = nullptr;
+        e = hipMalloc(&flag, sizeof(int));
+        if (e == hipSuccess) e = hipMemsetAsync(flag, 0, sizeof(int), t->stream);
+        if (e == hipSuccess) {
+            const size_t nb = std::min<size_t>((nval + 255) / 256, 65536);
+            hipLaunchKernelGGL(flag_not_nonneg, dim3((unsigned)nb), dim3(256), 0, t->stream, raw, (long long)nval, flag);
+            e = hipGetLastError();
+        }
+        int hf = 1;
+        if (e == hipSuccess) e = hipMemcpyAsync(&hf, flag, sizeof(int), hipMemcpyDeviceToHost, t->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+        dfree(flag);
+        if (e != hipSuccess) {
+            dfree(raw);
+            return fail(CUSLINES_EHIP, std::string("PTT preparation: ") + hipGetErrorString(e));
+        }
+        nonneg = hf ? 0 : 1;
+    }
+    int rc = ptt_take_data(t, raw, nonneg);
     dfree(raw);
     if (rc) return rc;
     rc = cuslines_set_ptt_lut(t, hl.data());
@@ -3235,6 +3293,33 @@ int cuslines_proposal_stats(cuslines_tracker *t, int64_t *out3, int reset) {
         if (reset) HIPC(hipMemset(t->d_stats, 0, sizeof(h)));
     }
     for (int i = 0; i < 3; ++i) out3[i] = (int64_t)h[i];
+    return CUSLINES_OK;
+}
+
+int cuslines_set_tap_count(cuslines_tracker *t, int enable) {
+    if (!t) return fail(CUSLINES_EINVAL, "null tracker");
+    if (t->p.model_type != CUSLINES_PTT) return fail(CUSLINES_EINVAL, "tap counting is for PTT");
+    HIPC(hipSetDevice(t->device));
+    if (enable && !t->d_stats) {
+        HIPC(hipMalloc(&t->d_stats, 4 * sizeof(unsigned long long)));
+        HIPC(hipMemset(t->d_stats, 0, 4 * sizeof(unsigned long long)));
+    }
+    t->count_taps = enable ? 1 : 0;
+    return CUSLINES_OK;
+}
+
+int cuslines_tap_count(cuslines_tracker *t, int64_t *out2, int reset) {
+    if (!t || !out2) return fail(CUSLINES_EINVAL, "null argument");
+    if (t->d_mlp) return fail(CUSLINES_ESTATE, "the counters hold the learned proposal's statistics");
+    HIPC(hipSetDevice(t->device));
+    unsigned long long h[4] = {0, 0, 0, 0};
+    if (t->d_stats) {
+        HIPC(hipStreamSynchronize(t->stream));
+        HIPC(hipMemcpy(h, t->d_stats, sizeof(h), hipMemcpyDeviceToHost));
+        if (reset) HIPC(hipMemset(t->d_stats, 0, sizeof(h)));
+    }
+    out2[0] = (int64_t)h[0];
+    out2[1] = (int64_t)h[1];
     return CUSLINES_OK;
 }
 

@@ -218,6 +218,16 @@ int cuslines_set_recording(cuslines_tracker *t, float *pairs_d, int32_t capacity
  * proposal.  reset != 0 zeroes them after reading. */
 int cuslines_proposal_stats(cuslines_tracker *t, int64_t *out3, int reset);
 
+/* Instrumentation of PTT phase 2 (no counterpart in the reference): while
+ * enabled, phase 2 runs an instrumented copy of the plain getter (same
+ * results) that counts every ODF tap it reads (one LUT entry + one 8-texel
+ * block each) and every sub-step.  cuslines_tap_count returns out[0] taps and
+ * out[1] sub-steps since the last reset (reset != 0 zeroes them); it is
+ * refused while a learned proposal is set, whose statistics share the
+ * counters.  Used to state the bytes the kernel must move per step. */
+int cuslines_set_tap_count(cuslines_tracker *t, int enable);
+int cuslines_tap_count(cuslines_tracker *t, int64_t *out2, int reset);
+
 /* Training of the proposal network on device buffers (no counterpart in the
  * reference).  params_d: float32 master parameters [CUSLINES_MLP_NPARAM];
  * the forward pass rounds them to bf16 as inference does.

@@ -462,8 +462,26 @@ struct SeedScratch {
 // ---------------------------------------------------------------------------
 // Probabilistic getter (generate_streamlines_cuda.cu:49-275)
 // ---------------------------------------------------------------------------
+// Inverse-CDF selection (generate_streamlines_cuda.cu:184-201): bisection of
+// the inclusive prefix sum down to a window of at most 32 entries, then the
+// first entry of the window whose CDF exceeds sel (lane ballot + ffs).  An
+// empty ballot gives low - 1 (index -1 clamped to 0, the reference reads
+// pmf[-1] there).
+int prob_select(const float *cdf, int n, float sel) {
+    int low = 0, high = n - 1;
+    while ((high - low) >= LANES) {
+        const int mid = (low + high) / 2;
+        if (cdf[mid] < sel) low = mid; else high = mid;
+    }
+    int ind = low - 1;
+    for (int t = 0; t < LANES; ++t)
+        if (low + t <= high && sel < cdf[low + t]) { ind = low + t; break; }
+    if (ind < 0) ind = 0;
+    return ind;
+}
+
 int get_direction_prob(const Problem &P, SeedScratch &S, bool is_start, const float *dir,
-                       const float *pt, float *new_dir, int *peaks_out) {
+                       const float *pt, float *new_dir, int *peaks_out, int *ind_out = nullptr) {
     const int n = P.p.nverts;
     float *pmf = S.a.data();
     if (trilinear(P, P.dataf.data(), P.p.dimt, pt, pmf) != 0) return 0;
@@ -482,15 +500,8 @@ int get_direction_prob(const Problem &P, SeedScratch &S, bool is_start, const fl
     const float last = pmf[n - 1];
     if (last == 0.0f) return 0;
     const float sel = S.rng[0].uniform() * last;
-    int low = 0, high = n - 1;
-    while ((high - low) >= LANES) {
-        const int mid = (low + high) / 2;
-        if (pmf[mid] < sel) low = mid; else high = mid;
-    }
-    int ind = low - 1;
-    for (int t = 0; t < LANES; ++t)
-        if (low + t <= high && sel < pmf[low + t]) { ind = low + t; break; }
-    if (ind < 0) ind = 0;
+    const int ind = prob_select(pmf, n, sel);
+    if (ind_out) *ind_out = ind;
     const float *v = V + 3 * ind;
     if (dot3(dir, v) > 0.0f) { new_dir[0] = v[0]; new_dir[1] = v[1]; new_dir[2] = v[2]; }
     else { new_dir[0] = -v[0]; new_dir[1] = -v[1]; new_dir[2] = -v[2]; }
@@ -785,8 +796,17 @@ constexpr int DISC_VERT_CNT = 24, DISC_FACE_CNT = 31;
 // Face CDF over the 31 disc faces from vert_pdf and up to `rounds` rounds
 // of 32 candidates (ptt.cu:270-410).  1 with new_dir (and the frame
 // advanced), 0 when the CDF is empty or no round has a winner.
+// What one sub-step drew and measured (for the oracle's own distribution
+// tests): the last round's 32 candidate curvatures and supports, the winning
+// lane (-1: none) and the round it won in.
+struct PttTrace {
+    float k[2 * LANES];
+    float sup[LANES];
+    int win = -1, round = -1;
+};
+
 int ptt_sample(const Problem &P, SeedScratch &S, bool init, const float *dir, float *frame, const float *pos,
-               float first_val, const float *vert_pdf, int rounds, float *new_dir) {
+               float first_val, const float *vert_pdf, int rounds, float *new_dir, PttTrace *tr = nullptr) {
     const double min_support = (double)P.p.probe_quality * PMF_THRESHOLD_P;
     float cdf[DISC_FACE_CNT];
     for (int ii = 0; ii < DISC_FACE_CNT; ++ii) {
@@ -823,6 +843,10 @@ int ptt_sample(const Problem &P, SeedScratch &S, bool init, const float *dir, fl
             sup[t] = data_support(P, first_val, pos, k1s[t], k2s[t], pfs[t]);
         }
         int win = -1;
+        if (tr) {
+            for (int t = 0; t < LANES; ++t) { tr->k[2 * t] = k1s[t]; tr->k[2 * t + 1] = k2s[t]; tr->sup[t] = sup[t]; }
+            tr->round = round;
+        }
         if (init) {
             float mx = sup[0];
             for (int t = 1; t < LANES; ++t) mx = std::fmax(mx, sup[t]);
@@ -832,6 +856,7 @@ int ptt_sample(const Problem &P, SeedScratch &S, bool init, const float *dir, fl
             for (int t = 0; t < LANES; ++t) if ((double)sup[t] >= min_support) { win = t; break; }
         }
         if (win != -1) {
+            if (tr) tr->win = win;
             if (init) {
                 new_dir[0] = dir[0]; new_dir[1] = dir[1]; new_dir[2] = dir[2];
                 for (int i = 0; i < 9; ++i) frame[i] = pfs[win][i];
@@ -870,12 +895,12 @@ bool ptt_vertex_supports(const Problem &P, SeedScratch &S, bool init, const floa
 }
 
 int get_direction_ptt(const Problem &P, SeedScratch &S, bool init, const float *dir, float *frame,
-                      const float *pos, float *new_dir, float *raw = nullptr) {
+                      const float *pos, float *new_dir, float *raw = nullptr, PttTrace *tr = nullptr) {
     if (init) { frame[0] = dir[0]; frame[1] = dir[1]; frame[2] = dir[2]; }
     const float first_val = interp4(P, pos, frame);
     float vert_pdf[DISC_VERT_CNT];
     if (!ptt_vertex_supports(P, S, init, frame, pos, first_val, vert_pdf, raw)) return 0;
-    return ptt_sample(P, S, init, dir, frame, pos, first_val, vert_pdf, 1024 / LANES, new_dir);
+    return ptt_sample(P, S, init, dir, frame, pos, first_val, vert_pdf, 1024 / LANES, new_dir, tr);
 }
 
 bool init_frame_ptt(const Problem &P, SeedScratch &S, const float *first_step, const float *seed) {
@@ -1292,6 +1317,87 @@ void oracle_prepare_propagator(float k1, float k2, float arclen, float *p) {
     prepare_propagator(k1, k2, arclen, p);
 }
 void oracle_propagate_frame(const float *p, float *f, float *d) { propagate_frame(p, f, d); }
+// --- sampler entry points for the oracle's distribution tests ------------
+int oracle_prob_select(const float *cdf, int n, float sel) { return prob_select(cdf, n, sel); }
+
+// The prob getter's step draw (non-initial) at point pt with incoming
+// direction dir, on nstreams independent lane-0 streams (subsequences
+// sub0, sub0 + 1, ...): idx_out[k] = chosen vertex or -1 (no direction),
+// dirs_out[3k..] the returned direction.
+int oracle_prob_sample(oracle_problem *h, const float *pt, const float *dir, int nstreams, long long sub0,
+                       int32_t *idx_out, float *dirs_out) {
+    const Problem &P = *reinterpret_cast<Problem *>(h);
+    if (P.p.model_type != CUSLINES_PROB) return 1;
+#pragma omp parallel
+    {
+        SeedScratch S;
+        S.size(P);
+#pragma omp for schedule(static)
+        for (int k = 0; k < nstreams; ++k) {
+            S.rng[0].init(P.p.rng_seed, (uint64_t)(sub0 + k));
+            int ind = -1;
+            float nd[3] = {0, 0, 0};
+            if (!get_direction_prob(P, S, false, dir, pt, nd, nullptr, &ind)) ind = -1;
+            idx_out[k] = ind;
+            for (int a = 0; a < 3; ++a) dirs_out[3 * k + a] = nd[a];
+        }
+    }
+    return 0;
+}
+
+// One PTT sub-step at pos with the 9-float frame (init: frame[0..2] is the
+// direction, the rest is drawn), lanes on subsequences sub0 + lane.  Returns
+// the getter's result; trace: k[64] candidate curvatures, sup[32] supports
+// of the deciding round, then win, round; vert_pdf[24] the vertex values
+// after the 0.4 cut; frame_out[9] the frame afterwards.
+int oracle_ptt_substep(oracle_problem *h, const float *pos, const float *frame, int init, long long sub0,
+                       float *k_out, float *sup_out, int32_t *win_round, float *vert_pdf, float *frame_out) {
+    const Problem &P = *reinterpret_cast<Problem *>(h);
+    if (P.p.model_type != CUSLINES_PTT) return -1;
+    SeedScratch S;
+    S.size(P);
+    for (int l = 0; l < LANES; ++l) S.rng[l].init(P.p.rng_seed, (uint64_t)(sub0 + l));
+    float fr[18];
+    for (int i = 0; i < 9; ++i) fr[i] = frame[i];
+    const float first_val = interp4(P, pos, fr);
+    float vp[DISC_VERT_CNT];
+    PttTrace tr;
+    int ok = 0;
+    if (init) fr[0] = frame[0], fr[1] = frame[1], fr[2] = frame[2];
+    if (ptt_vertex_supports(P, S, init != 0, fr, pos, first_val, vp, nullptr)) {
+        float nd[3];
+        ok = ptt_sample(P, S, init != 0, frame, fr, pos, first_val, vp, 1024 / LANES, nd, &tr);
+    }
+    for (int v = 0; v < DISC_VERT_CNT; ++v) vert_pdf[v] = vp[v];
+    for (int t = 0; t < 2 * LANES; ++t) k_out[t] = tr.k[t];
+    for (int t = 0; t < LANES; ++t) sup_out[t] = tr.sup[t];
+    win_round[0] = tr.win;
+    win_round[1] = tr.round;
+    for (int i = 0; i < 9; ++i) frame_out[i] = fr[i];
+    return ok;
+}
+
+// init_frame_ptt_d for one seed and first step on lanes sub0 + lane:
+// returns 1 and the 18-float frame pair, or 0.
+int oracle_ptt_init_frame(oracle_problem *h, const float *seed, const float *first_step, long long sub0,
+                          float *frame_out) {
+    const Problem &P = *reinterpret_cast<Problem *>(h);
+    if (P.p.model_type != CUSLINES_PTT) return -1;
+    SeedScratch S;
+    S.size(P);
+    for (int l = 0; l < LANES; ++l) S.rng[l].init(P.p.rng_seed, (uint64_t)(sub0 + l));
+    const bool ok = init_frame_ptt(P, S, first_step, seed);
+    for (int i = 0; i < 18; ++i) frame_out[i] = S.frame[i];
+    return ok ? 1 : 0;
+}
+
+void oracle_set_num_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
 int oracle_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

@@ -71,6 +71,15 @@ def lib():
         L.oracle_prepare_propagator.argtypes = [C.c_float, C.c_float, C.c_float, fp]
         L.oracle_propagate_frame.argtypes = [fp, fp, fp]
         L.oracle_num_threads.restype = C.c_int
+        L.oracle_set_num_threads.argtypes = [C.c_int]
+        L.oracle_prob_select.restype = C.c_int
+        L.oracle_prob_select.argtypes = [fp, C.c_int, C.c_float]
+        L.oracle_prob_sample.restype = C.c_int
+        L.oracle_prob_sample.argtypes = [vp, fp, fp, C.c_int, C.c_longlong, ip, fp]
+        L.oracle_ptt_substep.restype = C.c_int
+        L.oracle_ptt_substep.argtypes = [vp, fp, fp, C.c_int, C.c_longlong, fp, fp, ip, fp, fp]
+        L.oracle_ptt_init_frame.restype = C.c_int
+        L.oracle_ptt_init_frame.argtypes = [vp, fp, fp, C.c_longlong, fp]
         L.oracle_mlp_forward.argtypes = [C.POINTER(C.c_uint16), fp, fp]
         L.oracle_mlp_features.argtypes = [vp, fp, fp, fp, fp]
         L.oracle_vertex_supports.argtypes = [vp, fp, fp, fp]
@@ -211,6 +220,35 @@ class Oracle:
         rv = self.L.oracle_boot_odf(self.h, _f(pt), int(slot), int(phase), _f(odf), _f(sig))
         return rv, odf, sig
 
+    def prob_sample(self, pt, direction, nstreams, sub0=0):
+        """Non-initial prob draws at ``pt`` with incoming ``direction`` on
+        ``nstreams`` lane-0 streams: (vertex index or -1, direction) each."""
+        idx = np.zeros(nstreams, np.int32)
+        dirs = np.zeros((nstreams, 3), np.float32)
+        rc = self.L.oracle_prob_sample(self.h, _f(np.asarray(pt, np.float32)), _f(np.asarray(direction, np.float32)),
+                                       int(nstreams), int(sub0), _i(idx), _f(dirs))
+        assert rc == 0
+        return idx, dirs
+
+    def ptt_substep(self, pos, frame, init=False, sub0=0):
+        """One PTT sub-step; returns (ok, dict(k [32,2], sup [32], win, round,
+        vert_pdf [24], frame [9]))."""
+        k = np.zeros(64, np.float32)
+        sup = np.zeros(32, np.float32)
+        wr = np.zeros(2, np.int32)
+        vp = np.zeros(24, np.float32)
+        fo = np.zeros(9, np.float32)
+        fr = np.ascontiguousarray(frame, np.float32).reshape(9)
+        ok = self.L.oracle_ptt_substep(self.h, _f(np.asarray(pos, np.float32)), _f(fr), int(bool(init)), int(sub0),
+                                       _f(k), _f(sup), _i(wr), _f(vp), _f(fo))
+        return ok, dict(k=k.reshape(32, 2), sup=sup, win=int(wr[0]), round=int(wr[1]), vert_pdf=vp, frame=fo)
+
+    def ptt_init_frame(self, seed, first_step, sub0=0):
+        fo = np.zeros(18, np.float32)
+        ok = self.L.oracle_ptt_init_frame(self.h, _f(np.asarray(seed, np.float32)),
+                                          _f(np.asarray(first_step, np.float32)), int(sub0), _f(fo))
+        return ok, fo
+
     def interp4(self, pos, frame):
         pos = np.ascontiguousarray(pos, dtype=np.float32)
         frame = np.ascontiguousarray(frame, dtype=np.float32)
@@ -265,6 +303,11 @@ def warp_sort_dec(keys, vals):
     return k, v
 
 
+def prob_select(cdf, sel):
+    c = np.ascontiguousarray(cdf, np.float32)
+    return lib().oracle_prob_select(_f(c), len(c), float(sel))
+
+
 def prefix_sum(a):
     a = np.ascontiguousarray(a, dtype=np.float32).copy()
     lib().oracle_prefix_sum(_f(a), len(a))
@@ -287,3 +330,8 @@ def propagate_frame(prop, frame):
 
 def num_threads():
     return lib().oracle_num_threads()
+
+
+def set_num_threads(n: int):
+    """OpenMP threads of later oracle calls (in this process)."""
+    lib().oracle_set_num_threads(int(n))

@@ -1,0 +1,149 @@
+"""GPU parity at the benchmark workloads' own sizes (81x106x76 phantom, 160 DWI
++ 10 b0, CSA ODF on the 724-vertex sphere, step 0.25 voxel, random streams
+numbered per 25,000-seed chunk):
+
+- the 100,000-seed case of every getter, bit-exact against the CPU oracle
+  (prob at the full 100,000 seeds; bootstrap and PTT on the first seeds of
+  the same seed list, as many as the oracle tracks in seconds);
+- the 1M-seed uneven-length case (half the seeds in FA > 0.2, half in
+  0.05-0.15 border voxels): size-independent properties of the whole output
+  and bit-exactness of the first 2,000 seeds, whose chunk-relative streams
+  do not depend on the seeds after them;
+- NaN ODF voxels in PTT data and the tap-counting instrumented kernel.
+"""
+
+import numpy as np
+import pytest
+
+from conftest import packed_points
+
+pytestmark = pytest.mark.gpu
+
+CHUNK = 25000
+# Seeds compared with the oracle at the full volume (the oracle's time on
+# 16 host threads: about 4 s prob, 10 s CSA/OPDT, 15 s PTT).
+FULL_SEEDS = {"prob": 100_000, "csa": 8_000, "opdt": 10_000, "ptt": 2_000}
+
+
+def _full(kind):
+    from gpustreamlines_amd import workloads as W
+
+    return W.build(kind)
+
+
+def _gpu(w, seeds, rng_chunk=CHUNK, keep_dirs=True, taps=False):
+    from gpustreamlines_amd._device import DeviceTracker
+
+    dt = DeviceTracker(w.params, device=0)
+    try:
+        dt.upload(w.dataf, w.stop_map, w.vertices, w.edges, boot=w.boot, ptt_lut=w.ptt_lut)
+        if taps:
+            dt.set_tap_count(True)
+        g = dt.track(seeds, rng_chunk=rng_chunk, keep_dirs=keep_dirs)
+        if taps:
+            g["taps"] = dt.tap_count(reset=True)
+    finally:
+        dt.close()
+    return g
+
+
+def _oracle(w, seeds, rng_chunk=CHUNK):
+    from oracle.oracle import Oracle
+
+    return Oracle(w.params, w.dataf, w.stop_map, w.vertices, w.edges, boot=w.boot,
+                  ptt_lut=w.ptt_lut).track(seeds, rng_chunk=rng_chunk)
+
+
+def _assert_same(g, o, n=None, dirs=True):
+    """Bit-identity of the first n seeds' outputs (all seeds when n is None)."""
+    n = len(o["counts"]) if n is None else n
+    assert np.array_equal(g["counts"][:n], o["counts"][:n]), "phase-1 direction counts differ"
+    if dirs:
+        nv = g["dirs"].shape[1]
+        mask = np.arange(nv)[None, :] < o["counts"][:n, None]
+        assert np.array_equal(np.where(mask, g["dirs"][:n], -1), np.where(mask, o["dirs"][:n, :nv], -1)), \
+            "peak vertex indices differ"
+    nsl = int(o["offsets"][n])
+    assert np.array_equal(g["offsets"][: n + 1], o["offsets"][: n + 1])
+    assert np.array_equal(g["sline_seed"][:nsl], o["sline_seed"][:nsl]), "streamline seed indices differ"
+    bad = np.nonzero(g["sline_len"][:nsl] != o["sline_len"][:nsl])[0]
+    assert len(bad) == 0, f"{len(bad)} of {nsl} streamline lengths differ, first {bad[:5]}"
+    npts = int(g["point_offsets"][nsl])
+    assert np.array_equal(g["points"][:npts], packed_points(o)), "streamline points differ"
+
+
+@pytest.mark.parametrize("kind", ["prob", "csa", "opdt", "ptt"])
+def test_full_volume_bit_exact(kind):
+    from gpustreamlines_amd import workloads as W
+
+    w = _full(kind)
+    seeds = W.seeds_for(w, 100_000, placement="mask")[: FULL_SEEDS[kind]]
+    print(f"[workloads] {kind} 81x106x76 seeds={len(seeds)} chunk={CHUNK}", flush=True)
+    g = _gpu(w, seeds)
+    o = _oracle(w, seeds)
+    assert len(o["sline_len"]) > len(seeds)
+    _assert_same(g, o)
+
+
+@pytest.mark.parametrize("kind", ["csa", "ptt"])
+def test_million_mixed_seeds_properties(kind):
+    from gpustreamlines_amd import workloads as W
+
+    w = _full(kind)
+    seeds = W.seeds_for(w, 1_000_000, placement="mixed")
+    print(f"[workloads] {kind} 1M mixed seeds", flush=True)
+    g = _gpu(w, seeds, keep_dirs=False)
+    counts, lens = g["counts"].astype(np.int64), g["sline_len"].astype(np.int64)
+    L = w.params["max_sline_len"]
+    # every phase-1 direction becomes exactly one streamline, in seed order
+    assert len(lens) == counts.sum() == g["offsets"][-1]
+    assert np.array_equal(np.repeat(np.arange(len(seeds)), counts), g["sline_seed"])
+    assert lens.min() >= 1 and lens.max() <= 2 * L - 1
+    # lengths from 1 to hundreds of points: the uneven-length workload
+    assert (lens == 1).any() and lens.max() > 300
+    pts = g["points"]
+    assert len(pts) == lens.sum() and np.isfinite(pts).all()
+    # the stop map reads -1 only where every texel of the tap lies outside
+    # (p < -1 or p >= dim on an axis), so a point can pass the edge by at
+    # most that much plus the step that reached it
+    dims = np.array([w.params[k] for k in ("dimx", "dimy", "dimz")], dtype=np.float32)
+    step = np.float32(w.params["step_size"])
+    assert (pts >= -1.0 - step).all() and (pts <= dims + step).all(), "a point left the volume"
+    # the first 2,000 seeds (chunk 0) are bit-exact with the oracle run on them alone
+    o = _oracle(w, seeds[:2000])
+    _assert_same(g, o, n=2000, dirs=False)
+
+
+def test_ptt_nan_voxels_bit_exact(small_workloads):
+    """NaN ODF values stay NaN through the reference's preparation; a probe
+    that touches one has a NaN support, which the vertex phase keeps (its
+    test is `support < 0.4`) and the rejection rounds refuse (`>= 0.4`,
+    ptt.cu:384).  The early stop of rejection rounds is off for such data."""
+    from gpustreamlines_amd import workloads as W
+
+    w = small_workloads["ptt"]
+    data = w.dataf.copy()
+    data[14:18, 14:20, 10:16, :] = np.nan
+    data[8:10, 20:22, 12:14, ::7] = np.nan
+    wn = W.Workload(w.kind, dict(w.params), data, w.stop_map, w.vertices, w.edges, ptt_lut=w.ptt_lut, extra=w.extra)
+    seeds = W.seeds_for(w, 64, placement="mask", seed=21)
+    g = _gpu(wn, seeds, rng_chunk=0)
+    o = _oracle(wn, seeds, rng_chunk=0)
+    _assert_same(g, o)
+
+
+def test_tap_counting_pass_same_output(small_workloads):
+    """The instrumented PTT kernel gives the plain getter's output, and every
+    non-initial sub-step reads at least the vertex phase (1 + 24 x 8 taps)."""
+    from gpustreamlines_amd import workloads as W
+
+    w = small_workloads["ptt"]
+    seeds = W.seeds_for(w, 48, placement="mixed", seed=5)
+    g = _gpu(w, seeds, rng_chunk=0, taps=True)
+    o = _oracle(w, seeds, rng_chunk=0)
+    _assert_same(g, o)
+    taps, subs = (int(x) for x in g["taps"])
+    assert subs > 0
+    assert taps >= 193 * subs
+    # an upper bound: every sub-step at its 32 full rounds, plus the init steps
+    assert taps <= (1 + 24 * 8 + 32 * 32 * 8) * (subs + 2 * 32 * len(o["sline_len"]))
