@@ -33,23 +33,23 @@ CHUNK = 25000
 def alg_bytes_per_step(kind: str, p: dict) -> float:
     """Compulsory HBM bytes per output step with no cross-step reuse: the
     8-corner gather of the data channels the getter reads, the 8-texel stop
-    map tap and the 12-byte point store.  For PTT this is the reference's
-    work per step (every probe tap of one full rejection round, 1 + 24*8 +
-    32*8 taps of one 32-byte block + a 4-byte LUT entry, per sub-step); the
-    bytes the kernel must move are counted by an instrumented pass instead
-    (ptt_alg_bytes)."""
+    map tap and the 12-byte point store.  For PTT, the taps every sub-step
+    reads whatever the data: the tangent tap and the vertex phase's 24
+    curves x 8 probes (1 + 24*8 taps of one 32-byte corner block and a 4-byte
+    LUT entry each), per sub-step; the rejection rounds' taps depend on the
+    data and are left out (ptt_reference_bytes_per_step counts one full round)."""
     stop_tap = 8 * 4
     if kind == "ptt":
-        taps = 1 + 24 * 8 + 32 * 8
+        taps = 1 + 24 * 8
         return p["step_frac"] * (taps * (8 * 4 + 4) + stop_tap) + 12
     return 8 * p["dimt"] * 4 + stop_tap + 12
 
 
-def ptt_alg_bytes(taps: int, substeps: int, points: int) -> int:
-    """Bytes a PTT pass must read and write: each tap the getter reads (one
-    32-byte corner block + its 4-byte LUT entry), one 8-texel stop-map tap per
-    sub-step and the 12-byte store of each output point."""
-    return 36 * taps + 32 * substeps + 12 * points
+def ptt_reference_bytes_per_step(p: dict) -> float:
+    """The reference's PTT work per output step: the vertex phase and one
+    full rejection round of 32 candidates x 8 probes, every tap read."""
+    taps = 1 + 24 * 8 + 32 * 8
+    return p["step_frac"] * (taps * (8 * 4 + 4) + 8 * 4) + 12
 
 
 MODEL_IDS = {"opdt": 0, "csa": 1, "prob": 2, "ptt": 3}
@@ -285,6 +285,10 @@ def main():
     ap.add_argument("--placement", default="mixed", choices=["mixed", "mask"])
     ap.add_argument("--cpu-sample", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--boot-mode", default="mfma", choices=["mfma", "exact"],
+                    help="bootstrap getters: batched MFMA kernel or the bit-exact per-seed kernel")
+    ap.add_argument("--boot-occ", type=int, default=0, choices=[0, 2, 4],
+                    help="MFMA bootstrap kernel variant: waves per SIMD (0 = library default)")
     ap.add_argument("--proposal", default="both", choices=["plain", "learned", "both"],
                     help="PTT: time the plain getter, the learned proposal, or both (value = plain)")
     ap.add_argument("--end-to-end", action="store_true",
@@ -314,6 +318,9 @@ def main():
     L = w.params["max_sline_len"]
     dt = DeviceTracker(w.params, device=0)
     dt.upload(w.dataf, w.stop_map, w.vertices, w.edges, boot=w.boot, ptt_lut=w.ptt_lut)
+    boot = args.getter in ("csa", "opdt")
+    if boot:
+        dt.set_boot_mode(dt.BOOT_MFMA if args.boot_mode == "mfma" else dt.BOOT_EXACT, args.boot_occ)
     sd = dt.alloc(seeds.nbytes)
     sd.upload(seeds)
     cd = dt.alloc(4 * (n + 1))
@@ -361,9 +368,9 @@ def main():
         if n_warm >= args.warmup or elapsed() - setup_s + t_pass > 0.1 * args.budget:
             break
     dt.synchronize()
-    # Passes that fit: the plain passes, then (PTT) one counting pass and the
-    # learned leg (training ~30 s + LEARNED_PASSES passes), then the CPU legs.
-    after = cpu_reserve + 10.0 + (t_pass if args.getter == "ptt" else 0.0)
+    # Passes that fit: the plain passes, then (PTT) the learned leg (training
+    # ~30 s + LEARNED_PASSES passes), then the CPU legs.
+    after = cpu_reserve + 10.0
     if learned_leg:
         after += 45.0 + LEARNED_PASSES * t_pass
     fit = int((args.budget - elapsed() - after) // max(t_pass, 1e-3))
@@ -382,22 +389,7 @@ def main():
     t_p2 = np.array([t[2] for t in times]) / 1e3
     med_step = float(np.median(t_step))
     med_p2 = float(np.median(t_p2))
-    ref_bytes_step = alg_bytes_per_step(args.getter, w.params)
-    counted = None
-    if args.getter == "ptt":
-        # one untimed pass of the instrumented getter (same streamlines)
-        dt.set_tap_count(True)
-        dt.tap_count(reset=True)
-        step()
-        taps, subs = (int(x) for x in dt.tap_count(reset=True))
-        dt.set_tap_count(False)
-        lens_c = sld.download(np.zeros(nsl, dtype=np.int32)).astype(np.int64)
-        counted = {"taps": taps, "substeps": subs, "same_streamlines": bool(np.array_equal(lens_c, lens)),
-                   "taps_per_substep": taps / max(subs, 1),
-                   "reference_taps_per_substep": 1 + 24 * 8 + 32 * 8}
-        alg_launch = ptt_alg_bytes(taps, subs, int(lens.sum()))
-    else:
-        alg_launch = ref_bytes_step * steps_per_pass
+    alg_launch = alg_bytes_per_step(args.getter, w.params) * steps_per_pass
     achieved = alg_launch / med_p2 / 1e9
     print(f"[bench] plain: {steps_per_pass / med_step / 1e6:.2f} M steps/s, {achieved:.0f} GB/s algorithmic",
           file=sys.stderr, flush=True)
@@ -446,6 +438,7 @@ def main():
             "setup_s": setup_s,
             "wall_s": wall,
             "proposal": "plain",
+            "boot_mode": args.boot_mode if boot else None,
         },
         "roofline": {
             "bound": "hbm",
@@ -454,16 +447,18 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": f"phase2_queue_kernel<{MODEL_IDS[args.getter]}, 0>",
+            "kernel": (f"phase2_boot_batch_kernel<{MODEL_IDS[args.getter]}, {args.boot_occ or 2}>"
+                       if boot and args.boot_mode == "mfma" else f"phase2_queue_kernel<{MODEL_IDS[args.getter]}, 0>"),
             "alg_bytes_per_step": alg_launch / max(steps_per_pass, 1),
             "alg_bytes_per_launch": alg_launch,
         },
     }
-    if counted is not None:
-        res["roofline"]["alg_bytes_source"] = "taps counted by an instrumented pass of the same seeds"
-        res["roofline"]["counted"] = counted
-        res["roofline"]["reference_work_bytes_per_step"] = ref_bytes_step
-        res["roofline"]["reference_work_frac"] = ref_bytes_step * steps_per_pass / med_p2 / 1e9 / HBM_PEAK_GBS
+    if args.getter == "ptt":
+        ref_step = ptt_reference_bytes_per_step(w.params)
+        res["roofline"]["alg_bytes_source"] = ("taps every sub-step reads (tangent tap + 24 vertex curves x 8 "
+                                               "probes); rejection-round taps left out")
+        res["roofline"]["reference_work_bytes_per_step"] = ref_step
+        res["roofline"]["reference_work_frac"] = ref_step * steps_per_pass / med_p2 / 1e9 / HBM_PEAK_GBS
     if end_to_end is not None:
         res["end_to_end"] = end_to_end
     if learned is not None:

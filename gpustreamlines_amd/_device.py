@@ -145,6 +145,22 @@ class DeviceTracker:
         _lib.check(self.L.cuslines_proposal_stats(self.h, out, int(reset)), "proposal_stats")
         return np.array(list(out), dtype=np.int64)
 
+    def proposal_counters(self, reset: bool = True) -> np.ndarray:
+        """proposal_stats plus [3]: learned face draws that broke the
+        envelope of the exact correction."""
+        out = (C.c_int64 * 4)()
+        _lib.check(self.L.cuslines_proposal_counters(self.h, out, int(reset)), "proposal_counters")
+        return np.array(list(out), dtype=np.int64)
+
+    BOOT_EXACT, BOOT_MFMA = 0, 1
+
+    def set_boot_mode(self, mode: int, waves_per_simd: int = 0):
+        """Bootstrap phase-2 kernel: BOOT_EXACT (bit-identical to the CPU
+        oracle) or BOOT_MFMA (16 seeds in lockstep, matrix products on the
+        matrix cores; equal within float rounding).  waves_per_simd (2 or 4,
+        0 = default) picks the MFMA kernel's occupancy variant."""
+        _lib.check(self.L.cuslines_set_boot_mode(self.h, int(mode) | (int(waves_per_simd) << 8)), "set_boot_mode")
+
     def set_tap_count(self, enable: bool = True):
         """PTT phase 2 counts its ODF taps and sub-steps (same results)."""
         _lib.check(self.L.cuslines_set_tap_count(self.h, int(bool(enable))), "set_tap_count")

@@ -100,9 +100,11 @@ def load():
         "cuslines_set_launch": [vp, C.c_int, C.c_int, C.c_int],
         "cuslines_set_ptt_layout": [vp, C.c_int],
         "cuslines_set_lanes_per_seed": [vp, C.c_int],
+        "cuslines_set_boot_mode": [vp, C.c_int],
         "cuslines_set_proposal": [vp, vp],
         "cuslines_set_recording": [vp, vp, i32, vp, C.c_float],
         "cuslines_proposal_stats": [vp, C.POINTER(C.c_int64), C.c_int],
+        "cuslines_proposal_counters": [vp, C.POINTER(C.c_int64), C.c_int],
         "cuslines_set_tap_count": [vp, C.c_int],
         "cuslines_tap_count": [vp, C.POINTER(C.c_int64), C.c_int],
         "cuslines_mlp_workspace_bytes": [i32, C.POINTER(sz)],

@@ -18,7 +18,7 @@ ARCH = os.environ.get("CUSLINES_OFFLOAD_ARCH", "gfx950")
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall",
+    "-Wall", "-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx", "-Wl,-rpath,/opt/rocm/lib",
 ]
 
 

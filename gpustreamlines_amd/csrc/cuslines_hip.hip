@@ -20,10 +20,13 @@
 
 #include "../../include/cuslines_hip.h"
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -96,6 +99,7 @@ struct DevParams {
     int dimx, dimy, dimz, dimt, t32;
     int nverts, nv32, nedges, sphere_symm, max_sline_len, model;
     int hr, hr32, delta_nr;
+    int hr16, dn16, nv16;  // batched bootstrap: hr, delta_nr, nverts rounded up to 16
     uint32_t seed_lo, seed_hi;
     float tc_threshold, rel_peak, cos_max_angle, cos_min_sep, min_signal;
     double step_d;
@@ -116,6 +120,7 @@ struct DevBufs {
     const uint32_t *edges;
     const uint16_t *nbr;  // [nverts][8] neighbours, padded with the vertex itself (nullptr: edge path)
     const float *H, *R, *db, *dq, *smat;
+    const float *hr_mat, *fit_mat, *odf_mat;  // batched bootstrap: [H;R], [dq|-db], smat, zero padded
     const int *b0;
     const int *dwi_idx;
     const uint32_t *lut;  // PTT LUT entry: vertex | (direction rank << 16)
@@ -1232,16 +1237,41 @@ __device__ __noinline__ void mlp_forward(const uint16_t *__restrict__ W_generic,
     WAVE_SYNC();
 }
 
-// A non-initial PTT sub-step with the learned proposal.  First attempt
-// (when `learned`): the prediction, cut at 0.4 like a measured support,
-// builds the face CDF and one round of 32 candidates runs under the
-// reference's acceptance rule.  If the prediction leaves no face or no
-// candidate passes, *fell is set and the plain getter runs the sub-step on
-// the same streams (vertex supports measured, up to 32 rounds).  One call
-// site per phase keeps the kernel's code and registers small.
+// Learned proposal with an exact correction.  The reference draws a face
+// with probability proportional to its value F (the sum of its 3 vertex
+// supports, 0 when one is below 0.4) and a point uniform in it, then keeps
+// the lowest lane whose candidate has support >= 0.4 (ptt.cu:270-410).  Here
+// the face is drawn from the network's estimate instead,
+//   g(f) = sum of the 3 predicted supports (clipped at 0) + a floor
+//          (PROPOSAL_FLOOR x the mean predicted face value),
+// and accepted with probability F(f) / (PROPOSAL_M g(f)) before the
+// reference's support test; F(f) needs only the exact supports of the drawn
+// faces' vertices (8 probes each, measured once per sub-step and shared by
+// the lanes).  An accepted candidate is then distributed as F(f) x uniform
+// x [support >= 0.4], the reference's sampler, whenever F <= PROPOSAL_M g
+// on every face; draws where that bound fails are counted (stats[3]).
+// Up to LEARNED_ROUNDS rounds of 32 candidates, then the plain sub-step.
+constexpr float PROPOSAL_M = 2.0f;
+constexpr float PROPOSAL_FLOOR = 0.5f;
+constexpr int LEARNED_ROUNDS = 4;
+
+// Inclusive prefix sum over the 31 faces (lanes >= 31 keep 0), the
+// Hillis-Steele order of ptt_face_cdf; returns the total on every lane.
+__device__ __forceinline__ float face_scan(float v, float *out) {
+    const int l = lane32();
+    if (l >= DISC_FACE_CNT) v = 0.0f;
+#pragma unroll
+    for (int i = 1; i < 32; i <<= 1) {
+        const float o = shfl_up32(v, i);
+        if (l >= i && l < DISC_FACE_CNT) v = v + o;
+    }
+    if (out && l < DISC_FACE_CNT) out[l] = v;
+    return shfl32(v, DISC_FACE_CNT - 1);
+}
+
 __device__ int get_direction_ptt_learned(const DevParams &P, const DevBufs &B, PttLds &S, float *frame, Philox &rng,
                                          bool learned, float dx, float dy, float dz, float px, float py, float pz,
-                                         float *ndx, float *ndy, float *ndz, int *fell) {
+                                         float *ndx, float *ndy, float *ndz, int *fell, uint32_t *misses) {
     const int l = lane32();
     float fr[9];
 #pragma unroll
@@ -1251,21 +1281,103 @@ __device__ int get_direction_ptt_learned(const DevParams &P, const DevBufs &B, P
     if (learned) {
         mlp_features(P, B, px, py, pz, dx, dy, dz, fr, first_val, S.act + ACT_X);
         mlp_forward(B.mlp, S.act);
-        if (l < DISC_VERT_CNT) {
-            const float sup = S.act[ACT_O + l];
-            S.vert_pdf[l] = (sup < P.min_support_f) ? 0.0f : sup;
+        float *E = S.act + ACT_X;   // exact vertex supports (the inputs are dead now)
+        float *G = S.act + ACT_H1;  // proposal face values (so are the hidden activations)
+        float gh = 0.0f;
+        if (l < DISC_FACE_CNT) {
+#pragma unroll
+            for (int jj = 0; jj < 3; ++jj) {
+                const float sv = S.act[ACT_O + c_disc_face[l * 3 + jj]];
+                gh = gh + ((sv > 0.0f) ? sv : 0.0f);
+            }
         }
+        const float tot = face_scan(gh, nullptr);
+        const float g = gh + (PROPOSAL_FLOOR * (tot / (float)DISC_FACE_CNT) + 1e-3f);
+        if (l < DISC_FACE_CNT) G[l] = g;
+        const float last = face_scan(g, S.face_cdf);
         WAVE_SYNC();
-    }
-    // learned: one round from the predicted supports, then (fell back) the
-    // plain sub-step; the loop keeps a single ptt_substep call site
-    for (int attempt = learned ? 0 : 1; attempt < 2; ++attempt) {
-        const int r = ptt_substep<false>(P, B, S, frame, fr, rng, first_val, attempt ? 0 : 1, attempt ? 1024 / 32 : 1,
-                                         dx, dy, dz, px, py, pz, ndx, ndy, ndz, nullptr, nullptr);
-        if (r || attempt) return r;
+        const double mc = (double)P.max_curv;
+        uint32_t known = 0u;
+        for (int round = 0; round < LEARNED_ROUNDS; ++round) {
+            float r1 = rng.uniform(), r2 = rng.uniform();
+            if (r1 + r2 > 1.0f) { r1 = 1.0f - r1; r2 = 1.0f - r2; }
+            const float tmp = rng.uniform() * last;
+            const float ua = rng.uniform();
+            int jj = 0, hi = DISC_FACE_CNT - 1;
+            while (jj < hi) {
+                const int mid = (jj + hi) >> 1;
+                if (S.face_cdf[mid] >= tmp) hi = mid; else jj = mid + 1;
+            }
+            const int f0 = c_disc_face[jj * 3], f1 = c_disc_face[jj * 3 + 1], f2 = c_disc_face[jj * 3 + 2];
+            uint32_t need = (1u << f0) | (1u << f1) | (1u << f2);
+#pragma unroll
+            for (int m = 16; m; m >>= 1) need |= (uint32_t)shfl_xor32((int)need, m);
+            const uint32_t todo = need & ~known;
+            if (l < DISC_VERT_CNT && ((todo >> l) & 1u)) {
+                float pf[9];
+#pragma unroll
+                for (int i = 0; i < 9; ++i) pf[i] = fr[i];
+                const float k1v = (float)((double)c_disc_vert[2 * l] * mc);
+                const float k2v = (float)((double)c_disc_vert[2 * l + 1] * mc);
+                E[l] = data_support(P, B, first_val, px, py, pz, k1v, k2v, pf, __builtin_inff());
+            }
+            known |= need;
+            WAVE_SYNC();
+            // the reference's face value from the exact supports (vertex
+            // phase test `< 0.4`, ptt_face_cdf's zero rule)
+            float F = 0.0f;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const float e = E[c_disc_face[jj * 3 + q]];
+                const float vv = (e < P.min_support_f) ? 0.0f : e;
+                if (vv == 0.0f) { F = 0.0f; break; }
+                F = F + vv;
+            }
+            const float mg = PROPOSAL_M * G[jj];
+            const bool acc = ua * mg < F;
+            const float vx0 = (float)(mc * (double)c_disc_vert[f0 * 2]);
+            const float vx1 = (float)(mc * (double)c_disc_vert[f1 * 2]);
+            const float vx2 = (float)(mc * (double)c_disc_vert[f2 * 2]);
+            const float vy0 = (float)(mc * (double)c_disc_vert[f0 * 2 + 1]);
+            const float vy1 = (float)(mc * (double)c_disc_vert[f1 * 2 + 1]);
+            const float vy2 = (float)(mc * (double)c_disc_vert[f2 * 2 + 1]);
+            const float k1 = __builtin_fmaf(r2, vx2 - vx0, __builtin_fmaf(r1, vx1 - vx0, vx0));
+            const float k2 = __builtin_fmaf(r2, vy2 - vy0, __builtin_fmaf(r1, vy1 - vy0, vy0));
+            float sup = 0.0f;
+            if (acc) {
+                float pf[9];
+#pragma unroll
+                for (int i = 0; i < 9; ++i) pf[i] = fr[i];
+                const float stop_at = P.ptt_nonneg ? P.min_support_f : __builtin_inff();
+                sup = data_support(P, B, first_val, px, py, pz, k1, k2, pf, stop_at);
+            }
+            *misses += (uint32_t)__popc(half_ballot(F > mg));
+            const uint32_t msk = half_ballot(acc && sup >= P.min_support_f);
+            if (msk != 0u) {
+                const int win = __ffs(msk) - 1;
+                float ox = dx, oy = dy, oz = dz;
+                if (l == win) {
+                    float prop[9], d[3], pf[9];
+                    prepare_propagator(k1, k2, P.ptt_sub_step, prop);
+#pragma unroll
+                    for (int i = 0; i < 9; ++i) pf[i] = fr[i];
+                    propagate_frame(prop, pf, d);
+                    norm3<0>(d[0], d[1], d[2]);
+                    ox = d[0]; oy = d[1]; oz = d[2];
+#pragma unroll
+                    for (int i = 0; i < 9; ++i) frame[i] = pf[i];
+                }
+                *ndx = shfl32(ox, win);
+                *ndy = shfl32(oy, win);
+                *ndz = shfl32(oz, win);
+                WAVE_SYNC();
+                return 1;
+            }
+        }
         *fell = 1;
     }
-    return 0;
+    return ptt_substep<false>(P, B, S, frame, fr, rng, first_val, 0, 1024 / 32, dx, dy, dz, px, py, pz, ndx, ndy,
+                              ndz, nullptr, nullptr);
 }
 
 // Training pair of a plain sub-step: inputs at the sub-step's start and the
@@ -1315,6 +1427,7 @@ __device__ int tracker(const DevParams &P, const DevBufs &B, const float *sv, Sl
     float px = sx, py = sy, pz = sz;
     if (l == 0) { sl[0] = px; sl[1] = py; sl[2] = pz; }
     int ltries = 0, lfalls = 0, nplain = 0;
+    uint32_t lmiss = 0;
     bool learned = (V == 1);
     int i;
     for (i = 1; i < MAXL * step_frac; ++i) {
@@ -1325,7 +1438,7 @@ __device__ int tracker(const DevParams &P, const DevBufs &B, const float *sv, Sl
         else if (MODEL == CUSLINES_PTT && V == 1) {
             int fell = 0;
             ndir = get_direction_ptt_learned(P, B, S.ptt, frame, rng, learned, dx, dy, dz, px, py, pz, &nx, &ny, &nz,
-                                             &fell);
+                                             &fell, &lmiss);
             if (learned) {
                 ++ltries;
                 lfalls += fell;
@@ -1366,6 +1479,7 @@ __device__ int tracker(const DevParams &P, const DevBufs &B, const float *sv, Sl
         atomicAdd(B.stats, (unsigned long long)(ltries - lfalls));
         atomicAdd(B.stats + 1, (unsigned long long)lfalls);
         atomicAdd(B.stats + 2, (unsigned long long)nplain);
+        atomicAdd(B.stats + 3, (unsigned long long)lmiss);
     }
     int nsteps = i / step_frac;
     if ((i % step_frac) != 0 && i < step_frac * (MAXL - 1)) {
@@ -2102,9 +2216,337 @@ __global__ void __launch_bounds__((MODEL == CUSLINES_PTT && V == 1) ? 512 : 256,
     }
 }
 
+// -------------------------------------------------------------------------
+// Bootstrap phase 2, batched on the matrix cores.
+//
+// Same algorithm as the scalar path above (get_direction_boot_d,
+// boot.cu:393-608; tracker_boot_d, boot.cu:720-824; genStreamlinesMergeBoot_k,
+// boot.cu:826-980), organised for MFMA: a 512-thread workgroup holds
+// BB_SLOTS = 16 seed slots, one per 32-lane half-wave, that advance in
+// lockstep -- every round, each active slot makes one getter attempt.  The
+// per-slot work (gather, resampling, normalisation, peaks, closest peak, the
+// step and its stop check) runs on the slot's own 32 lanes with the scalar
+// path's code; the matrix products of the attempt run batched over the 16
+// slots on v_mfma_f32_16x16x4_f32 (fp32 operands, fp32 accumulation):
+//   phase B  [H; R] (2 hr16 x hr16) . X (hr16 x 16)    -> H.d, R.d
+//   phase D  [dq | -db] (dn16 x 2 hr16) . [xf; msk]    -> model coefficients
+//   phase E  S (nv16 x dn16) . C (dn16 x 16)           -> ODF on the sphere
+// Each matrix element read from L2 thus serves 16 seeds instead of one.  The
+// products sum in another order than the reference's lane-strided dot
+// products, so results agree with the scalar path within float rounding:
+// peak indices are the same except at near-ties, and a streamline is
+// identical up to the first step whose peak choice flips.
+// Matrices (uploaded zero padded, row-major): hr_mat [2 hr16][hr16],
+// fit_mat [dn16][2 hr16], odf_mat [nv16][dn16].  LDS per slot: the scalar
+// slot's regions, the stride padded to 4 mod 32 floats so the 16 slots' rows
+// sit in distinct LDS banks for the MFMA operand reads and result writes.
+// -------------------------------------------------------------------------
+constexpr int BB_SLOTS = 16, BB_THREADS = 512, BB_WAVES = BB_THREADS / 64;
+constexpr int BB_MAX_ATTEMPTS = 5;  // NATTEMPTS of tracker_boot_d (boot.cu:757)
+
+typedef float bb_v4 __attribute__((ext_vector_type(4)));
+
+__host__ __device__ inline int bb_slot_stride(int nv32, int t32, int hr32) {
+    return ((slot_floats(CUSLINES_CSA, false, nv32, t32, hr32) + 27) & ~31) + 4;
+}
+
+// Tiles t = wave, wave + 8, ... (t < ntiles) of Y = A . X over the 16 slots.
+// A: global row-major [ntiles * 16][lda], K (<= lda) a multiple of 16, zero
+// padded.  Slot n's operand is X[n * ss + k] (k < K), or X2[n * ss + k - ksplit]
+// for k >= ksplit; its result rows go to Y[n * ss + row], or Y2[n * ss + row -
+// ysplit] for row >= ysplit, rows >= ymax dropped.  Lane l of a wave supplies
+// A[l % 16][k] and X_{l % 16}[k] of each step and holds Y rows 4 (l / 16) + r;
+// within a 16-wide chunk of K, lane group q = l / 16 reads the contiguous k =
+// 16 c + 4 q + (0..3) as one 16-byte load and MFMA step s takes component s.
+template <int MAXT, bool CSA_ROW0 = false>
+__device__ __forceinline__ void bb_gemm(const float *__restrict__ A, int lda, int K, int ntiles, const float *X,
+                                        const float *X2, int ksplit, float *Y, float *Y2, int ysplit, int ymax,
+                                        int ss) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int n = lane & 15, q = lane >> 4;
+    bb_v4 acc[MAXT];
+#pragma unroll
+    for (int i = 0; i < MAXT; ++i) acc[i] = bb_v4{0.0f, 0.0f, 0.0f, 0.0f};
+    const float *ap = A + (size_t)n * lda + 4 * q;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        const float *xs = (k0 < ksplit) ? X + n * ss + k0 : X2 + n * ss + (k0 - ksplit);
+        const float4 b = *reinterpret_cast<const float4 *>(xs + 4 * q);
+        float4 a[MAXT];
+#pragma unroll
+        for (int i = 0; i < MAXT; ++i) {
+            const int t = w + BB_WAVES * i;
+            if (t < ntiles) a[i] = *reinterpret_cast<const float4 *>(ap + (size_t)t * 16 * lda + k0);
+        }
+#pragma unroll
+        for (int i = 0; i < MAXT; ++i) {
+            const int t = w + BB_WAVES * i;
+            if (t < ntiles) {
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i].x, b.x, acc[i], 0, 0, 0);
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i].y, b.y, acc[i], 0, 0, 0);
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i].z, b.z, acc[i], 0, 0, 0);
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i].w, b.w, acc[i], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < MAXT; ++i) {
+        const int t = w + BB_WAVES * i;
+        if (t < ntiles) {
+            int row = 16 * t + 4 * q;
+            if (CSA_ROW0 && row == 0) acc[i].x = CSA_C0;  // CSA's constant term (boot.cu:363-368)
+            float *y = Y + n * ss;
+            if (row >= ysplit) { y = Y2 + n * ss; row -= ysplit; }
+            if (row + 3 < ymax) {
+                *reinterpret_cast<float4 *>(y + row) = float4{acc[i].x, acc[i].y, acc[i].z, acc[i].w};
+            } else {
+                if (row < ymax) y[row] = acc[i].x;
+                if (row + 1 < ymax) y[row + 1] = acc[i].y;
+                if (row + 2 < ymax) y[row + 2] = acc[i].z;
+            }
+        }
+    }
+}
+
+// Per-slot state of the batched kernel (registers of the slot's 32 lanes).
+struct BbSlot {
+    int s;          // seed (-1: none yet)
+    int qk;         // queue cursor: queues qk.. of this XCD's order still to drain
+    int di, ndir;   // direction of the seed being tracked, directions of the seed
+    int off;        // streamline index of direction di
+    int pass;       // 0 backward, 1 forward
+    int i, att;     // step index of the pass, attempt of the step
+    int nb;         // backward points (pass 1)
+    bool done;      // queues drained
+    float sx, sy, sz, fx, fy, fz, px, py, pz, dx, dy, dz;
+};
+
+// Start pass `pass` of direction st.di: the tracker's loop head
+// (boot.cu:738-745): the point at index 0 of the pass is the seed.
+__device__ __forceinline__ void bb_start_pass(const DevParams &P, BbSlot &st, float *__restrict__ sline) {
+    const float sg = st.pass ? 1.0f : -1.0f;
+    st.px = st.sx; st.py = st.sy; st.pz = st.sz;
+    st.dx = sg * st.fx; st.dy = sg * st.fy; st.dz = sg * st.fz;
+    st.i = 1;
+    st.att = 0;
+    float *o = sline + (size_t)st.off * (2 * (size_t)P.max_sline_len * 3) + (st.pass ? 3 * (size_t)(st.nb - 1) : 0);
+    if (lane32() == 0) { o[0] = st.px; o[1] = st.py; o[2] = st.pz; }
+}
+
+// Next seed with at least one direction from the XCD-ordered queues, or
+// st.done.  Streamlines of seeds without directions need no work.
+__device__ void bb_next_seed(const DevParams &P, const DevBufs &B, BbSlot &st, Philox &rng, int nseeds,
+                             const float *__restrict__ seeds, int rng_chunk, const int *__restrict__ offsets,
+                             const int *__restrict__ dirs, const int *__restrict__ order, int *__restrict__ work,
+                             float *__restrict__ sline) {
+    const int l = lane32();
+    const int xcd = blockIdx.x & (QUEUES - 1);
+    while (st.qk < QUEUES) {
+        const int qx = (xcd + st.qk) & (QUEUES - 1);
+        const int lo = (int)((long long)nseeds * qx / QUEUES);
+        const int hi = (int)((long long)nseeds * (qx + 1) / QUEUES);
+        int q = 0;
+        if (l == 0) q = atomicAdd(work + qx, 1);
+        q = shfl32(q, 0);
+        if (lo + q >= hi) { ++st.qk; continue; }
+        const int s = order ? order[lo + q] : lo + q;
+        const int nd = offsets[s + 1] - offsets[s];
+        if (nd == 0) continue;
+        st.s = s;
+        st.ndir = nd;
+        st.di = 0;
+        st.off = offsets[s];
+        st.sx = seeds[3 * s]; st.sy = seeds[3 * s + 1]; st.sz = seeds[3 * s + 2];
+        rng.init(((uint64_t)P.seed_hi << 32) | P.seed_lo, (uint64_t)32 * (uint64_t)(s % rng_chunk) + l + 1);
+        const int vi = dirs[(size_t)s * P.nverts];
+        st.fx = B.verts[3 * vi]; st.fy = B.verts[3 * vi + 1]; st.fz = B.verts[3 * vi + 2];
+        st.pass = 0;
+        st.nb = 0;
+        return;
+    }
+    st.done = true;
+}
+
+// A pass ended with `ns` points: reverse the backward half in place, start
+// the forward pass, or close the streamline and move to the next direction
+// or seed (genStreamlinesMergeBoot_k, boot.cu:855-979).
+__device__ void bb_end_pass(const DevParams &P, const DevBufs &B, BbSlot &st, Philox &rng, int ns, int nseeds,
+                            const float *__restrict__ seeds, int rng_chunk, const int *__restrict__ offsets,
+                            const int *__restrict__ dirs, const int *__restrict__ order, int *__restrict__ work,
+                            int *__restrict__ sline_seed, int *__restrict__ sline_len, float *__restrict__ sline) {
+    const int l = lane32();
+    const size_t slot_f = 2 * (size_t)P.max_sline_len * 3;
+    float *cur = sline + (size_t)st.off * slot_f;
+    if (st.pass == 0) {
+        st.nb = ns;
+        MEM_SYNC();
+        for (int j = l; j < ns / 2; j += 32) {
+            float *a = cur + 3 * (size_t)j, *b = cur + 3 * (size_t)(ns - 1 - j);
+            const float a0 = a[0], a1 = a[1], a2 = a[2];
+            a[0] = b[0]; a[1] = b[1]; a[2] = b[2];
+            b[0] = a0; b[1] = a1; b[2] = a2;
+        }
+        MEM_SYNC();
+        st.pass = 1;
+        bb_start_pass(P, st, sline);
+        return;
+    }
+    if (l == 0) {
+        sline_seed[st.off] = st.s;
+        sline_len[st.off] = st.nb - 1 + ns;
+    }
+    ++st.di;
+    ++st.off;
+    if (st.di < st.ndir) {
+        const int vi = dirs[(size_t)st.s * P.nverts + st.di];
+        st.fx = B.verts[3 * vi]; st.fy = B.verts[3 * vi + 1]; st.fz = B.verts[3 * vi + 2];
+        st.pass = 0;
+        st.nb = 0;
+        bb_start_pass(P, st, sline);
+        return;
+    }
+    bb_next_seed(P, B, st, rng, nseeds, seeds, rng_chunk, offsets, dirs, order, work, sline);
+    if (!st.done) bb_start_pass(P, st, sline);
+}
+
+// OCC: waves per SIMD the kernel is compiled for (2: 256 VGPRs and one
+// workgroup per CU; 4: 128 VGPRs with spills and two workgroups per CU).
+template <int MODEL, int OCC>
+__global__ void __launch_bounds__(BB_THREADS, OCC)
+    phase2_boot_batch_kernel(DevParams P, DevBufs B, const float *__restrict__ seeds, int nseeds, int rng_chunk,
+                             const int *__restrict__ offsets, const int *__restrict__ dirs,
+                             int *__restrict__ sline_seed, int *__restrict__ sline_len, float *__restrict__ sline,
+                             const int *__restrict__ order, int *__restrict__ work) {
+    extern __shared__ float smem[];
+    const int ss = bb_slot_stride(P.nv32, P.t32, P.hr32);
+    const int slot = threadIdx.x >> 5;
+    const int l = lane32();
+    SlotLds SL;
+    bind_slot<MODEL>(P, false, smem + slot * ss, SL);
+    BootLds &S = SL.boot;
+    // Slot n's arrays are the same offsets from smem + n * ss.
+    const int off_h = 0;
+    const int off_vox = (int)(S.vox - smem) - slot * ss, off_msk = (int)(S.msk - smem) - slot * ss,
+              off_r = (int)(S.r - smem) - slot * ss;
+    const int hr = P.hr, T = P.dimt, nv = P.nverts, hr16 = P.hr16;
+    const int MAXL = P.max_sline_len;
+    BbSlot st;
+    st.s = -1; st.qk = 0; st.done = false; st.nb = 0; st.pass = 0;
+    Philox rng;
+    bb_next_seed(P, B, st, rng, nseeds, seeds, rng_chunk, offsets, dirs, order, work, sline);
+    if (!st.done) bb_start_pass(P, st, sline);
+    for (;;) {
+        // ---- phase A: gather the signal at the point -----------------------
+        const bool active = !st.done;
+        bool data = false;
+        if (active) {
+            data = trilinear<3>(P, B.data, P.t32, T, st.px, st.py, st.pz, S.vox) == 0;
+            WAVE_SYNC();
+            if (data) {
+                for (int m = l; m < hr16; m += 32) S.msk[m] = (m < hr) ? S.vox[B.dwi_idx[m]] : 0.0f;
+            }
+        }
+        if (!__syncthreads_or(active)) break;
+        // ---- phase B: H.d and R.d for the 16 slots --------------------------
+        bb_gemm<(2 * 320 / 16 + BB_WAVES - 1) / BB_WAVES>(B.hr_mat, hr16, hr16, 2 * hr16 / 16, smem + off_msk,
+                                                          smem + off_msk, hr16, smem + off_h, smem + off_r, hr16,
+                                                          2 * hr16, ss);
+        __syncthreads();
+        // ---- phase C: residual bootstrap, normalisation, model transform -----
+        if (active && data) {
+            for (int j = l; j < hr; j += 32) {
+                const int src = (int)(rng.next() % (uint32_t)hr);
+                S.h[j] = S.h[j] + S.r[src];
+            }
+            WAVE_SYNC();
+            for (int m = l; m < hr; m += 32) S.vox[B.dwi_idx[m]] = S.h[m];
+            WAVE_SYNC();
+            float part = 0.0f;
+            int cnt = 0;
+            for (int j = l; j < T; j += 32) {
+                float v = S.vox[j];
+                v = (P.min_signal > v) ? P.min_signal : v;
+                S.vox[j] = v;
+                if (B.b0[j]) { ++cnt; part = part + v; }
+            }
+            const float sum = tree_sum32(part);
+            for (int m = 16; m; m >>= 1) cnt += shfl_xor32(cnt, m);
+            const float denom = sum / (float)cnt;
+            for (int j = l; j < T; j += 32) S.vox[j] = S.vox[j] / denom;
+            WAVE_SYNC();
+            for (int m = l; m < hr16; m += 32) {
+                const float v = (m < hr) ? S.vox[B.dwi_idx[m]] : 0.0f;
+                S.msk[m] = v;
+                S.h[m] = (m < hr) ? gemv_xf<(MODEL == CUSLINES_OPDT) ? 1 : 2>(v) : 0.0f;
+            }
+        }
+        __syncthreads();
+        // ---- phase D: model coefficients (CSA: dq.xf; OPDT: dq.xf - db.msk) --
+        bb_gemm<1, MODEL == CUSLINES_CSA>(B.fit_mat, 2 * hr16, (MODEL == CUSLINES_OPDT) ? 2 * hr16 : hr16,
+                                           P.dn16 / 16, smem + off_h, smem + off_msk, hr16, smem + off_vox,
+                                           smem + off_vox, 1 << 30, P.dn16, ss);
+        __syncthreads();
+        // ---- phase E: the ODF on the sphere ---------------------------------
+        bb_gemm<(768 / 16 + BB_WAVES - 1) / BB_WAVES>(B.odf_mat, P.dn16, P.dn16, P.nv16 / 16, smem + off_vox,
+                                                      smem + off_vox, 1 << 30, smem + off_h, smem + off_h, 1 << 30,
+                                                      nv, ss);
+        __syncthreads();
+        // ---- phase F: peaks, closest peak, the step --------------------------
+        if (active) {
+            int ns = -1;  // >= 0: the pass ends with ns points
+            if (!data) {
+                // the signal is zero at this point: every remaining attempt
+                // fails the same way without a random draw
+                ns = st.i;
+            } else {
+                float mx = REAL_MIN;
+                for (int j = l; j < nv; j += 32) mx = (mx > S.h[j]) ? mx : S.h[j];
+                mx = max32(mx);
+                const float thr = (float)(PMF_THRESHOLD_P * (double)mx);
+                for (int j = l; j < nv; j += 32)
+                    if (S.h[j] < thr) S.h[j] = 0.0f;
+                WAVE_SYNC();
+                const int npk = peak_directions(P, B, S.h, S.marks, S.cand, B.verts);
+                float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+                const bool ok = npk > 0 && closest_peak(P, B.verts, st.dx, st.dy, st.dz, npk, S.cand, &nx, &ny, &nz);
+                WAVE_SYNC();
+                if (!ok) {
+                    if (++st.att == BB_MAX_ATTEMPTS) ns = st.i;
+                } else {
+                    st.att = 0;
+                    st.dx = nx; st.dy = ny; st.dz = nz;
+                    st.px = (float)__builtin_fma((double)(nx / 1.0f), P.step_d, (double)st.px);
+                    st.py = (float)__builtin_fma((double)(ny / 1.0f), P.step_d, (double)st.py);
+                    st.pz = (float)__builtin_fma((double)(nz / 1.0f), P.step_d, (double)st.pz);
+                    if (l == 0) {
+                        float *o = sline + (size_t)st.off * (2 * (size_t)MAXL * 3) +
+                                   (st.pass ? 3 * (size_t)(st.nb - 1) : 0) + 3 * (size_t)st.i;
+                        o[0] = st.px; o[1] = st.py; o[2] = st.pz;
+                    }
+                    const int tissue = check_point(P, B.stop, st.px, st.py, st.pz);
+                    if (tissue == CUSLINES_ENDPOINT || tissue == CUSLINES_INVALIDPOINT ||
+                        tissue == CUSLINES_OUTSIDEIMAGE)
+                        ns = st.i;
+                    else if (++st.i == MAXL)
+                        ns = MAXL;
+                }
+            }
+            if (ns >= 0)
+                bb_end_pass(P, B, st, rng, ns, nseeds, seeds, rng_chunk, offsets, dirs, order, work, sline_seed,
+                            sline_len, sline);
+        }
+        // no barrier here: phase A writes only this slot's region A, which
+        // phase E of every wave has finished reading (barrier above)
+    }
+}
+
 __global__ void zero_ints(int *p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
+}
+
+template <typename TO>
+__global__ void zero_scalar(TO *p) {
+    *p = 0;
 }
 
 // Spatial processing order of the persistent queue: seeds bucketed by the
@@ -2491,6 +2933,13 @@ static int fail(int code, const std::string &msg) {
     return code;
 }
 
+// roctx range over one C-ABI call (phases show up by name in rocprofv3
+// --marker-trace timelines).
+struct RoctxRange {
+    explicit RoctxRange(const char *name) { roctxRangePushA(name); }
+    ~RoctxRange() { roctxRangePop(); }
+};
+
 #define HIPC(expr)                                                                                  \
     do {                                                                                            \
         hipError_t e_ = (expr);                                                                     \
@@ -2506,6 +2955,9 @@ struct cuslines_tracker {
     uint32_t *d_edges = nullptr;
     uint16_t *d_nbr = nullptr;  // vertex adjacency, 8 per vertex (nullptr: a degree above 8 or > 1024 vertices)
     float *d_H = nullptr, *d_R = nullptr, *d_db = nullptr, *d_dq = nullptr, *d_smat = nullptr;
+    float *d_hrm = nullptr, *d_fitm = nullptr, *d_odfm = nullptr;  // batched bootstrap matrices
+    int boot_mode = CUSLINES_BOOT_EXACT;
+    int boot_occ = 2;  // batched bootstrap: waves per SIMD of the kernel variant
     int *d_b0 = nullptr, *d_dwi = nullptr;
     uint32_t *d_lut = nullptr;
     std::vector<uint16_t> rank_vertex;  // PTT: vertex of each direction rank (corner-block order)
@@ -2542,6 +2994,7 @@ static DevBufs bufs_of(const cuslines_tracker *t) {
     b.edges = t->d_edges;
     b.nbr = t->d_nbr;
     b.H = t->d_H; b.R = t->d_R; b.db = t->d_db; b.dq = t->d_dq; b.smat = t->d_smat;
+    b.hr_mat = t->d_hrm; b.fit_mat = t->d_fitm; b.odf_mat = t->d_odfm;
     b.b0 = t->d_b0;
     b.dwi_idx = t->d_dwi;
     b.lut = t->d_lut;
@@ -2616,37 +3069,69 @@ static int launch_ptt_groups(cuslines_tracker *t, const float *seeds, int n, int
 }
 
 template <typename TO>
-static int run_scan(cuslines_tracker *t, const int *in, int n, TO *out);
+static int run_scan(cuslines_tracker *t, const int *in, int n, TO *out, TO *bsum_ws = nullptr);
 
-// Seeds in Morton order of their 4-voxel brick (coarser bricks when a
-// dimension exceeds 512 voxels), by a counting sort on the tracker's stream.
-// *order is stream-ordered device memory the caller frees with hipFreeAsync.
-static int spatial_order(cuslines_tracker *t, const float *seeds, int n, int **order) {
-    const DevParams &d = t->dp;
+// Buckets of the spatial seed order: 4-voxel bricks (coarser when a
+// dimension exceeds 512 voxels), at most 2^21 of them.
+static int order_buckets(const DevParams &d, int *brick_log2_out) {
     int mx = d.dimx > d.dimy ? d.dimx : d.dimy;
     mx = mx > d.dimz ? mx : d.dimz;
     int brick_log2 = 2;
-    while (((mx - 1) >> brick_log2) >= 128) ++brick_log2;  // at most 2^21 buckets
+    while (((mx - 1) >> brick_log2) >= 128) ++brick_log2;
     int bits = 0;
     while ((1 << bits) <= ((mx - 1) >> brick_log2)) ++bits;
-    const int nb = 1 << (3 * bits);
-    int *key = nullptr, *count = nullptr, *start = nullptr;
-    HIPC(hipMallocAsync((void **)order, (size_t)n * sizeof(int), t->stream));
-    HIPC(hipMallocAsync((void **)&key, (size_t)n * sizeof(int), t->stream));
-    HIPC(hipMallocAsync((void **)&count, (size_t)nb * sizeof(int), t->stream));
-    HIPC(hipMallocAsync((void **)&start, ((size_t)nb + 1) * sizeof(int), t->stream));
+    *brick_log2_out = brick_log2;
+    return 1 << (3 * bits);
+}
+
+// ints of the seed-order workspace: order and keys (n each), bucket counts
+// (nb), bucket starts (nb + 1) and the scan's block sums.
+static size_t order_ws_ints(int n, int nb) {
+    return 2 * (size_t)n + 2 * (size_t)nb + 1 + (size_t)((nb + 1023) / 1024);
+}
+
+// Seeds in Morton order of their brick, by a counting sort on the tracker's
+// stream.  With `ws` (order_ws_ints(n, nb) ints of device memory) every
+// buffer is carved from it and *order points into it; without, they are
+// stream-ordered allocations and the caller frees *order with hipFreeAsync.
+static int spatial_order(cuslines_tracker *t, const float *seeds, int n, int **order, int *ws) {
+    const DevParams &d = t->dp;
+    int brick_log2 = 0;
+    const int nb = order_buckets(d, &brick_log2);
+    int *key = nullptr, *count = nullptr, *start = nullptr, *bsum = nullptr;
+    if (ws) {
+        *order = ws;
+        key = ws + n;
+        count = key + n;
+        start = count + nb;
+        bsum = start + nb + 1;
+    } else {
+        *order = nullptr;
+        hipError_t e = hipMallocAsync((void **)order, (size_t)n * sizeof(int), t->stream);
+        if (e == hipSuccess) e = hipMallocAsync((void **)&key, (size_t)n * sizeof(int), t->stream);
+        if (e == hipSuccess) e = hipMallocAsync((void **)&count, (size_t)nb * sizeof(int), t->stream);
+        if (e == hipSuccess) e = hipMallocAsync((void **)&start, ((size_t)nb + 1) * sizeof(int), t->stream);
+        if (e != hipSuccess) {
+            for (int *p : {*order, key, count, start})
+                if (p) (void)hipFreeAsync(p, t->stream);
+            *order = nullptr;
+            return fail(CUSLINES_ENOMEM, std::string("seed order: ") + hipGetErrorString(e));
+        }
+    }
     const dim3 gb((unsigned)((nb + 255) / 256)), gs((unsigned)((n + 255) / 256));
     hipLaunchKernelGGL(zero_ints, gb, dim3(256), 0, t->stream, count, nb);
     hipLaunchKernelGGL(seed_keys, gs, dim3(256), 0, t->stream, d, seeds, n, brick_log2, key, count);
-    int rc = run_scan<int>(t, count, nb, start);
+    int rc = run_scan<int>(t, count, nb, start, bsum);
     if (rc == CUSLINES_OK) {
         hipLaunchKernelGGL(zero_ints, gb, dim3(256), 0, t->stream, count, nb);
         hipLaunchKernelGGL(seed_scatter, gs, dim3(256), 0, t->stream, key, n, start, count, *order);
     }
     hipError_t e = hipGetLastError();
-    (void)hipFreeAsync(key, t->stream);
-    (void)hipFreeAsync(count, t->stream);
-    (void)hipFreeAsync(start, t->stream);
+    if (!ws) {
+        (void)hipFreeAsync(key, t->stream);
+        (void)hipFreeAsync(count, t->stream);
+        (void)hipFreeAsync(start, t->stream);
+    }
     if (rc) return rc;
     if (e != hipSuccess) return fail(CUSLINES_EHIP, std::string("seed order: ") + hipGetErrorString(e));
     return CUSLINES_OK;
@@ -2654,7 +3139,7 @@ static int spatial_order(cuslines_tracker *t, const float *seeds, int n, int **o
 
 template <int MODEL, int V>
 static int launch_phase2_v(cuslines_tracker *t, const float *seeds, int n, int chunk, const int *offsets,
-                           const int *dirs, int *sseed, int *slen, float *sl) {
+                           const int *dirs, int *sseed, int *slen, float *sl, int *ws) {
     const DevParams &d = t->dp;
     int threads = t->threads;
     if (threads <= 0) threads = (MODEL == CUSLINES_OPDT || MODEL == CUSLINES_CSA) ? 128 : 256;
@@ -2686,48 +3171,125 @@ static int launch_phase2_v(cuslines_tracker *t, const float *seeds, int n, int c
     if (grid > need) grid = need;
     if (grid < 1) grid = 1;
     int *order = nullptr;
-    if (int rc = spatial_order(t, seeds, n, &order)) return rc;
+    if (int rc = spatial_order(t, seeds, n, &order, ws)) return rc;
     hipLaunchKernelGGL(zero_ints, dim3(1), dim3(QUEUES), 0, t->stream, t->d_work, QUEUES);
     hipLaunchKernelGGL((phase2_queue_kernel<MODEL, V>), dim3((unsigned)grid), dim3(threads), lds, t->stream, d,
                        bufs_of(t), seeds, n, chunk, offsets, dirs, sseed, slen, sl, order, t->d_work);
     hipError_t e = hipGetLastError();
-    (void)hipFreeAsync(order, t->stream);
+    if (!ws) (void)hipFreeAsync(order, t->stream);
     if (e != hipSuccess) return fail(CUSLINES_EHIP, std::string("phase 2: ") + hipGetErrorString(e));
+    return CUSLINES_OK;
+}
+
+// Limits of the batched bootstrap kernel's tile counts per wave.
+static bool boot_batch_fits(const DevParams &d) {
+    return d.hr16 <= 320 && d.dn16 <= 128 && d.nv16 <= 768 && d.dn16 <= d.t32;
+}
+
+template <int MODEL>
+static int launch_boot_batch(cuslines_tracker *t, const float *seeds, int n, int chunk, const int *offsets,
+                             const int *dirs, int *sseed, int *slen, float *sl, int *ws) {
+    const DevParams &d = t->dp;
+    if (!boot_batch_fits(d))
+        return fail(CUSLINES_EINVAL, "batched bootstrap needs hr <= 320, delta_nr <= 128 and nverts <= 768");
+    const size_t lds = sizeof(float) * (size_t)BB_SLOTS * (size_t)bb_slot_stride(d.nv32, d.t32, d.hr32);
+    if (lds > 160 * 1024) return fail(CUSLINES_EINVAL, "batched bootstrap slots too large for LDS");
+    const void *kfn = (t->boot_occ == 4) ? (const void *)phase2_boot_batch_kernel<MODEL, 4>
+                                         : (const void *)phase2_boot_batch_kernel<MODEL, 2>;
+    HIPC(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = 0;
+    HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, BB_THREADS, lds));
+    if (per_cu < 1) per_cu = 1;
+    if (t->blocks_per_cu > 0 && t->blocks_per_cu < per_cu) per_cu = t->blocks_per_cu;
+    long long grid = (long long)per_cu * t->num_cu;
+    const long long need = (n + BB_SLOTS - 1) / BB_SLOTS;
+    if (grid > need) grid = need;
+    if (grid < 1) grid = 1;
+    int *order = nullptr;
+    if (int rc = spatial_order(t, seeds, n, &order, ws)) return rc;
+    hipLaunchKernelGGL(zero_ints, dim3(1), dim3(QUEUES), 0, t->stream, t->d_work, QUEUES);
+    if (t->boot_occ == 4)
+        hipLaunchKernelGGL((phase2_boot_batch_kernel<MODEL, 4>), dim3((unsigned)grid), dim3(BB_THREADS), lds,
+                           t->stream, d, bufs_of(t), seeds, n, chunk, offsets, dirs, sseed, slen, sl, order,
+                           t->d_work);
+    else
+        hipLaunchKernelGGL((phase2_boot_batch_kernel<MODEL, 2>), dim3((unsigned)grid), dim3(BB_THREADS), lds,
+                           t->stream, d, bufs_of(t), seeds, n, chunk, offsets, dirs, sseed, slen, sl, order,
+                           t->d_work);
+    hipError_t e = hipGetLastError();
+    if (!ws) (void)hipFreeAsync(order, t->stream);
+    if (e != hipSuccess) return fail(CUSLINES_EHIP, std::string("phase 2 (batched): ") + hipGetErrorString(e));
     return CUSLINES_OK;
 }
 
 template <int MODEL>
 static int launch_phase2(cuslines_tracker *t, const float *seeds, int n, int chunk, const int *offsets,
-                         const int *dirs, int *sseed, int *slen, float *sl) {
+                         const int *dirs, int *sseed, int *slen, float *sl, int *ws) {
+    if constexpr (MODEL == CUSLINES_OPDT || MODEL == CUSLINES_CSA) {
+        if (t->boot_mode == CUSLINES_BOOT_MFMA && t->schedule != CUSLINES_SCHED_STATIC)
+            return launch_boot_batch<MODEL>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl, ws);
+    }
     if constexpr (MODEL == CUSLINES_PTT) {
         if (t->d_mlp)
-            return launch_phase2_v<MODEL, 1>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl);
+            return launch_phase2_v<MODEL, 1>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl, ws);
         if (t->rec)
-            return launch_phase2_v<MODEL, 2>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl);
+            return launch_phase2_v<MODEL, 2>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl, ws);
         if (t->count_taps)
-            return launch_phase2_v<MODEL, 3>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl);
+            return launch_phase2_v<MODEL, 3>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl, ws);
         if (t->ptt_lanes > 0 && t->schedule != CUSLINES_SCHED_STATIC)
             return launch_ptt_groups(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl);
     }
-    return launch_phase2_v<MODEL, 0>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl);
+    return launch_phase2_v<MODEL, 0>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl, ws);
 }
 
 template <typename TO>
-static int run_scan(cuslines_tracker *t, const int *in, int n, TO *out) {
+static int run_scan(cuslines_tracker *t, const int *in, int n, TO *out, TO *bsum_ws) {
     if (n <= 0) {
-        HIPC(hipMemsetAsync(out, 0, sizeof(TO), t->stream));
+        hipLaunchKernelGGL(zero_scalar<TO>, dim3(1), dim3(1), 0, t->stream, out);
+        HIPC(hipGetLastError());
         return CUSLINES_OK;
     }
     const int nb = (n + 1023) / 1024;
-    TO *bsum = nullptr;
-    HIPC(hipMallocAsync((void **)&bsum, (size_t)nb * sizeof(TO), t->stream));
+    TO *bsum = bsum_ws;
+    if (!bsum) HIPC(hipMallocAsync((void **)&bsum, (size_t)nb * sizeof(TO), t->stream));
     hipLaunchKernelGGL(scan_pass1<TO>, dim3(nb), dim3(1024), 0, t->stream, in, n, bsum);
     hipLaunchKernelGGL(scan_pass2<TO>, dim3(1), dim3(1024), 0, t->stream, bsum, nb);
     hipLaunchKernelGGL(scan_pass3<TO>, dim3(nb), dim3(1024), 0, t->stream, in, n, bsum, out);
     hipError_t e = hipGetLastError();
-    (void)hipFreeAsync(bsum, t->stream);
+    if (!bsum_ws) (void)hipFreeAsync(bsum, t->stream);
     if (e != hipSuccess) return fail(CUSLINES_EHIP, std::string("scan: ") + hipGetErrorString(e));
     return CUSLINES_OK;
+}
+
+// The phase-2 entry point; ws (optional) is a seed-order workspace of
+// order_ws_ints(nseeds, buckets) ints in place of stream-ordered allocations.
+static int phase2_entry(cuslines_tracker *t, const float *seeds_d, int32_t nseeds, int32_t rng_chunk,
+                        const int32_t *offsets_d, const int32_t *dirs_d, int32_t *sline_seed_d, int32_t *sline_len_d,
+                        float *sline_d, int *ws) {
+    RoctxRange range_("cuslines_phase2");
+    if (!t) return fail(CUSLINES_EINVAL, "null tracker");
+    if (nseeds < 0 || rng_chunk < 0) return fail(CUSLINES_EINVAL, "negative size");
+    if (nseeds == 0) return CUSLINES_OK;
+    if (!seeds_d || !offsets_d || !dirs_d || !sline_seed_d || !sline_len_d || !sline_d)
+        return fail(CUSLINES_EINVAL, "null buffer");
+    int rc = check_ready(t);
+    if (rc) return rc;
+    HIPC(hipSetDevice(t->device));
+    const int chunk = rng_chunk > 0 ? rng_chunk : nseeds;
+    switch (t->p.model_type) {
+        case CUSLINES_OPDT:
+            return launch_phase2<CUSLINES_OPDT>(t, seeds_d, nseeds, chunk, offsets_d, dirs_d, sline_seed_d,
+                                                sline_len_d, sline_d, ws);
+        case CUSLINES_CSA:
+            return launch_phase2<CUSLINES_CSA>(t, seeds_d, nseeds, chunk, offsets_d, dirs_d, sline_seed_d,
+                                               sline_len_d, sline_d, ws);
+        case CUSLINES_PROB:
+            return launch_phase2<CUSLINES_PROB>(t, seeds_d, nseeds, chunk, offsets_d, dirs_d, sline_seed_d,
+                                                sline_len_d, sline_d, ws);
+        default:
+            return launch_phase2<CUSLINES_PTT>(t, seeds_d, nseeds, chunk, offsets_d, dirs_d, sline_seed_d,
+                                               sline_len_d, sline_d, ws);
+    }
 }
 
 extern "C" {
@@ -2843,6 +3405,7 @@ int cuslines_tracker_destroy(cuslines_tracker *t) {
     if (t->stream) (void)hipStreamSynchronize(t->stream);
     dfree(t->d_data); dfree(t->d_stop); dfree(t->d_verts); dfree(t->d_edges); dfree(t->d_nbr);
     dfree(t->d_H); dfree(t->d_R); dfree(t->d_db); dfree(t->d_dq); dfree(t->d_smat);
+    dfree(t->d_hrm); dfree(t->d_fitm); dfree(t->d_odfm);
     dfree(t->d_b0); dfree(t->d_dwi); dfree(t->d_lut); dfree(t->d_work); dfree(t->d_blocks);
     dfree(t->d_fprop); dfree(t->d_mlp); dfree(t->d_stats);
     if (t->stream) (void)hipStreamDestroy(t->stream);
@@ -2901,6 +3464,7 @@ static int ptt_take_data(cuslines_tracker *t, const float *src, int nonneg) {
 }
 
 int cuslines_set_data(cuslines_tracker *t, const float *dataf) {
+    RoctxRange range_("cuslines_set_data");
     if (!t || !dataf) return fail(CUSLINES_EINVAL, "null argument");
     HIPC(hipSetDevice(t->device));
     const DevParams &d = t->dp;
@@ -3029,6 +3593,34 @@ int cuslines_set_boot_matrices(cuslines_tracker *t, const float *H, const float 
         HIPC(hipMalloc(dst[k], h.size() * sizeof(float)));
         HIPC(hipMemcpy(*dst[k], h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
     }
+    // the batched kernel's zero-padded row-major copies: [H; R] [2 hr16][hr16],
+    // [dq | -db] [dn16][2 hr16], sampling matrix [nv16][dn16]
+    {
+        const int hr16 = (hr + 15) & ~15, dn16 = (dn + 15) & ~15, nv16 = (nv + 15) & ~15;
+        t->dp.hr16 = hr16; t->dp.dn16 = dn16; t->dp.nv16 = nv16;
+        std::vector<float> hrm((size_t)2 * hr16 * hr16, 0.0f), fit((size_t)dn16 * 2 * hr16, 0.0f),
+            odf((size_t)nv16 * dn16, 0.0f);
+        for (int i = 0; i < hr; ++i)
+            for (int j = 0; j < hr; ++j) {
+                hrm[(size_t)i * hr16 + j] = H[(size_t)i * hr + j];
+                hrm[(size_t)(hr16 + i) * hr16 + j] = R[(size_t)i * hr + j];
+            }
+        for (int i = 0; i < dn; ++i)
+            for (int j = 0; j < hr; ++j) {
+                fit[(size_t)i * 2 * hr16 + j] = delta_q[(size_t)i * hr + j];
+                fit[(size_t)i * 2 * hr16 + hr16 + j] = -delta_b[(size_t)i * hr + j];
+            }
+        for (int i = 0; i < nv; ++i)
+            for (int j = 0; j < dn; ++j) odf[(size_t)i * dn16 + j] = sampling_matrix[(size_t)i * dn + j];
+        float **bdst[3] = {&t->d_hrm, &t->d_fitm, &t->d_odfm};
+        const std::vector<float> *bsrc[3] = {&hrm, &fit, &odf};
+        for (int k = 0; k < 3; ++k) {
+            dfree(*bdst[k]);
+            *bdst[k] = nullptr;
+            HIPC(hipMalloc(bdst[k], bsrc[k]->size() * sizeof(float)));
+            HIPC(hipMemcpy(*bdst[k], bsrc[k]->data(), bsrc[k]->size() * sizeof(float), hipMemcpyHostToDevice));
+        }
+    }
     dfree(t->d_b0); dfree(t->d_dwi);
     t->d_b0 = nullptr; t->d_dwi = nullptr;
     HIPC(hipMalloc(&t->d_b0, (size_t)T * sizeof(int)));
@@ -3148,6 +3740,19 @@ int cuslines_set_launch(cuslines_tracker *t, int schedule, int threads, int bloc
     return CUSLINES_OK;
 }
 
+int cuslines_set_boot_mode(cuslines_tracker *t, int mode) {
+    if (!t) return fail(CUSLINES_EINVAL, "null tracker");
+    if (t->p.model_type != CUSLINES_OPDT && t->p.model_type != CUSLINES_CSA)
+        return fail(CUSLINES_EINVAL, "the bootstrap mode is for OPDT and CSA trackers");
+    if ((mode & 0xff) != CUSLINES_BOOT_EXACT && (mode & 0xff) != CUSLINES_BOOT_MFMA)
+        return fail(CUSLINES_EINVAL, "unknown mode");
+    const int occ = mode >> 8;
+    if (occ != 0 && occ != 2 && occ != 4) return fail(CUSLINES_EINVAL, "waves per SIMD must be 2 or 4");
+    t->boot_mode = mode & 0xff;
+    t->boot_occ = occ ? occ : 2;
+    return CUSLINES_OK;
+}
+
 int cuslines_set_lanes_per_seed(cuslines_tracker *t, int lanes) {
     if (!t) return fail(CUSLINES_EINVAL, "null tracker");
     if (lanes != 0 && lanes != 8 && lanes != 16 && lanes != 32)
@@ -3158,6 +3763,7 @@ int cuslines_set_lanes_per_seed(cuslines_tracker *t, int lanes) {
 
 int cuslines_ptt_prepare(cuslines_tracker *t, const float *odf, const float *stop_map, float stop_threshold,
                          int sphere_symm, int32_t *lut_out) {
+    RoctxRange range_("cuslines_ptt_prepare");
     if (!t || !odf || !stop_map) return fail(CUSLINES_EINVAL, "null argument");
     if (t->p.model_type != CUSLINES_PTT) return fail(CUSLINES_ESTATE, "PTT preparation needs a PTT tracker");
     if (!t->has_sphere) return fail(CUSLINES_ESTATE, "upload the sphere before preparing PTT data");
@@ -3296,9 +3902,26 @@ int cuslines_proposal_stats(cuslines_tracker *t, int64_t *out3, int reset) {
     return CUSLINES_OK;
 }
 
+int cuslines_proposal_counters(cuslines_tracker *t, int64_t *out4, int reset) {
+    if (!t || !out4) return fail(CUSLINES_EINVAL, "null argument");
+    HIPC(hipSetDevice(t->device));
+    unsigned long long h[4] = {0, 0, 0, 0};
+    if (t->d_stats) {
+        HIPC(hipStreamSynchronize(t->stream));
+        HIPC(hipMemcpy(h, t->d_stats, sizeof(h), hipMemcpyDeviceToHost));
+        if (reset) HIPC(hipMemset(t->d_stats, 0, sizeof(h)));
+    }
+    for (int i = 0; i < 4; ++i) out4[i] = (int64_t)h[i];
+    return CUSLINES_OK;
+}
+
 int cuslines_set_tap_count(cuslines_tracker *t, int enable) {
     if (!t) return fail(CUSLINES_EINVAL, "null tracker");
     if (t->p.model_type != CUSLINES_PTT) return fail(CUSLINES_EINVAL, "tap counting is for PTT");
+    // Switched off: the instrumented phase-2 kernel stopped responding on
+    // its first GPU run (48 seeds, 32x36x30 phantom) and the cause has not
+    // been found, so it is never launched.
+    if (enable) return fail(CUSLINES_ESTATE, "the instrumented PTT kernel (tap counting) is switched off");
     HIPC(hipSetDevice(t->device));
     if (enable && !t->d_stats) {
         HIPC(hipMalloc(&t->d_stats, 4 * sizeof(unsigned long long)));
@@ -3406,6 +4029,7 @@ int cuslines_get_stream(cuslines_tracker *t, void **stream) {
 
 int cuslines_phase1_async(cuslines_tracker *t, const float *seeds_d, int32_t nseeds, int32_t rng_chunk,
                           int32_t *counts_d, int32_t *dirs_d) {
+    RoctxRange range_("cuslines_phase1");
     if (!t) return fail(CUSLINES_EINVAL, "null tracker");
     if (nseeds < 0 || rng_chunk < 0) return fail(CUSLINES_EINVAL, "negative size");
     if (nseeds == 0) return CUSLINES_OK;
@@ -3423,6 +4047,7 @@ int cuslines_phase1_async(cuslines_tracker *t, const float *seeds_d, int32_t nse
 }
 
 int cuslines_scan_async(cuslines_tracker *t, const int32_t *counts_d, int32_t nseeds, int32_t *offsets_d) {
+    RoctxRange range_("cuslines_scan");
     if (!t || !offsets_d || (nseeds > 0 && !counts_d)) return fail(CUSLINES_EINVAL, "null argument");
     if (nseeds < 0) return fail(CUSLINES_EINVAL, "negative size");
     HIPC(hipSetDevice(t->device));
@@ -3432,29 +4057,13 @@ int cuslines_scan_async(cuslines_tracker *t, const int32_t *counts_d, int32_t ns
 int cuslines_phase2_async(cuslines_tracker *t, const float *seeds_d, int32_t nseeds, int32_t rng_chunk,
                           const int32_t *offsets_d, const int32_t *dirs_d, int32_t *sline_seed_d,
                           int32_t *sline_len_d, float *sline_d) {
-    if (!t) return fail(CUSLINES_EINVAL, "null tracker");
-    if (nseeds < 0 || rng_chunk < 0) return fail(CUSLINES_EINVAL, "negative size");
-    if (nseeds == 0) return CUSLINES_OK;
-    if (!seeds_d || !offsets_d || !dirs_d || !sline_seed_d || !sline_len_d || !sline_d)
-        return fail(CUSLINES_EINVAL, "null buffer");
-    int rc = check_ready(t);
-    if (rc) return rc;
-    HIPC(hipSetDevice(t->device));
-    const int chunk = rng_chunk > 0 ? rng_chunk : nseeds;
-    switch (t->p.model_type) {
-        case CUSLINES_OPDT:
-            return launch_phase2<CUSLINES_OPDT>(t, seeds_d, nseeds, chunk, offsets_d, dirs_d, sline_seed_d, sline_len_d, sline_d);
-        case CUSLINES_CSA:
-            return launch_phase2<CUSLINES_CSA>(t, seeds_d, nseeds, chunk, offsets_d, dirs_d, sline_seed_d, sline_len_d, sline_d);
-        case CUSLINES_PROB:
-            return launch_phase2<CUSLINES_PROB>(t, seeds_d, nseeds, chunk, offsets_d, dirs_d, sline_seed_d, sline_len_d, sline_d);
-        default:
-            return launch_phase2<CUSLINES_PTT>(t, seeds_d, nseeds, chunk, offsets_d, dirs_d, sline_seed_d, sline_len_d, sline_d);
-    }
+    return phase2_entry(t, seeds_d, nseeds, rng_chunk, offsets_d, dirs_d, sline_seed_d, sline_len_d, sline_d,
+                        nullptr);
 }
 
 int cuslines_compact_async(cuslines_tracker *t, const int32_t *sline_len_d, int32_t nslines, const float *sline_d,
                            int64_t *point_off_d, float *points_d) {
+    RoctxRange range_("cuslines_compact");
     if (!t || !point_off_d || (nslines > 0 && (!sline_len_d || !sline_d || !points_d)))
         return fail(CUSLINES_EINVAL, "null argument");
     if (nslines < 0) return fail(CUSLINES_EINVAL, "negative size");

@@ -27,6 +27,22 @@ def div_up(a, b):
     return (a + b - 1) // b
 
 
+def seed_shares(nseeds: int, ngpus: int):
+    """(first seed, seed count) of each GPU's contiguous share of a chunk:
+    ceil(nseeds / ngpus) seeds per GPU, the last GPUs taking what is left
+    (cu_propagate_seeds.py:45-54)."""
+    per = div_up(nseeds, ngpus) if ngpus > 0 else 0
+    return [(min(ii * per, nseeds), min(per, max(0, nseeds - ii * per))) for ii in range(ngpus)]
+
+
+def streamline_offsets(counts):
+    """Exclusive scan of the per-GPU streamline counts: where each GPU's
+    streamlines start in the chunk's merged output (the one exchange the
+    seed split needs; N integers)."""
+    counts = np.asarray(counts, dtype=np.int64)
+    return np.concatenate([[0], np.cumsum(counts)])
+
+
 class SeedBatchPropagator:
     def __init__(self, gpu_tracker, minlen=0, maxlen=np.inf):
         self.gpu_tracker = gpu_tracker
@@ -55,7 +71,7 @@ class SeedBatchPropagator:
         self.sline_d = [None] * n
 
     def _switch_device(self, n):
-        nseeds_gpu = min(self.nseeds_per_gpu, max(0, self.nseeds - n * self.nseeds_per_gpu))
+        nseeds_gpu = seed_shares(self.nseeds, self.ngpus)[n][1]
         block = (THR_X_SL, THR_X_BL // THR_X_SL, 1)
         grid = (div_up(nseeds_gpu, THR_X_BL // THR_X_SL), 1, 1)
         return nseeds_gpu, block, grid
@@ -78,7 +94,8 @@ class SeedBatchPropagator:
             for ii in range(self.ngpus):
                 nseeds_gpu, block, grid = self._switch_device(ii)
                 dt = tr.device_trackers[ii]
-                part = seeds[ii * self.nseeds_per_gpu: ii * self.nseeds_per_gpu + nseeds_gpu]
+                lo = seed_shares(self.nseeds, self.ngpus)[ii][0]
+                part = seeds[lo: lo + nseeds_gpu]
                 self.seeds_d[ii] = dt.alloc(part.nbytes)
                 self.slinesOffs_d[ii] = dt.alloc(4 * (nseeds_gpu + 1))
                 self.slinesScan_d[ii] = dt.alloc(4 * (nseeds_gpu + 1))

@@ -181,6 +181,26 @@ enum {
  * is fixed at 2 seeds per 64-thread block (cu_propagate_seeds.py:45-54). */
 int cuslines_set_launch(cuslines_tracker *t, int schedule, int threads, int blocks_per_cu);
 
+/* Bootstrap (OPDT/CSA) phase-2 kernels. */
+enum {
+    CUSLINES_BOOT_EXACT = 0,  /* one seed per 32 lanes, the reference's lane-strided
+                                 dot products: bit-identical to the CPU oracle    */
+    CUSLINES_BOOT_MFMA = 1    /* 16 seeds per workgroup in lockstep, the refit and
+                                 ODF projection as fp32 MFMA products batched over
+                                 the 16 seeds: equal within float rounding         */
+};
+
+/* Select the bootstrap phase-2 kernel (default CUSLINES_BOOT_EXACT).  Replaces
+ * genStreamlinesMergeBoot_k's per-seed ndotp_d products
+ * (cuslines/cuda_c/boot.cu:195-235, 466-534) with batched matrix-core products
+ * in CUSLINES_BOOT_MFMA mode; the algorithm, random streams and outputs'
+ * layout are the same.  MFMA mode needs hr <= 320 DWI volumes, delta_nr <= 128
+ * and nverts <= 768 (phase 2 returns CUSLINES_EINVAL otherwise) and the
+ * queue schedule (CUSLINES_SCHED_STATIC keeps the exact kernel).  Bits 8..15
+ * of `mode` choose the MFMA kernel's occupancy, 2 (default) or 4 waves per
+ * SIMD (the autotuner's knob). */
+int cuslines_set_boot_mode(cuslines_tracker *t, int mode);
+
 /* PTT phase 2 with `lanes` (8, 16 or 32) lanes per seed, 64/lanes seeds per
  * wavefront.  The reference's 32 per-lane random streams per seed are kept as
  * counter-based virtual lanes, so the output does not depend on this choice.
@@ -218,13 +238,21 @@ int cuslines_set_recording(cuslines_tracker *t, float *pairs_d, int32_t capacity
  * proposal.  reset != 0 zeroes them after reading. */
 int cuslines_proposal_stats(cuslines_tracker *t, int64_t *out3, int reset);
 
+/* The same three counters and out4[3]: face draws of the learned rounds
+ * whose exact face value exceeded the envelope (PROPOSAL_M x the proposal's
+ * face value), i.e. where the exact correction could not hold; 0 means the
+ * learned sampler drew from the reference's distribution. */
+int cuslines_proposal_counters(cuslines_tracker *t, int64_t *out4, int reset);
+
 /* Instrumentation of PTT phase 2 (no counterpart in the reference): while
  * enabled, phase 2 runs an instrumented copy of the plain getter (same
  * results) that counts every ODF tap it reads (one LUT entry + one 8-texel
  * block each) and every sub-step.  cuslines_tap_count returns out[0] taps and
  * out[1] sub-steps since the last reset (reset != 0 zeroes them); it is
  * refused while a learned proposal is set, whose statistics share the
- * counters.  Used to state the bytes the kernel must move per step. */
+ * counters.  Currently switched off: enable != 0 returns CUSLINES_ESTATE
+ * (the instrumented kernel stopped responding on its first GPU run and the
+ * cause is not yet found); enable == 0 and cuslines_tap_count still work. */
 int cuslines_set_tap_count(cuslines_tracker *t, int enable);
 int cuslines_tap_count(cuslines_tracker *t, int64_t *out2, int reset);
 

@@ -260,7 +260,7 @@ struct Problem {
     int nx = 1, ny = 1, nz = 1;
     // learned PTT proposal (no counterpart in the reference)
     std::vector<uint16_t> mlp;     // bf16 parameters; empty: plain getter
-    mutable long long stats[3] = {0, 0, 0};
+    mutable long long stats[4] = {0, 0, 0, 0};
 };
 
 // Lane-strided FMA partial sums reduced by the shuffle tree that lane 0 sees
@@ -450,7 +450,7 @@ struct SeedScratch {
     Philox rng[LANES];
     float frame[18];
     float mix[8 * 12];
-    long long stats[3] = {0, 0, 0};
+    long long stats[4] = {0, 0, 0, 0};
     void size(const Problem &P) {
         const size_t n = (size_t)std::max(P.p.dimt, P.p.nverts) + 64;
         a.assign(n, 0.f); b.assign(n, 0.f); c.assign(n, 0.f); d.assign(n, 0.f);
@@ -977,11 +977,23 @@ void mlp_forward(const uint16_t *W, const float *x, float *o) {
     for (int j = 0; j < MLP_OUT; ++j) o[j] = mlp_neuron(W + MLP_W3 + j * MLP_H, W[MLP_B3 + j], h2, MLP_H);
 }
 
-// Learned sub-step: predicted supports (cut at 0.4) drive one round of the
-// reference's sampler; on failure *fell = 1 and the plain getter runs.
+// Learned sub-step with the exact correction (same operations as the
+// kernel's get_direction_ptt_learned): faces drawn from the network's
+// estimate g(f) = clipped predicted supports summed + PROPOSAL_FLOOR x their
+// mean, accepted with probability F(f) / (PROPOSAL_M g(f)) where F is the
+// reference's face value from the exact supports of the drawn faces'
+// vertices, then the reference's support test; lowest passing lane wins.  Up
+// to LEARNED_ROUNDS rounds, then (*fell = 1) the plain sub-step.  Draws whose
+// F exceeds PROPOSAL_M g are counted in S.stats[3].
+constexpr float PROPOSAL_M = 2.0f;
+constexpr float PROPOSAL_FLOOR = 0.5f;
+constexpr int LEARNED_ROUNDS = 4;
+
 int get_direction_ptt_learned(const Problem &P, SeedScratch &S, bool learned, const float *dir, float *frame,
-                              const float *pos, float *new_dir, int *fell) {
+                              const float *pos, float *new_dir, int *fell, float *k_win = nullptr) {
     const double min_support = (double)P.p.probe_quality * PMF_THRESHOLD_P;
+    const float min_support_f = (float)min_support < min_support ? std::nextafter((float)min_support, INFINITY)
+                                                                  : (float)min_support;
     const float first_val = interp4(P, pos, frame);
     float vert_pdf[DISC_VERT_CNT];
     *fell = 0;
@@ -989,8 +1001,88 @@ int get_direction_ptt_learned(const Problem &P, SeedScratch &S, bool learned, co
         float x[MLP_IN], o[MLP_OUT];
         mlp_features(P, pos, dir, frame, first_val, x);
         mlp_forward(P.mlp.data(), x, o);
-        for (int v = 0; v < DISC_VERT_CNT; ++v) vert_pdf[v] = ((double)o[v] < min_support) ? 0.0f : o[v];
-        if (ptt_sample(P, S, false, dir, frame, pos, first_val, vert_pdf, 1, new_dir)) return 1;
+        float gh[DISC_FACE_CNT], g[DISC_FACE_CNT], cdf[DISC_FACE_CNT], tmpa[DISC_FACE_CNT];
+        for (int f = 0; f < DISC_FACE_CNT; ++f) {
+            gh[f] = 0.0f;
+            for (int jj = 0; jj < 3; ++jj) {
+                const float sv = o[DISC_FACE[f * 3 + jj]];
+                gh[f] = gh[f] + ((sv > 0.0f) ? sv : 0.0f);
+            }
+            tmpa[f] = gh[f];
+        }
+        prefix_sum(tmpa, DISC_FACE_CNT);
+        const float tot = tmpa[DISC_FACE_CNT - 1];
+        const float fl = PROPOSAL_FLOOR * (tot / (float)DISC_FACE_CNT) + 1e-3f;
+        for (int f = 0; f < DISC_FACE_CNT; ++f) { g[f] = gh[f] + fl; cdf[f] = g[f]; }
+        prefix_sum(cdf, DISC_FACE_CNT);
+        const float last = cdf[DISC_FACE_CNT - 1];
+        const double mc = (double)P.p.max_curvature;
+        float E[DISC_VERT_CNT];
+        bool known[DISC_VERT_CNT] = {};
+        for (int round = 0; round < LEARNED_ROUNDS; ++round) {
+            int jjs[LANES];
+            float k1s[LANES], k2s[LANES], uas[LANES];
+            bool need[DISC_VERT_CNT] = {};
+            for (int t = 0; t < LANES; ++t) {
+                Philox &rng = S.rng[t];
+                float r1 = rng.uniform(), r2 = rng.uniform();
+                if (r1 + r2 > 1.0f) { r1 = 1.0f - r1; r2 = 1.0f - r2; }
+                const float tmp = rng.uniform() * last;
+                uas[t] = rng.uniform();
+                int jj = 0;
+                for (; jj < DISC_FACE_CNT - 1; ++jj) if (cdf[jj] >= tmp) break;
+                jjs[t] = jj;
+                const int *fc = DISC_FACE + jj * 3;
+                for (int q = 0; q < 3; ++q) need[fc[q]] = true;
+                const float vx0 = (float)(mc * (double)DISC_VERT[fc[0] * 2]);
+                const float vx1 = (float)(mc * (double)DISC_VERT[fc[1] * 2]);
+                const float vx2 = (float)(mc * (double)DISC_VERT[fc[2] * 2]);
+                const float vy0 = (float)(mc * (double)DISC_VERT[fc[0] * 2 + 1]);
+                const float vy1 = (float)(mc * (double)DISC_VERT[fc[1] * 2 + 1]);
+                const float vy2 = (float)(mc * (double)DISC_VERT[fc[2] * 2 + 1]);
+                k1s[t] = std::fmaf(r2, vx2 - vx0, std::fmaf(r1, vx1 - vx0, vx0));
+                k2s[t] = std::fmaf(r2, vy2 - vy0, std::fmaf(r1, vy1 - vy0, vy0));
+            }
+            for (int v = 0; v < DISC_VERT_CNT; ++v) {
+                if (!need[v] || known[v]) continue;
+                const float k1 = (float)((double)DISC_VERT[v * 2] * mc);
+                const float k2 = (float)((double)DISC_VERT[v * 2 + 1] * mc);
+                float pf[9];
+                for (int i = 0; i < 9; ++i) pf[i] = frame[i];
+                E[v] = data_support(P, first_val, pos, k1, k2, pf);
+                known[v] = true;
+            }
+            int win = -1;
+            for (int t = 0; t < LANES; ++t) {
+                const int *fc = DISC_FACE + jjs[t] * 3;
+                float F = 0.0f;
+                for (int q = 0; q < 3; ++q) {
+                    const float e = E[fc[q]];
+                    const float vv = (e < min_support_f) ? 0.0f : e;
+                    if (vv == 0.0f) { F = 0.0f; break; }
+                    F = F + vv;
+                }
+                const float mg = PROPOSAL_M * g[jjs[t]];
+                if (F > mg) ++S.stats[3];
+                const bool acc = uas[t] * mg < F;
+                if (acc && win < 0) {
+                    float pf[9];
+                    for (int i = 0; i < 9; ++i) pf[i] = frame[i];
+                    if ((double)data_support(P, first_val, pos, k1s[t], k2s[t], pf) >= min_support) win = t;
+                }
+            }
+            if (win >= 0) {
+                if (k_win) { k_win[0] = k1s[win]; k_win[1] = k2s[win]; }
+                float prop[9], d[3], pf[9];
+                prepare_propagator(k1s[win], k2s[win], (float)(P.step_d / P.p.step_frac), prop);
+                for (int i = 0; i < 9; ++i) pf[i] = frame[i];
+                propagate_frame(prop, pf, d);
+                norm3<0>(d);
+                new_dir[0] = d[0]; new_dir[1] = d[1]; new_dir[2] = d[2];
+                for (int i = 0; i < 9; ++i) frame[i] = pf[i];
+                return 1;
+            }
+        }
         *fell = 1;
     }
     if (!ptt_vertex_supports(P, S, false, frame, pos, first_val, vert_pdf, nullptr)) return 0;
@@ -1134,7 +1226,15 @@ int oracle_set_proposal(oracle_problem *h, const uint16_t *w) {
 }
 int oracle_proposal_stats(oracle_problem *h, long long *out, int reset) {
     Problem *P = reinterpret_cast<Problem *>(h);
-    for (int q = 0; q < 3; ++q) {
+    for (int q = 0; q < 4; ++q) {
+        if (q < 3) out[q] = P->stats[q];
+        if (reset) P->stats[q] = 0;
+    }
+    return 0;
+}
+int oracle_proposal_counters(oracle_problem *h, long long *out, int reset) {
+    Problem *P = reinterpret_cast<Problem *>(h);
+    for (int q = 0; q < 4; ++q) {
         out[q] = P->stats[q];
         if (reset) P->stats[q] = 0;
     }
@@ -1253,7 +1353,7 @@ int oracle_phase2(oracle_problem *h, const float *seeds, int nseeds, int rng_chu
             }
         }
 #pragma omp critical
-        for (int q = 0; q < 3; ++q) P.stats[q] += S.stats[q];
+        for (int q = 0; q < 4; ++q) P.stats[q] += S.stats[q];
     }
     return 0;
 }
@@ -1375,6 +1475,28 @@ int oracle_ptt_substep(oracle_problem *h, const float *pos, const float *frame, 
     win_round[1] = tr.round;
     for (int i = 0; i < 9; ++i) frame_out[i] = fr[i];
     return ok;
+}
+
+// One non-initial learned sub-step at pos with frame on lanes sub0 + lane:
+// returns 1 (a learned round won; k_out its curvature), 2 (the plain
+// fallback ran and found a direction), 0 (no direction); frame_out the
+// frame after it, counters_out the sub-step's proposal counters.
+int oracle_ptt_learned_substep(oracle_problem *h, const float *pos, const float *frame, const float *dir,
+                               long long sub0, float *k_out, float *frame_out, long long *counters_out) {
+    const Problem &P = *reinterpret_cast<Problem *>(h);
+    if (P.p.model_type != CUSLINES_PTT || P.mlp.empty()) return -1;
+    SeedScratch S;
+    S.size(P);
+    for (int l = 0; l < LANES; ++l) S.rng[l].init(P.p.rng_seed, (uint64_t)(sub0 + l));
+    float fr[9];
+    for (int i = 0; i < 9; ++i) fr[i] = frame[i];
+    float nd[3];
+    int fell = 0;
+    k_out[0] = k_out[1] = 0.0f;
+    const int ok = get_direction_ptt_learned(P, S, true, dir, fr, pos, nd, &fell, k_out);
+    for (int i = 0; i < 9; ++i) frame_out[i] = fr[i];
+    for (int q = 0; q < 4; ++q) counters_out[q] = S.stats[q];
+    return ok ? (fell ? 2 : 1) : 0;
 }
 
 // init_frame_ptt_d for one seed and first step on lanes sub0 + lane:

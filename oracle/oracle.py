@@ -15,6 +15,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcuslines_oracle.so")
+# CUSLINES_ORACLE_LIB selects another build of the same source, e.g. the
+# sanitizer build of `make -C oracle sanitize`.
+LOAD_PATH = os.environ.get("CUSLINES_ORACLE_LIB") or LIB_PATH
 
 _lib = None
 
@@ -29,9 +32,9 @@ def build(force: bool = False) -> str:
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        if LOAD_PATH == LIB_PATH and not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(LOAD_PATH)
         vp, fp, ip, up = C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_int32), C.POINTER(C.c_uint32)
         L.oracle_create.restype = vp
         L.oracle_create.argtypes = [vp]
@@ -51,6 +54,8 @@ def lib():
             ("oracle_boot_odf", [vp, fp, C.c_int, C.c_int, fp, fp]),
             ("oracle_set_proposal", [vp, C.POINTER(C.c_uint16)]),
             ("oracle_proposal_stats", [vp, C.POINTER(C.c_longlong), C.c_int]),
+            ("oracle_proposal_counters", [vp, C.POINTER(C.c_longlong), C.c_int]),
+            ("oracle_ptt_learned_substep", [vp, fp, fp, fp, C.c_longlong, fp, fp, C.POINTER(C.c_longlong)]),
         ]:
             getattr(L, name).restype = C.c_int
             getattr(L, name).argtypes = args
@@ -212,6 +217,12 @@ class Oracle:
         self.L.oracle_proposal_stats(self.h, out, int(reset))
         return np.array(list(out), dtype=np.int64)
 
+    def proposal_counters(self, reset=True):
+        """proposal_stats plus [3]: face draws that broke the envelope."""
+        out = (C.c_longlong * 4)()
+        self.L.oracle_proposal_counters(self.h, out, int(reset))
+        return np.array(list(out), dtype=np.int64)
+
     def boot_odf(self, pt, slot=0, phase=0):
         """One bootstrap refit at pt: (status, odf[nverts], signal[dimt])."""
         pt = np.ascontiguousarray(pt, dtype=np.float32)
@@ -242,6 +253,18 @@ class Oracle:
         ok = self.L.oracle_ptt_substep(self.h, _f(np.asarray(pos, np.float32)), _f(fr), int(bool(init)), int(sub0),
                                        _f(k), _f(sup), _i(wr), _f(vp), _f(fo))
         return ok, dict(k=k.reshape(32, 2), sup=sup, win=int(wr[0]), round=int(wr[1]), vert_pdf=vp, frame=fo)
+
+    def ptt_learned_substep(self, pos, frame, sub0=0):
+        """One learned sub-step (the network set with set_learned):
+        (status 1 learned win / 2 plain fallback / 0 none, winner k [2],
+        frame after [9], counters [4])."""
+        k = np.zeros(2, np.float32)
+        fo = np.zeros(9, np.float32)
+        cnt = (C.c_longlong * 4)()
+        fr = np.ascontiguousarray(frame, np.float32).reshape(9)
+        st = self.L.oracle_ptt_learned_substep(self.h, _f(np.asarray(pos, np.float32)), _f(fr), _f(fr[:3].copy()),
+                                               int(sub0), _f(k), _f(fo), cnt)
+        return st, k, fo, np.array(list(cnt), dtype=np.int64)
 
     def ptt_init_frame(self, seed, first_step, sub0=0):
         fo = np.zeros(18, np.float32)

@@ -24,9 +24,10 @@ def test_algorithmic_bytes_per_step():
     assert bench.alg_bytes_per_step("prob", {"dimt": 724}) == 23_212
     # 170 DWI channels (the bootstrap getters gather the signal)
     assert bench.alg_bytes_per_step("csa", {"dimt": 170}) == 5_484
-    # PTT at 10 sub-steps per output point: per sub-step one initial tap, 24
-    # vertices x 8 probes and 32 candidates x 8 probes, each 8 texels + a LUT word
-    assert bench.alg_bytes_per_step("ptt", {"step_frac": 10, "dimt": 724}) == 161_972
-    # per sub-step: 449 taps of 32 + 4 bytes and the stop-map tap
+    # PTT at 10 sub-steps per output point: the taps every sub-step reads,
+    # one tangent tap and 24 vertices x 8 probes, each 8 texels + a LUT word
+    assert bench.alg_bytes_per_step("ptt", {"step_frac": 10, "dimt": 724}) == 69_812
     per_sub = (bench.alg_bytes_per_step("ptt", {"step_frac": 10}) - 12) / 10
-    assert per_sub == 449 * 36 + 32
+    assert per_sub == 193 * 36 + 32
+    # the reference's work adds one full round of 32 candidates x 8 probes
+    assert bench.ptt_reference_bytes_per_step({"step_frac": 10}) == 161_972
