@@ -204,13 +204,23 @@ class DeviceTracker:
                                                 C.c_void_p(offsets_d), C.c_void_p(dirs_d), C.c_void_p(sseed_d),
                                                 C.c_void_p(slen_d), C.c_void_p(sline_d)), "phase2")
 
+    def phase2_noalloc_async(self, seeds_d, n, rng_chunk, offsets_d, dirs_d, sseed_d, slen_d, sline_d):
+        """phase2_async without any allocation by the library: the seed-order
+        workspace is the library's static per-device array (n <= 1,048,576)."""
+        _lib.check(self.L.cuslines_phase2_noalloc_async(self.h, C.c_void_p(seeds_d), int(n), int(rng_chunk),
+                                                        C.c_void_p(offsets_d), C.c_void_p(dirs_d),
+                                                        C.c_void_p(sseed_d), C.c_void_p(slen_d),
+                                                        C.c_void_p(sline_d)), "phase2_noalloc")
+
     def compact_async(self, slen_d, nsl, sline_d, poff_d, pts_d):
         _lib.check(self.L.cuslines_compact_async(self.h, C.c_void_p(slen_d), int(nsl), C.c_void_p(sline_d),
                                                  C.c_void_p(poff_d), C.c_void_p(pts_d)), "compact")
 
     # -- the whole protocol on host arrays ----------------------------------
-    def track(self, seeds, rng_chunk: int = 0, keep_dirs: bool = False, keep_slots: bool = False) -> dict:
+    def track(self, seeds, rng_chunk: int = 0, keep_dirs: bool = False, keep_slots: bool = False,
+              noalloc: bool = False) -> dict:
         """Phase 1, scan, phase 2 and compaction for ``seeds`` (float32 [n,3]).
+        ``noalloc`` runs phase 2 through the entry point that allocates nothing.
 
         Returns host arrays: ``counts`` [n], ``offsets`` [n+1],
         ``sline_seed``/``sline_len`` [nsl], ``points`` [sum(len),3] and
@@ -240,7 +250,8 @@ class DeviceTracker:
             ssd = self.alloc(4 * nsl); bufs.append(ssd)
             sld = self.alloc(4 * nsl); bufs.append(sld)
             sl = self.alloc(4 * 3 * 2 * L * nsl); bufs.append(sl)
-            self.phase2_async(sd.ptr, n, rng_chunk, od.ptr, dd.ptr, ssd.ptr, sld.ptr, sl.ptr)
+            p2 = self.phase2_noalloc_async if noalloc else self.phase2_async
+            p2(sd.ptr, n, rng_chunk, od.ptr, dd.ptr, ssd.ptr, sld.ptr, sl.ptr)
             slen = sld.download(np.zeros(nsl, dtype=np.int32))
             out["sline_seed"] = ssd.download(np.zeros(nsl, dtype=np.int32))
             out["sline_len"] = slen

@@ -113,6 +113,7 @@ def load():
         "cuslines_phase1_async": [vp, vp, i32, i32, vp, vp],
         "cuslines_scan_async": [vp, vp, i32, vp],
         "cuslines_phase2_async": [vp, vp, i32, i32, vp, vp, vp, vp, vp],
+        "cuslines_phase2_noalloc_async": [vp, vp, i32, i32, vp, vp, vp, vp, vp],
         "cuslines_compact_async": [vp, vp, i32, vp, vp, vp],
     }
     for name, args in sigs.items():

@@ -2923,6 +2923,14 @@ __global__ void compact_kernel(const int *__restrict__ len, int nsl, const float
 
 }  // namespace
 
+// Workspace of cuslines_phase2_noalloc_async: the spatial seed order of at
+// most CUSLINES_NOALLOC_MAX_SEEDS seeds (order and keys), its bucket counts
+// and starts (at most 2^21 buckets) and the scan's block sums.  One copy per
+// device for the life of the process, shared by every call on that device.
+constexpr size_t SCRATCH_INTS = 2 * (size_t)CUSLINES_NOALLOC_MAX_SEEDS + 2 * ((size_t)1 << 21) + 1 + 2048;
+constexpr size_t SCRATCH_BYTES = SCRATCH_INTS * sizeof(int);
+static __device__ __align__(256) int scratch[SCRATCH_INTS];
+
 // =========================================================================
 // Host side: C ABI
 // =========================================================================
@@ -2962,6 +2970,7 @@ struct cuslines_tracker {
     uint32_t *d_lut = nullptr;
     std::vector<uint16_t> rank_vertex;  // PTT: vertex of each direction rank (corner-block order)
     int *d_work = nullptr;      // work-queue counters of the persistent phase-2 kernels (QUEUES)
+    void *scratch = nullptr;    // device address of this device's `scratch` array
     float *d_blocks = nullptr;  // PTT corner blocks
     int use_blocks = 1;
     int schedule = CUSLINES_SCHED_QUEUE;
@@ -3009,6 +3018,25 @@ static DevBufs bufs_of(const cuslines_tracker *t) {
 
 static void dfree(void *p) {
     if (p) (void)hipFree(p);
+}
+
+// Device address of `scratch` on each device, looked up once per device with
+// that device current (each device holds its own copy of the array).
+static std::mutex g_scratch_mu;
+static std::vector<void *> g_scratch_ptr;
+
+static int scratch_of_device(int device, void **out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(CUSLINES_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return CUSLINES_OK;
 }
 
 static int check_ready(const cuslines_tracker *t) {
@@ -3261,19 +3289,32 @@ static int run_scan(cuslines_tracker *t, const int *in, int n, TO *out, TO *bsum
     return CUSLINES_OK;
 }
 
-// The phase-2 entry point; ws (optional) is a seed-order workspace of
-// order_ws_ints(nseeds, buckets) ints in place of stream-ordered allocations.
+// Both phase-2 entry points.  noalloc: the seed-order workspace is this
+// device's `scratch` array instead of stream-ordered allocations.
 static int phase2_entry(cuslines_tracker *t, const float *seeds_d, int32_t nseeds, int32_t rng_chunk,
                         const int32_t *offsets_d, const int32_t *dirs_d, int32_t *sline_seed_d, int32_t *sline_len_d,
-                        float *sline_d, int *ws) {
+                        float *sline_d, bool noalloc) {
     RoctxRange range_("cuslines_phase2");
     if (!t) return fail(CUSLINES_EINVAL, "null tracker");
     if (nseeds < 0 || rng_chunk < 0) return fail(CUSLINES_EINVAL, "negative size");
+    if (noalloc) {
+        if (!t->scratch) return fail(CUSLINES_ESTATE, "no scratch array on this device");
+        if (nseeds > CUSLINES_NOALLOC_MAX_SEEDS)
+            return fail(CUSLINES_EINVAL, "nseeds exceeds CUSLINES_NOALLOC_MAX_SEEDS (" +
+                                             std::to_string(CUSLINES_NOALLOC_MAX_SEEDS) + ")");
+    }
     if (nseeds == 0) return CUSLINES_OK;
     if (!seeds_d || !offsets_d || !dirs_d || !sline_seed_d || !sline_len_d || !sline_d)
         return fail(CUSLINES_EINVAL, "null buffer");
     int rc = check_ready(t);
     if (rc) return rc;
+    int *ws = nullptr;
+    if (noalloc) {
+        int brick_log2 = 0;
+        if (order_ws_ints(nseeds, order_buckets(t->dp, &brick_log2)) * sizeof(int) > SCRATCH_BYTES)
+            return fail(CUSLINES_EINVAL, "seed-order workspace exceeds the scratch array");
+        ws = static_cast<int *>(t->scratch);
+    }
     HIPC(hipSetDevice(t->device));
     const int chunk = rng_chunk > 0 ? rng_chunk : nseeds;
     switch (t->p.model_type) {
@@ -3357,6 +3398,12 @@ int cuslines_tracker_create(const cuslines_params *p, int device, cuslines_track
         (void)hipStreamDestroy(t->stream);
         delete t;
         return fail(CUSLINES_EHIP, std::string("tracker setup: ") + hipGetErrorString(e));
+    }
+    if (int rc = scratch_of_device(device, &t->scratch)) {
+        dfree(t->d_work);
+        (void)hipStreamDestroy(t->stream);
+        delete t;
+        return rc;
     }
     unsigned char net[SORT_LAYERS][32];
     int nl = 0;
@@ -4057,8 +4104,13 @@ int cuslines_scan_async(cuslines_tracker *t, const int32_t *counts_d, int32_t ns
 int cuslines_phase2_async(cuslines_tracker *t, const float *seeds_d, int32_t nseeds, int32_t rng_chunk,
                           const int32_t *offsets_d, const int32_t *dirs_d, int32_t *sline_seed_d,
                           int32_t *sline_len_d, float *sline_d) {
-    return phase2_entry(t, seeds_d, nseeds, rng_chunk, offsets_d, dirs_d, sline_seed_d, sline_len_d, sline_d,
-                        nullptr);
+    return phase2_entry(t, seeds_d, nseeds, rng_chunk, offsets_d, dirs_d, sline_seed_d, sline_len_d, sline_d, false);
+}
+
+int cuslines_phase2_noalloc_async(cuslines_tracker *t, const float *seeds_d, int32_t nseeds, int32_t rng_chunk,
+                                  const int32_t *offsets_d, const int32_t *dirs_d, int32_t *sline_seed_d,
+                                  int32_t *sline_len_d, float *sline_d) {
+    return phase2_entry(t, seeds_d, nseeds, rng_chunk, offsets_d, dirs_d, sline_seed_d, sline_len_d, sline_d, true);
 }
 
 int cuslines_compact_async(cuslines_tracker *t, const int32_t *sline_len_d, int32_t nslines, const float *sline_d,

@@ -318,6 +318,26 @@ int cuslines_phase2_async(cuslines_tracker *t, const float *seeds_d, int32_t nse
                           const int32_t *dirs_d, int32_t *sline_seed_d,
                           int32_t *sline_len_d, float *sline_d);
 
+/*
+ * Phase 2 without allocation: cuslines_phase2_async with the same arguments,
+ * kernels and results, except that it calls no hipMalloc / hipMallocAsync /
+ * hipFree of its own.  No counterpart in the reference; it is for callers
+ * that cannot allocate (code under HIP graph capture, serving loops,
+ * harnesses that forbid allocation).  The spatial seed order that the
+ * persistent queue reads (order, keys, bucket counts, scan block sums) lives
+ * in a static device array of the library (about 24 MiB, one per device,
+ * held until the process ends) instead of stream-ordered allocations.
+ * nseeds above CUSLINES_NOALLOC_MAX_SEEDS is refused with CUSLINES_EINVAL.
+ * Every call on one device shares that array: calls that overlap on one
+ * device (several trackers or streams) race and must be serialised by the
+ * caller.
+ */
+#define CUSLINES_NOALLOC_MAX_SEEDS 1048576
+int cuslines_phase2_noalloc_async(cuslines_tracker *t, const float *seeds_d, int32_t nseeds,
+                                  int32_t rng_chunk, const int32_t *offsets_d,
+                                  const int32_t *dirs_d, int32_t *sline_seed_d,
+                                  int32_t *sline_len_d, float *sline_d);
+
 /* Compact padded slots into one point array: points_d[sum(len)][3], with
  * point_off_d int64 [nslines + 1] the exclusive scan of sline_len_d. */
 int cuslines_compact_async(cuslines_tracker *t, const int32_t *sline_len_d, int32_t nslines,
