@@ -238,11 +238,13 @@ def measure_end_to_end(args, w, seeds, kernel_steps):
     elif args.getter == "ptt":
         dg, data = PttDirectionGetter(), w.extra["odf"]
     else:
-        dg, data = BootDirectionGetter(args.getter.upper(), p["min_signal"], **w.boot), w.dataf
+        dg, data = BootDirectionGetter(args.getter.upper(), p["min_signal"], **w.boot,
+                                       mfma=args.boot_mode == "mfma"), w.dataf
     t0 = time.perf_counter()
     tr = GPUTracker(dg, data, w.stop_map, p["tc_threshold"], w.vertices, w.edges, max_angle=p["max_angle"],
                     step_size=p["step_size"], relative_peak_thresh=p["relative_peak_thresh"],
-                    min_separation_angle=p["min_separation_angle"], chunk_size=CHUNK)
+                    min_separation_angle=p["min_separation_angle"], chunk_size=CHUNK,
+                    launch_seeds=args.launch_seeds, chunk_pipeline=args.chunk_pipeline)
     with tr:
         prep = time.perf_counter() - t0
         tr.generate_streamlines(seeds[:CHUNK])
@@ -254,7 +256,9 @@ def measure_end_to_end(args, w, seeds, kernel_steps):
     print(f"[bench] end to end: {el:.2f}s, {steps / el / 1e6:.2f} M steps/s (prep {prep:.1f}s)", file=sys.stderr,
           flush=True)
     return {"steps_per_s": steps / el, "s": el, "streamlines": int(len(lens)), "points": int(lens.sum()),
-            "chunks": -(-len(seeds) // CHUNK), "prep_s": prep, "same_steps_as_kernel_pass": steps == kernel_steps}
+            "chunks": -(-len(seeds) // CHUNK), "launch_seeds": args.launch_seeds,
+            "chunk_pipeline": args.chunk_pipeline, "prep_s": prep,
+            "same_steps_as_kernel_pass": steps == kernel_steps}
 
 
 def copy_calibration(nbytes):
@@ -293,6 +297,10 @@ def main():
                     help="PTT: time the plain getter, the learned proposal, or both (value = plain)")
     ap.add_argument("--end-to-end", action="store_true",
                     help="also time GPUTracker.generate_streamlines over 25,000-seed chunks (host copies included)")
+    ap.add_argument("--launch-seeds", type=int, default=250000,
+                    help="end to end: seeds per launch (0: one launch per 25,000-seed chunk, the reference's protocol)")
+    ap.add_argument("--chunk-pipeline", type=int, default=4,
+                    help="end to end with one launch per chunk: chunks in flight at once (1: one after the other)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (counter calibration)")
     args = ap.parse_args()

@@ -57,6 +57,17 @@ class DeviceTracker:
         self.h = h.value
         self.device = device
 
+    def fork(self) -> "DeviceTracker":
+        """A handle sharing this tracker's device data with its own stream
+        (cuslines_tracker_fork): consecutive chunks on different forks run
+        concurrently.  Close forks before this tracker."""
+        f = DeviceTracker.__new__(DeviceTracker)
+        f.L, f.params, f._ps, f.device = self.L, dict(self.params), self._ps, self.device
+        h = C.c_void_p()
+        _lib.check(self.L.cuslines_tracker_fork(self.h, C.byref(h)), "tracker_fork")
+        f.h = h.value
+        return f
+
     # -- setup ---------------------------------------------------------------
     def upload(self, dataf, stop_map, vertices, edges, boot=None, ptt_lut=None):
         """``dataf`` None: the data come later (``ptt_prepare``)."""
@@ -160,6 +171,13 @@ class DeviceTracker:
         matrix cores; equal within float rounding).  waves_per_simd (2 or 4,
         0 = default) picks the MFMA kernel's occupancy variant."""
         _lib.check(self.L.cuslines_set_boot_mode(self.h, int(mode) | (int(waves_per_simd) << 8)), "set_boot_mode")
+
+    def boot_counters(self, reset: bool = True) -> np.ndarray:
+        """MFMA bootstrap kernel: [slot attempts, workgroup rounds] since the
+        last reset."""
+        out = (C.c_int64 * 2)()
+        _lib.check(self.L.cuslines_boot_counters(self.h, out, int(reset)), "boot_counters")
+        return np.array(list(out), dtype=np.int64)
 
     def set_tap_count(self, enable: bool = True):
         """PTT phase 2 counts its ODF taps and sub-steps (same results)."""

@@ -85,6 +85,7 @@ def load():
         "cuslines_device_count": [C.POINTER(C.c_int)],
         "cuslines_tracker_create": [C.POINTER(cuslines_params), C.c_int, C.POINTER(vp)],
         "cuslines_tracker_destroy": [vp],
+        "cuslines_tracker_fork": [vp, C.POINTER(vp)],
         "cuslines_set_data": [vp, vp],
         "cuslines_set_stop_map": [vp, vp],
         "cuslines_set_sphere": [vp, vp, vp],
@@ -101,6 +102,7 @@ def load():
         "cuslines_set_ptt_layout": [vp, C.c_int],
         "cuslines_set_lanes_per_seed": [vp, C.c_int],
         "cuslines_set_boot_mode": [vp, C.c_int],
+        "cuslines_boot_counters": [vp, C.POINTER(C.c_int64), C.c_int],
         "cuslines_set_proposal": [vp, vp],
         "cuslines_set_recording": [vp, vp, i32, vp, C.c_float],
         "cuslines_proposal_stats": [vp, C.POINTER(C.c_int64), C.c_int],

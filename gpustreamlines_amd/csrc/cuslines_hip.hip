@@ -2434,6 +2434,7 @@ __global__ void __launch_bounds__(BB_THREADS, OCC)
     Philox rng;
     bb_next_seed(P, B, st, rng, nseeds, seeds, rng_chunk, offsets, dirs, order, work, sline);
     if (!st.done) bb_start_pass(P, st, sline);
+    unsigned long long attempts = 0, rounds = 0;  // slot attempts and rounds of this workgroup
     for (;;) {
         // ---- phase A: gather the signal at the point -----------------------
         const bool active = !st.done;
@@ -2445,7 +2446,10 @@ __global__ void __launch_bounds__(BB_THREADS, OCC)
                 for (int m = l; m < hr16; m += 32) S.msk[m] = (m < hr) ? S.vox[B.dwi_idx[m]] : 0.0f;
             }
         }
-        if (!__syncthreads_or(active)) break;
+        const int nact = __syncthreads_count(active);
+        if (nact == 0) break;
+        attempts += (unsigned long long)nact;
+        ++rounds;
         // ---- phase B: H.d and R.d for the 16 slots --------------------------
         bb_gemm<(2 * 320 / 16 + BB_WAVES - 1) / BB_WAVES>(B.hr_mat, hr16, hr16, 2 * hr16 / 16, smem + off_msk,
                                                           smem + off_msk, hr16, smem + off_h, smem + off_r, hr16,
@@ -2536,6 +2540,10 @@ __global__ void __launch_bounds__(BB_THREADS, OCC)
         }
         // no barrier here: phase A writes only this slot's region A, which
         // phase E of every wave has finished reading (barrier above)
+    }
+    if (threadIdx.x == 0 && B.stats) {
+        atomicAdd(B.stats, attempts);
+        atomicAdd(B.stats + 1, rounds);
     }
 }
 
@@ -2985,6 +2993,8 @@ struct cuslines_tracker {
     float *rec = nullptr;                // caller's training-pair buffer (nullptr: no recording)
     int *rec_count = nullptr;
     bool has_data = false, has_stop = false, has_sphere = false, has_boot = false, has_lut = false;
+    cuslines_tracker *parent = nullptr;  // fork: the tracker whose device data it shares
+    int nforks = 0;                      // live forks of this tracker
 };
 
 static float smallest_float_at_least(double x) {
@@ -3045,6 +3055,14 @@ static int check_ready(const cuslines_tracker *t) {
     if ((t->p.model_type == CUSLINES_OPDT || t->p.model_type == CUSLINES_CSA) && !t->has_boot)
         return fail(CUSLINES_ESTATE, "bootstrap matrices must be uploaded first");
     if (t->p.model_type == CUSLINES_PTT && !t->has_lut) return fail(CUSLINES_ESTATE, "PTT LUT must be uploaded first");
+    return CUSLINES_OK;
+}
+
+// Setters that replace device data are refused on forks (they share their
+// parent's buffers) and on trackers with live forks.
+static int check_owner(const cuslines_tracker *t) {
+    if (t->parent) return fail(CUSLINES_ESTATE, "a fork shares its parent's data: configure the parent");
+    if (t->nforks) return fail(CUSLINES_ESTATE, "live forks share this tracker's data: destroy them first");
     return CUSLINES_OK;
 }
 
@@ -3233,6 +3251,10 @@ static int launch_boot_batch(cuslines_tracker *t, const float *seeds, int n, int
     const long long need = (n + BB_SLOTS - 1) / BB_SLOTS;
     if (grid > need) grid = need;
     if (grid < 1) grid = 1;
+    if (!t->d_stats && !ws) {  // the kernel's attempt and round counters (not allocated by the no-alloc entry)
+        HIPC(hipMalloc(&t->d_stats, 4 * sizeof(unsigned long long)));
+        HIPC(hipMemsetAsync(t->d_stats, 0, 4 * sizeof(unsigned long long), t->stream));
+    }
     int *order = nullptr;
     if (int rc = spatial_order(t, seeds, n, &order, ws)) return rc;
     hipLaunchKernelGGL(zero_ints, dim3(1), dim3(QUEUES), 0, t->stream, t->d_work, QUEUES);
@@ -3448,8 +3470,17 @@ int cuslines_tracker_create(const cuslines_params *p, int device, cuslines_track
 
 int cuslines_tracker_destroy(cuslines_tracker *t) {
     if (!t) return CUSLINES_OK;
+    if (t->nforks) return fail(CUSLINES_ESTATE, "destroy the tracker's forks first");
     (void)hipSetDevice(t->device);
     if (t->stream) (void)hipStreamSynchronize(t->stream);
+    if (t->parent) {  // a fork owns only its stream and counters
+        --t->parent->nforks;
+        dfree(t->d_work);
+        dfree(t->d_stats);
+        if (t->stream) (void)hipStreamDestroy(t->stream);
+        delete t;
+        return CUSLINES_OK;
+    }
     dfree(t->d_data); dfree(t->d_stop); dfree(t->d_verts); dfree(t->d_edges); dfree(t->d_nbr);
     dfree(t->d_H); dfree(t->d_R); dfree(t->d_db); dfree(t->d_dq); dfree(t->d_smat);
     dfree(t->d_hrm); dfree(t->d_fitm); dfree(t->d_odfm);
@@ -3457,6 +3488,33 @@ int cuslines_tracker_destroy(cuslines_tracker *t) {
     dfree(t->d_fprop); dfree(t->d_mlp); dfree(t->d_stats);
     if (t->stream) (void)hipStreamDestroy(t->stream);
     delete t;
+    return CUSLINES_OK;
+}
+
+int cuslines_tracker_fork(cuslines_tracker *t, cuslines_tracker **out) {
+    if (!t || !out) return fail(CUSLINES_EINVAL, "null argument");
+    *out = nullptr;
+    if (t->parent) return fail(CUSLINES_EINVAL, "fork the parent tracker, not a fork");
+    HIPC(hipSetDevice(t->device));
+    cuslines_tracker *f = new cuslines_tracker(*t);  // shares every device buffer pointer
+    f->parent = t;
+    f->nforks = 0;
+    f->stream = nullptr;
+    f->d_work = nullptr;
+    f->d_stats = nullptr;  // learned statistics and tap counts stay with the parent
+    f->rec = nullptr;
+    f->rec_count = nullptr;
+    f->count_taps = 0;
+    hipError_t e = hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&f->d_work, QUEUES * sizeof(int));
+    if (e != hipSuccess) {
+        dfree(f->d_work);
+        if (f->stream) (void)hipStreamDestroy(f->stream);
+        delete f;
+        return fail(CUSLINES_EHIP, std::string("tracker fork: ") + hipGetErrorString(e));
+    }
+    ++t->nforks;
+    *out = f;
     return CUSLINES_OK;
 }
 
@@ -3513,6 +3571,7 @@ static int ptt_take_data(cuslines_tracker *t, const float *src, int nonneg) {
 int cuslines_set_data(cuslines_tracker *t, const float *dataf) {
     RoctxRange range_("cuslines_set_data");
     if (!t || !dataf) return fail(CUSLINES_EINVAL, "null argument");
+    if (int rc = check_owner(t)) return rc;
     HIPC(hipSetDevice(t->device));
     const DevParams &d = t->dp;
     const size_t nvox = (size_t)d.dimx * d.dimy * d.dimz;
@@ -3546,6 +3605,7 @@ int cuslines_set_data(cuslines_tracker *t, const float *dataf) {
 
 int cuslines_set_stop_map(cuslines_tracker *t, const float *stop_map) {
     if (!t || !stop_map) return fail(CUSLINES_EINVAL, "null argument");
+    if (int rc = check_owner(t)) return rc;
     HIPC(hipSetDevice(t->device));
     const size_t n = (size_t)t->dp.dimx * t->dp.dimy * t->dp.dimz;
     // A failed re-upload must not leave the tracker ready with freed buffers.
@@ -3560,6 +3620,7 @@ int cuslines_set_stop_map(cuslines_tracker *t, const float *stop_map) {
 
 int cuslines_set_sphere(cuslines_tracker *t, const float *vertices, const int32_t *edges) {
     if (!t || !vertices || !edges) return fail(CUSLINES_EINVAL, "null argument");
+    if (int rc = check_owner(t)) return rc;
     HIPC(hipSetDevice(t->device));
     const int nv = t->dp.nverts, ne = t->dp.nedges;
     std::vector<uint32_t> packed(ne);
@@ -3605,6 +3666,7 @@ int cuslines_set_boot_matrices(cuslines_tracker *t, const float *H, const float 
                                const float *delta_q, const float *sampling_matrix, const int32_t *b0s_mask) {
     if (!t || !H || !R || !delta_b || !delta_q || !sampling_matrix || !b0s_mask)
         return fail(CUSLINES_EINVAL, "null argument");
+    if (int rc = check_owner(t)) return rc;
     if (t->p.model_type != CUSLINES_OPDT && t->p.model_type != CUSLINES_CSA)
         return fail(CUSLINES_ESTATE, "bootstrap matrices need an OPDT or CSA tracker");
     HIPC(hipSetDevice(t->device));
@@ -3680,6 +3742,7 @@ int cuslines_set_boot_matrices(cuslines_tracker *t, const float *H, const float 
 
 int cuslines_set_ptt_lut(cuslines_tracker *t, const int32_t *lut) {
     if (!t || !lut) return fail(CUSLINES_EINVAL, "null argument");
+    if (int rc = check_owner(t)) return rc;
     if (t->p.model_type != CUSLINES_PTT) return fail(CUSLINES_ESTATE, "LUT needs a PTT tracker");
     HIPC(hipSetDevice(t->device));
     const size_t r = (size_t)t->dp.lut_res;
@@ -3812,6 +3875,7 @@ int cuslines_ptt_prepare(cuslines_tracker *t, const float *odf, const float *sto
                          int sphere_symm, int32_t *lut_out) {
     RoctxRange range_("cuslines_ptt_prepare");
     if (!t || !odf || !stop_map) return fail(CUSLINES_EINVAL, "null argument");
+    if (int rc = check_owner(t)) return rc;
     if (t->p.model_type != CUSLINES_PTT) return fail(CUSLINES_ESTATE, "PTT preparation needs a PTT tracker");
     if (!t->has_sphere) return fail(CUSLINES_ESTATE, "upload the sphere before preparing PTT data");
     if (t->dp.lut_res < 2) return fail(CUSLINES_EINVAL, "lut_res must be at least 2");
@@ -3875,6 +3939,7 @@ int cuslines_ptt_prepare(cuslines_tracker *t, const float *odf, const float *sto
 
 int cuslines_set_ptt_layout(cuslines_tracker *t, int corner_blocks) {
     if (!t) return fail(CUSLINES_EINVAL, "null tracker");
+    if (int rc = check_owner(t)) return rc;
     t->use_blocks = corner_blocks ? 1 : 0;
     return CUSLINES_OK;
 }
@@ -3890,6 +3955,7 @@ static int ensure_feature_propagators(cuslines_tracker *t) {
 
 int cuslines_set_proposal(cuslines_tracker *t, const uint16_t *mlp_bf16) {
     if (!t) return fail(CUSLINES_EINVAL, "null tracker");
+    if (int rc = check_owner(t)) return rc;
     HIPC(hipSetDevice(t->device));
     if (!mlp_bf16) {
         dfree(t->d_mlp);
@@ -3919,6 +3985,7 @@ int cuslines_set_proposal(cuslines_tracker *t, const uint16_t *mlp_bf16) {
 
 int cuslines_set_recording(cuslines_tracker *t, float *pairs_d, int32_t capacity, int32_t *count_d, float rate) {
     if (!t) return fail(CUSLINES_EINVAL, "null tracker");
+    if (int rc = check_owner(t)) return rc;
     if (!pairs_d) {
         t->rec = nullptr;
         t->rec_count = nullptr;
@@ -3959,6 +4026,22 @@ int cuslines_proposal_counters(cuslines_tracker *t, int64_t *out4, int reset) {
         if (reset) HIPC(hipMemset(t->d_stats, 0, sizeof(h)));
     }
     for (int i = 0; i < 4; ++i) out4[i] = (int64_t)h[i];
+    return CUSLINES_OK;
+}
+
+int cuslines_boot_counters(cuslines_tracker *t, int64_t *out2, int reset) {
+    if (!t || !out2) return fail(CUSLINES_EINVAL, "null argument");
+    if (t->p.model_type != CUSLINES_OPDT && t->p.model_type != CUSLINES_CSA)
+        return fail(CUSLINES_EINVAL, "bootstrap counters are for OPDT and CSA trackers");
+    HIPC(hipSetDevice(t->device));
+    unsigned long long h[4] = {0, 0, 0, 0};
+    if (t->d_stats) {
+        HIPC(hipStreamSynchronize(t->stream));
+        HIPC(hipMemcpy(h, t->d_stats, sizeof(h), hipMemcpyDeviceToHost));
+        if (reset) HIPC(hipMemset(t->d_stats, 0, sizeof(h)));
+    }
+    out2[0] = (int64_t)h[0];
+    out2[1] = (int64_t)h[1];
     return CUSLINES_OK;
 }
 

@@ -87,7 +87,13 @@ class BootDirectionGetter(GPUDirectionGetter):
 
     def __init__(self, model_type: str, min_signal: float, H: np.ndarray, R: np.ndarray,
                  delta_b: np.ndarray, delta_q: np.ndarray, sampling_matrix: np.ndarray,
-                 b0s_mask: np.ndarray):
+                 b0s_mask: np.ndarray, mfma: bool = True):
+        """``mfma`` (not in the reference): True runs phase 2 on the batched
+        matrix-core kernel (16 seeds in lockstep; results equal the
+        reference's within float rounding, peak choices flip only at
+        near-ties); False runs the per-seed kernel that is bit-identical to
+        the CPU oracle."""
+        self.mfma = bool(mfma)
         mt = str(model_type).upper()
         if mt not in ("OPDT", "CSA"):
             raise ValueError(f"Invalid model_type {model_type}, must be one of 'OPDT', 'CSA'")
@@ -121,7 +127,9 @@ class BootDirectionGetter(GPUDirectionGetter):
                     sampling_matrix=self.sampling_matrix, b0s_mask=self.b0s_mask)
 
     def allocate_on_gpu(self, n):
-        self.gpu_tracker.device_trackers[n].set_boot(**self.boot_arrays())
+        dt = self.gpu_tracker.device_trackers[n]
+        dt.set_boot(**self.boot_arrays())
+        dt.set_boot_mode(dt.BOOT_MFMA if self.mfma else dt.BOOT_EXACT)
 
     def getNumStreamlines(self, n, nseeds_gpu, block, grid, sp):
         self._phase1(n, nseeds_gpu, sp)

@@ -66,6 +66,20 @@ class GenericTracker:
         per = 1
         if self._ngpus() == 1:
             per = max(1, int(getattr(self, "launch_seeds", 0) or 0) // gsz)
+            depth = int(getattr(self, "chunk_pipeline", 1) or 1)
+            if per == 1 and depth > 1 and nchunks > 1:
+                # one launch per chunk, consecutive chunks overlapping on
+                # handles that share the device data
+                from .propagate_seeds import ChunkPipeline
+
+                pipe = ChunkPipeline(self.device_trackers[0], depth=depth, minlen=sp.minlen, maxlen=sp.maxlen)
+                try:
+                    for pts, lens in pipe.run(seeds, gsz):
+                        if len(lens):
+                            yield pts, lens
+                finally:
+                    pipe.close()
+                return
         sp.rng_chunk = gsz if per > 1 else None
         for idx in range(0, nchunks, per):
             sp.propagate(seeds[idx * gsz: (idx + per) * gsz])

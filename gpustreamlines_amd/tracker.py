@@ -30,13 +30,17 @@ class GPUTracker(GenericTracker):
                  sphere_symm: bool = False, max_angle: float = radians(60), step_size: float = 0.5,
                  min_pts=0, max_pts=np.inf, relative_peak_thresh: float = 0.25,
                  min_separation_angle: float = radians(45), ngpus: int = 1, rng_seed: int = 0,
-                 rng_offset: int = 0, chunk_size: int = 25000, launch_seeds: int = 250000):
+                 rng_offset: int = 0, chunk_size: int = 25000, launch_seeds: int = 250000,
+                 chunk_pipeline: int = 4):
         """Same parameters as the reference GPUTracker
         (cu_tractography.py:57-109); ``step_size`` is in voxels.
         ``launch_seeds`` (not in the reference): on one GPU, chunks are
         grouped so that one launch tracks about this many seeds; streams stay
         chunk-relative, so the output does not depend on it (0: one chunk per
-        launch)."""
+        launch).  ``chunk_pipeline`` (not in the reference): with one chunk
+        per launch, this many chunks run concurrently on handles sharing the
+        device data (1: one after the other); the output does not depend on
+        it."""
         _lib.load()
         self.dimx, self.dimy, self.dimz, self.dimt = dataf.shape
         if hasattr(dg, "prepare_data"):
@@ -70,6 +74,7 @@ class GPUTracker(GenericTracker):
         self.rng_offset = int(rng_offset)
         self.chunk_size = int(chunk_size)
         self.launch_seeds = int(launch_seeds)
+        self.chunk_pipeline = int(chunk_pipeline)
         self.sphere_symm = bool(sphere_symm)
         avail = _lib.device_count()
         if self.ngpus > avail:

@@ -127,6 +127,19 @@ int cuslines_tracker_create(const cuslines_params *params, int device, cuslines_
 /* Free every device buffer and the stream (cu_tractography.py:260-300). */
 int cuslines_tracker_destroy(cuslines_tracker *t);
 
+/* A second handle on the same device that shares t's device data (volume,
+ * stop map, sphere, getter matrices, PTT LUT and blocks, proposal network)
+ * and has its own stream and queue counters, so consecutive seed chunks can
+ * run concurrently: a chunk's phase 2 fills the GPU while the previous
+ * chunk's longest streamlines finish.  Replaces nothing in the reference,
+ * whose chunks run one after the other on one stream per GPU
+ * (cuslines/cuda_python/cu_propagate_seeds.py:206-233).  Configure t before
+ * forking: calls that replace device data are refused on a fork and on a
+ * tracker with live forks (CUSLINES_ESTATE); destroy forks before t.  The
+ * fork copies t's launch and kernel settings and has no learned-proposal
+ * statistics or pair recording of its own. */
+int cuslines_tracker_fork(cuslines_tracker *t, cuslines_tracker **out);
+
 /* Upload the data volume: DWI for bootstrap, ODF amplitudes for prob, the
  * prepared ODF (clip >= 0, zero outside the stop mask, max-normalised) for
  * PTT.  Host float32 [dimx][dimy][dimz][dimt] (cu_tractography.py:204-221).
@@ -200,6 +213,13 @@ enum {
  * of `mode` choose the MFMA kernel's occupancy, 2 (default) or 4 waves per
  * SIMD (the autotuner's knob). */
 int cuslines_set_boot_mode(cuslines_tracker *t, int mode);
+
+/* Work counters of the MFMA bootstrap kernel since the last reset: out[0]
+ * getter attempts of all slots (each one refit and ODF projection, i.e. one
+ * column of the batched products), out[1] lockstep rounds of all workgroups
+ * (each round runs the products for 16 columns); reset != 0 zeroes them.
+ * attempts / (16 rounds) is the occupancy of the batched products. */
+int cuslines_boot_counters(cuslines_tracker *t, int64_t *out2, int reset);
 
 /* PTT phase 2 with `lanes` (8, 16 or 32) lanes per seed, 64/lanes seeds per
  * wavefront.  The reference's 32 per-lane random streams per seed are kept as

@@ -24,11 +24,12 @@ def _oracle_by_chunks(w, seeds, chunk):
     return np.concatenate(pts), np.concatenate(lens)
 
 
-@pytest.mark.parametrize("launch_seeds", [0, 250000])
+@pytest.mark.parametrize("launch_seeds,pipeline", [(0, 1), (0, 4), (250000, 1)])
 @pytest.mark.parametrize("kind", ["prob", "ptt", "csa"])
-def test_gpu_tracker_generate_streamlines(small_workloads, kind, launch_seeds):
-    """launch_seeds 0: one chunk per launch (the reference's protocol);
-    250000: all chunks in one launch with chunk-relative streams."""
+def test_gpu_tracker_generate_streamlines(small_workloads, kind, launch_seeds, pipeline):
+    """launch_seeds 0: one chunk per launch (the reference's protocol), one
+    chunk after the other (pipeline 1) or 4 chunks in flight on forked
+    handles; 250000: all chunks in one launch with chunk-relative streams."""
     from gpustreamlines_amd import (BootDirectionGetter, GPUTracker, ProbDirectionGetter, PttDirectionGetter)
     from gpustreamlines_amd import workloads as W
 
@@ -41,10 +42,10 @@ def test_gpu_tracker_generate_streamlines(small_workloads, kind, launch_seeds):
     elif kind == "ptt":
         dg, data = PttDirectionGetter(), small_workloads["prob"].dataf  # raw ODF; the getter prepares it
     else:
-        dg, data = BootDirectionGetter("CSA", 1.0, **{k: v for k, v in w.boot.items()}), w.dataf
-    print(f"[api] {kind} launch_seeds={launch_seeds}", flush=True)
+        dg, data = BootDirectionGetter("CSA", 1.0, **{k: v for k, v in w.boot.items()}, mfma=False), w.dataf
+    print(f"[api] {kind} launch_seeds={launch_seeds} pipeline={pipeline}", flush=True)
     with GPUTracker(dg, data, w.stop_map, p["tc_threshold"], w.vertices, w.edges, step_size=p["step_size"],
-                    chunk_size=chunk, launch_seeds=launch_seeds) as tr:
+                    chunk_size=chunk, launch_seeds=launch_seeds, chunk_pipeline=pipeline) as tr:
         out = tr.generate_streamlines(seeds)
     pts, lens = _oracle_by_chunks(w, seeds, chunk)
     assert np.array_equal(out._lengths, lens)
@@ -80,3 +81,35 @@ def test_ptt_prepare_on_device_matches_host(small_workloads):
     assert len(out["host"]._lengths) > 0
     assert np.array_equal(out["host"]._lengths, out["device"]._lengths)
     assert np.array_equal(out["host"]._data, out["device"]._data)
+
+
+def test_fork_shares_data_and_refuses_setters(small_workloads):
+    """A fork tracks with its parent's device data on its own stream (same
+    output); setters that replace device data are refused on the fork and on
+    the parent while the fork lives, and the parent cannot be destroyed
+    before it."""
+    from gpustreamlines_amd import _lib
+    from gpustreamlines_amd import workloads as W
+    from gpustreamlines_amd._device import DeviceTracker
+
+    w = small_workloads["prob"]
+    seeds = W.seeds_for(w, 64, placement="mixed", seed=3)
+    dt = DeviceTracker(w.params, device=0)
+    try:
+        dt.upload(w.dataf, w.stop_map, w.vertices, w.edges)
+        ref = dt.track(seeds)
+        f = dt.fork()
+        try:
+            got = f.track(seeds)
+            for k in ("sline_len", "points"):
+                assert np.array_equal(ref[k], got[k])
+            stop = np.ascontiguousarray(w.stop_map, np.float32)
+            assert f.L.cuslines_set_stop_map(f.h, stop.ctypes.data) == 3  # CUSLINES_ESTATE
+            assert dt.L.cuslines_set_stop_map(dt.h, stop.ctypes.data) == 3
+            assert dt.L.cuslines_tracker_destroy(dt.h) == 3
+            assert "forks" in _lib.last_error()
+        finally:
+            f.close()
+        assert np.array_equal(dt.track(seeds)["points"], ref["points"])
+    finally:
+        dt.close()

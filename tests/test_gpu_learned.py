@@ -1,8 +1,9 @@
 """Learned PTT proposal on the GPU: bit-exact against the oracle's restatement
-for fixed weights, the training gradient against the float64 objective, a
-short training run on pairs the plain getter records, what the trained
-network gains over the untrained one, and the tracking statistics of the
-learned sampler against the plain getter on the same seeds."""
+for fixed weights (points and proposal counters, envelope misses included),
+the training gradient against the float64 objective, a short training run
+on pairs the plain getter records, what the trained network gains over an
+untrained one, and the tracking statistics of the exactly corrected learned
+sampler against the plain getter on the same seeds."""
 
 import numpy as np
 import pytest
@@ -33,13 +34,13 @@ def test_learned_bit_exact_vs_oracle(small_workloads):
     try:
         dt.set_proposal(wts)
         g = dt.track(seeds, keep_dirs=True)
-        gstats = dt.proposal_stats()
+        gstats = dt.proposal_counters()
     finally:
         dt.close()
     o = Oracle(w.params, w.dataf, w.stop_map, w.vertices, w.edges, ptt_lut=w.ptt_lut)
     o.set_learned(wts)
     r = o.track(seeds)
-    ostats = o.proposal_stats()
+    ostats = o.proposal_counters()
     assert gstats.sum() > 0 and gstats[0] > 0
     assert np.array_equal(gstats, ostats), (gstats, ostats)
     assert np.array_equal(g["sline_len"], r["sline_len"])
@@ -112,32 +113,36 @@ def test_recorded_pairs_are_well_formed(small_workloads, trained):
     np.testing.assert_allclose(np.linalg.norm(f, axis=2), 1.0, atol=1e-4)
     T = P[:, 40:64]
     assert T.min() >= 0.0 and T.max() <= 9.0 + 1e-4
-def test_training_lowers_loss_and_raises_acceptance(small_workloads, trained):
-    """What the trained network gains over the method without it: the
-    squared error of the predicted supports falls by more than 10x, and the
-    learned rounds that find a winner go from a minority (the untrained
-    network's streamlines mostly leave the proposal for the plain getter) to
-    nearly all sub-steps."""
+def test_training_lowers_loss_and_envelope_misses(small_workloads, trained):
+    """What the trained network gains over the method without it (an
+    untrained network): the squared error of the predicted supports falls by
+    more than 10x, and the face draws whose exact value breaks the envelope
+    of the exact correction (F > M g, the only draws where the learned
+    sampler can depart from the reference's face law) fall to a small
+    fraction of the untrained network's.  Both keep nearly every sub-step in
+    the learned rounds: the correction accepts any proposal."""
     from gpustreamlines_amd import learned as LP
     from gpustreamlines_amd import workloads as W
 
     assert trained["hist"][-1] < 0.1 * trained["loss0"]
     w = small_workloads["ptt"]
     seeds = W.seeds_for(w, 1500, placement="mixed", seed=9)
-    share = {}
+    miss, share = {}, {}
     dt = _tracker(w)
     try:
         for name, p in (("untrained", LP.init_params()), ("trained", trained["params"])):
             dt.set_proposal(LP.to_bf16(p))
             dt.track(seeds)
-            s = dt.proposal_stats()
-            share[name] = s[0] / max(1, s.sum())
-            print(f"[learned] {name}: accepted {s[0]} fell back {s[1]} plain {s[2]} share {share[name]:.3f}",
-                  flush=True)
+            s = dt.proposal_counters()
+            share[name] = s[0] / max(1, s[:3].sum())
+            miss[name] = s[3] / max(1, s[0] + s[1])
+            print(f"[learned] {name}: accepted {s[0]} fell back {s[1]} plain {s[2]} envelope misses {s[3]} "
+                  f"({miss[name]:.4f} per learned sub-step)", flush=True)
     finally:
         dt.close()
     assert share["trained"] > 0.95
-    assert share["untrained"] < 0.5
+    assert miss["trained"] < 0.05
+    assert miss["trained"] < 0.2 * miss["untrained"]
 
 
 def _density(r, shape):
@@ -190,8 +195,6 @@ def test_learned_count_and_density_match_plain(small_workloads, stats_runs):
     assert r_learned > 0.98
 
 
-@pytest.mark.xfail(strict=False, reason="the learned proposal shortens streamlines (1M seeds: mean 108 vs 120 "
-                   "points, KS statistic 0.044 against 0.0007 between two plain seeds); bar: KS p > 0.05")
 def test_learned_length_distribution_matches_plain(stats_runs):
     """Two-sample KS of the streamline lengths, plain vs learned, against
     the bar of p > 0.05 (plain vs plain under another seed printed beside)."""
