@@ -455,7 +455,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": (f"phase2_boot_batch_kernel<{MODEL_IDS[args.getter]}, {args.boot_occ or 2}>"
+            "kernel": (f"phase2_boot_batch_kernel<{MODEL_IDS[args.getter]}, {args.boot_occ or 4}>"
                        if boot and args.boot_mode == "mfma" else f"phase2_queue_kernel<{MODEL_IDS[args.getter]}, 0>"),
             "alg_bytes_per_step": alg_launch / max(steps_per_pass, 1),
             "alg_bytes_per_launch": alg_launch,

@@ -169,7 +169,7 @@ class DeviceTracker:
         """Bootstrap phase-2 kernel: BOOT_EXACT (bit-identical to the CPU
         oracle) or BOOT_MFMA (16 seeds in lockstep, matrix products on the
         matrix cores; equal within float rounding).  waves_per_simd (2 or 4,
-        0 = default) picks the MFMA kernel's occupancy variant."""
+        0 = the default 4) picks the MFMA kernel's occupancy variant."""
         _lib.check(self.L.cuslines_set_boot_mode(self.h, int(mode) | (int(waves_per_simd) << 8)), "set_boot_mode")
 
     def boot_counters(self, reset: bool = True) -> np.ndarray:

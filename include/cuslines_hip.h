@@ -210,7 +210,7 @@ enum {
  * layout are the same.  MFMA mode needs hr <= 320 DWI volumes, delta_nr <= 128
  * and nverts <= 768 (phase 2 returns CUSLINES_EINVAL otherwise) and the
  * queue schedule (CUSLINES_SCHED_STATIC keeps the exact kernel).  Bits 8..15
- * of `mode` choose the MFMA kernel's occupancy, 2 (default) or 4 waves per
+ * of `mode` choose the MFMA kernel's occupancy, 2 or 4 (default) waves per
  * SIMD (the autotuner's knob). */
 int cuslines_set_boot_mode(cuslines_tracker *t, int mode);
 
