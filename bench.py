@@ -64,8 +64,8 @@ LEARNED_PASSES = 3
 
 def measured_traffic(workload: str, kernel: str):
     """HBM bytes per launch of `kernel` on `workload` from the committed
-    rocprofv3 FETCH_SIZE/WRITE_SIZE summaries (profiles/*_traffic.json,
-    copy-calibrated), or None when no summary covers this workload."""
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summaries (profiles/*_traffic.json), or
+    None when no summary covers this workload."""
     import glob
 
     for f in sorted(glob.glob(os.path.join(HERE, "profiles", "*_traffic.json"))):
@@ -477,7 +477,8 @@ def main():
             res["config"]["proposal"] = "learned"
     traffic = measured_traffic(res["config"]["workload"], res["roofline"]["kernel"])
     if traffic is not None:
-        res["roofline"]["traffic"] = traffic["traffic_bytes"]
+        # the counters as read (FETCH_SIZE + WRITE_SIZE, no calibration factor)
+        res["roofline"]["traffic"] = traffic.get("traffic_bytes_unscaled", traffic["traffic_bytes"])
         res["roofline"]["traffic_source"] = traffic["source"]
     for b in (sd, cd, dd, od, ssd, sld, sl):
         if b is not None:
