@@ -171,6 +171,7 @@ class DeviceTracker:
         matrix cores; equal within float rounding).  waves_per_simd (2 or 4,
         0 = the default 4) picks the MFMA kernel's occupancy variant."""
         _lib.check(self.L.cuslines_set_boot_mode(self.h, int(mode) | (int(waves_per_simd) << 8)), "set_boot_mode")
+        self.boot_mode = int(mode)
 
     def boot_counters(self, reset: bool = True) -> np.ndarray:
         """MFMA bootstrap kernel: [slot attempts, workgroup rounds] since the

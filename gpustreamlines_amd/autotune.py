@@ -6,8 +6,14 @@ measures: it runs phase 2 of a seed sample under each candidate shape on the
 tracker's stream, times it with HIP events, and applies the fastest.
 
 Tuned knobs: schedule (persistent queue or static), threads per block (seed
-slots per workgroup), resident blocks per CU and, for PTT, lanes per seed
-(seeds per wavefront).
+slots per workgroup), resident blocks per CU (the occupancy maximum or
+fewer workgroups, each keeping its LDS slots) and, for PTT, lanes per seed
+(seeds per wavefront).  For the batched MFMA bootstrap kernel, whose
+workgroup is 16 seed slots of LDS and one product tile column each, the
+knobs are its register/LDS tiling variant (2 waves per SIMD with 256
+registers, or 4 with 128 and two workgroups per CU) and the workgroups per
+CU; the bootstrap mode itself (exact or MFMA) is the caller's choice and is
+kept.
 """
 
 from __future__ import annotations
@@ -46,21 +52,29 @@ class _Events:
         return float(t.value)
 
 
-def candidates(model_type: int):
+def candidates(model_type: int, boot_mode: int = 0):
     if model_type == 3:  # PTT
         for lanes, threads in itertools.product((0, 16, 8), (256, 128)):
             yield dict(schedule=0, threads=threads, blocks_per_cu=0, lanes=lanes)
+        yield dict(schedule=0, threads=256, blocks_per_cu=2, lanes=0)
         yield dict(schedule=1, threads=256, blocks_per_cu=0, lanes=0)
+    elif model_type in (0, 1) and boot_mode == 1:  # batched MFMA bootstrap
+        for occ, bpc in ((4, 0), (4, 1), (2, 0)):
+            yield dict(schedule=0, threads=0, blocks_per_cu=bpc, lanes=0, waves_per_simd=occ)
     else:
         threads = (128, 64) if model_type in (0, 1) else (256, 128, 64)
         for t in threads:
             yield dict(schedule=0, threads=t, blocks_per_cu=0, lanes=0)
+        yield dict(schedule=0, threads=threads[0], blocks_per_cu=2, lanes=0)
         yield dict(schedule=1, threads=threads[0], blocks_per_cu=0, lanes=0)
 
 
 def apply(dt, cfg):
     dt.set_launch(cfg["schedule"], cfg["threads"], cfg["blocks_per_cu"])
-    dt.set_lanes_per_seed(cfg["lanes"])
+    if dt.params["model_type"] == 3:
+        dt.set_lanes_per_seed(cfg["lanes"])
+    if "waves_per_simd" in cfg:
+        dt.set_boot_mode(dt.BOOT_MFMA, cfg["waves_per_simd"])
 
 
 def tune(dt, seeds: np.ndarray, rng_chunk: int = 0, verbose: bool = False):
@@ -88,7 +102,7 @@ def tune(dt, seeds: np.ndarray, rng_chunk: int = 0, verbose: bool = False):
         stream = C.c_void_p(dt.stream())
         times = {}
         try:
-            for cfg in candidates(dt.params["model_type"]):
+            for cfg in candidates(dt.params["model_type"], getattr(dt, "boot_mode", 0)):
                 apply(dt, cfg)
                 ev.hip.hipEventRecord(a, stream)
                 dt.phase2_async(sd.ptr, n, rng_chunk, od.ptr, dd.ptr, ssd.ptr, sld.ptr, sl.ptr)
