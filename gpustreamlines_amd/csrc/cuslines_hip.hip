@@ -45,6 +45,8 @@ constexpr int DISC_VERT_CNT = 24;                      // disc.h:20
 constexpr int DISC_FACE_CNT = 31;                      // disc.h:21
 constexpr int SORT_LAYERS = 15;
 constexpr int QUEUES = 8;  // persistent-queue counters, one per XCD
+constexpr int VPROP_K = 24 * 9;                        // PTT table: curvatures after the 24 propagators
+constexpr int VPROP_FLOATS = VPROP_K + 2 * 24;
 
 // PTT disc tables (quality 2): numeric data of cuslines/cuda_c/disc.h:20-84
 // (nibrary, BSD-3-Clause, Copyright (c) 2024 Dogu Baran Aydogan).
@@ -126,6 +128,7 @@ struct DevBufs {
     const uint32_t *lut;  // PTT LUT entry: vertex | (direction rank << 16)
     const float *blocks;  // PTT corner blocks (nullptr: gather 8 texels)
     const float *fprop;    // learned proposal: per disc vertex, propagator of 4 probe steps [24][9]
+    const float *vprop;    // PTT: per disc vertex, propagator of one probe step [24][9], then (k1, k2) [24][2]
     const uint16_t *mlp;   // proposal network, bf16, lane-interleaved (IL_NPARAM)
     float *rec;            // training pairs [rec_cap][PAIR_FLOATS]
     int *rec_count;
@@ -274,6 +277,37 @@ __device__ int sort_dedup_peaks(const DevParams &P, const float *odf, uint16_t *
         WAVE_SYNC();
     }
     if (n == 0) return 0;
+    if (n <= 32) {
+        // remove_similar_vertices (tracking_helpers.cu:195-223) on 32 lanes:
+        // lane i holds candidate i's direction and marks which earlier
+        // candidates j lie within the minimum separation (the same dot
+        // product and test as the greedy loop below); the greedy pass then
+        // keeps candidate i iff none of its marked candidates was kept, and
+        // the kept indices are compacted in order.  Same kept set as the
+        // single-lane loop, without its chain of dependent vertex loads.
+        int ci = 0;
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        if (l < n) {
+            ci = cand[l];
+            ax = sv[3 * ci]; ay = sv[3 * ci + 1]; az = sv[3 * ci + 2];
+        }
+        uint32_t conf = 0u;
+        for (int j = 0; j < n - 1; ++j) {
+            const float bx = shfl32(ax, j), by = shfl32(ay, j), bz = shfl32(az, j);
+            float c = dot3(ax, ay, az, bx, by, bz);
+            if (P.sphere_symm) c = fabsf(c);
+            if (j < l && c > P.cos_min_sep) conf |= 1u << j;
+        }
+        uint32_t keep = 1u;
+        for (int i = 1; i < n; ++i) {
+            const uint32_t ci_conf = (uint32_t)shfl32((int)conf, i);
+            if ((ci_conf & keep) == 0u) keep |= 1u << i;
+        }
+        WAVE_SYNC();
+        if (l < n && ((keep >> l) & 1u)) cand[__popc(keep & ((1u << l) - 1u))] = (uint16_t)ci;
+        WAVE_SYNC();
+        return __popc(keep);
+    }
     int kept = 1;
     if (l == 0) {
         // remove_similar_vertices, single lane (tracking_helpers.cu:195-223),
@@ -877,9 +911,18 @@ __device__ __forceinline__ void probing_frame(const float *frame, Philox &rng, f
 // (instrumented kernels only; nullptr otherwise) counts the taps read.
 __device__ __forceinline__ float data_support(const DevParams &P, const DevBufs &B, float support,
                                               float px, float py, float pz, float k1, float k2, float *pf,
-                                              float stop_at, uint32_t *ntap = nullptr) {
+                                              float stop_at, uint32_t *ntap = nullptr,
+                                              const float *prop_pre = nullptr) {
+    // already at the threshold: no tap can change the outcome, and the
+    // probe frames are dropped by every caller that passes a finite stop_at
+    if (support >= stop_at) return support;
     float prop[9], direc[3];
-    prepare_propagator(k1, k2, P.probe_step, prop);
+    if (prop_pre) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) prop[i] = prop_pre[i];
+    } else {
+        prepare_propagator(k1, k2, P.probe_step, prop);
+    }
     propagate_frame(prop, pf, direc);
     px += direc[0]; py += direc[1]; pz += direc[2];
     uint32_t idx = B.lut[lut_index(P, pf[0], pf[1], pf[2])];
@@ -955,8 +998,13 @@ __device__ int ptt_substep(const DevParams &P, const DevBufs &B, PttLds &S, floa
         if (ph == 0) {
             active = l < DISC_VERT_CNT;
             if (active) {
-                k1 = (float)((double)c_disc_vert[2 * l] * mc);
-                k2 = (float)((double)c_disc_vert[2 * l + 1] * mc);
+                if (B.vprop) {
+                    k1 = B.vprop[VPROP_K + 2 * l];
+                    k2 = B.vprop[VPROP_K + 2 * l + 1];
+                } else {
+                    k1 = (float)((double)c_disc_vert[2 * l] * mc);
+                    k2 = (float)((double)c_disc_vert[2 * l + 1] * mc);
+                }
             }
         } else {
             float r1 = rng.uniform(), r2 = rng.uniform();
@@ -972,12 +1020,19 @@ __device__ int ptt_substep(const DevParams &P, const DevBufs &B, PttLds &S, floa
                 if (S.face_cdf[mid] >= tmp) hi = mid; else jj = mid + 1;
             }
             const int f0 = c_disc_face[jj * 3], f1 = c_disc_face[jj * 3 + 1], f2 = c_disc_face[jj * 3 + 2];
-            const float vx0 = (float)(mc * (double)c_disc_vert[f0 * 2]);
-            const float vx1 = (float)(mc * (double)c_disc_vert[f1 * 2]);
-            const float vx2 = (float)(mc * (double)c_disc_vert[f2 * 2]);
-            const float vy0 = (float)(mc * (double)c_disc_vert[f0 * 2 + 1]);
-            const float vy1 = (float)(mc * (double)c_disc_vert[f1 * 2 + 1]);
-            const float vy2 = (float)(mc * (double)c_disc_vert[f2 * 2 + 1]);
+            float vx0, vx1, vx2, vy0, vy1, vy2;
+            if (B.vprop) {  // the same products, computed once per tracker
+                const float *vk = B.vprop + VPROP_K;
+                vx0 = vk[2 * f0]; vx1 = vk[2 * f1]; vx2 = vk[2 * f2];
+                vy0 = vk[2 * f0 + 1]; vy1 = vk[2 * f1 + 1]; vy2 = vk[2 * f2 + 1];
+            } else {
+                vx0 = (float)(mc * (double)c_disc_vert[f0 * 2]);
+                vx1 = (float)(mc * (double)c_disc_vert[f1 * 2]);
+                vx2 = (float)(mc * (double)c_disc_vert[f2 * 2]);
+                vy0 = (float)(mc * (double)c_disc_vert[f0 * 2 + 1]);
+                vy1 = (float)(mc * (double)c_disc_vert[f1 * 2 + 1]);
+                vy2 = (float)(mc * (double)c_disc_vert[f2 * 2 + 1]);
+            }
             k1 = __builtin_fmaf(r2, vx2 - vx0, __builtin_fmaf(r1, vx1 - vx0, vx0));
             k2 = __builtin_fmaf(r2, vy2 - vy0, __builtin_fmaf(r1, vy1 - vy0, vy0));
         }
@@ -987,7 +1042,11 @@ __device__ int ptt_substep(const DevParams &P, const DevBufs &B, PttLds &S, floa
             probing_frame<INIT>(fr, rng, pf);
             // rejection rounds of a non-initial sub-step only test sup >= min_support_f
             const float stop_at = (ph >= 1 && !INIT && P.ptt_nonneg) ? P.min_support_f : __builtin_inff();
-            sup = data_support(P, B, first_val, px, py, pz, k1, k2, pf, stop_at, CNT ? ntap : nullptr);
+            // the vertex phase's curves are the disc vertices: their one-step
+            // propagators are the same at every sub-step (computed once by
+            // vertex_propagators with the same prepare_propagator)
+            sup = data_support(P, B, first_val, px, py, pz, k1, k2, pf, stop_at, CNT ? ntap : nullptr,
+                               (ph == 0 && B.vprop) ? B.vprop + 9 * l : nullptr);
         }
         if (ph == 0) {
             const bool ok = active && !(sup < P.min_support_f);
@@ -1319,7 +1378,8 @@ __device__ int get_direction_ptt_learned(const DevParams &P, const DevBufs &B, P
                 for (int i = 0; i < 9; ++i) pf[i] = fr[i];
                 const float k1v = (float)((double)c_disc_vert[2 * l] * mc);
                 const float k2v = (float)((double)c_disc_vert[2 * l + 1] * mc);
-                E[l] = data_support(P, B, first_val, px, py, pz, k1v, k2v, pf, __builtin_inff());
+                E[l] = data_support(P, B, first_val, px, py, pz, k1v, k2v, pf, __builtin_inff(), nullptr,
+                                    B.vprop ? B.vprop + 9 * l : nullptr);
             }
             known |= need;
             WAVE_SYNC();
@@ -2258,10 +2318,12 @@ __host__ __device__ inline int bb_slot_stride(int nv32, int t32, int hr32) {
 // A[l % 16][k] and X_{l % 16}[k] of each step and holds Y rows 4 (l / 16) + r;
 // within a 16-wide chunk of K, lane group q = l / 16 reads the contiguous k =
 // 16 c + 4 q + (0..3) as one 16-byte load and MFMA step s takes component s.
+// wmax (optional): this wave's maximum of each slot's result rows < ymax,
+// written to wmax[wave * 16 + slot].
 template <int MAXT, bool CSA_ROW0 = false>
 __device__ __forceinline__ void bb_gemm(const float *__restrict__ A, int lda, int K, int ntiles, const float *X,
                                         const float *X2, int ksplit, float *Y, float *Y2, int ysplit, int ymax,
-                                        int ss) {
+                                        int ss, float *wmax = nullptr) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int n = lane & 15, q = lane >> 4;
     bb_v4 acc[MAXT];
@@ -2288,6 +2350,23 @@ __device__ __forceinline__ void bb_gemm(const float *__restrict__ A, int lda, in
             }
         }
     }
+    if (wmax) {
+        float pm = REAL_MIN;
+#pragma unroll
+        for (int i = 0; i < MAXT; ++i) {
+            const int t = w + BB_WAVES * i;
+            const int row = 16 * t + 4 * q;
+            if (t < ntiles) {
+                if (row < ymax) pm = fmaxf(pm, acc[i].x);
+                if (row + 1 < ymax) pm = fmaxf(pm, acc[i].y);
+                if (row + 2 < ymax) pm = fmaxf(pm, acc[i].z);
+                if (row + 3 < ymax) pm = fmaxf(pm, acc[i].w);
+            }
+        }
+        pm = fmaxf(pm, __shfl_xor(pm, 16));
+        pm = fmaxf(pm, __shfl_xor(pm, 32));
+        if (q == 0) wmax[w * 16 + n] = pm;
+    }
 #pragma unroll
     for (int i = 0; i < MAXT; ++i) {
         const int t = w + BB_WAVES * i;
@@ -2305,6 +2384,71 @@ __device__ __forceinline__ void bb_gemm(const float *__restrict__ A, int lda, in
             }
         }
     }
+}
+
+// The models' transforms of the signal (gemv_xf) with the hardware base-2
+// logarithm: the batched kernel's results already differ from the
+// reference's in the last bits, and two series logarithms per channel (CSA)
+// are a large share of its per-slot work.
+template <int XF>
+__device__ __forceinline__ float bb_xf(float x) {
+    constexpr float LN2 = 0.693147180559945309f;
+    if (XF == 1) {
+        const float lx = __builtin_amdgcn_logf(x) * LN2;
+        return ((-lx) * (1.5f + lx)) * x;
+    }
+    const float c = fminf(fmaxf(x, 0.001f), 0.999f);
+    return __builtin_amdgcn_logf(-(__builtin_amdgcn_logf(c) * LN2)) * LN2;
+}
+
+// peak_directions_d on the ODF thresholded at `thr` (values below it read as
+// 0, tracking_helpers.cu:75-245 after generate_streamlines_cuda.cu's
+// threshold) without writing the thresholded array: the adjacency form of
+// peak_directions with the threshold applied to every value it reads.  A
+// peak is never below thr (its thresholded value would be 0 with no
+// strictly smaller neighbour), so the kept candidates' values, the sort
+// keys, are the raw ODF values.  Needs the adjacency table (B.nbr).
+__device__ int bb_peaks(const DevParams &P, const DevBufs &B, const float *odf, float thr, uint16_t *cand,
+                        const float *sv) {
+    const int l = lane32();
+    const int nv = P.nverts;
+    uint32_t pk = 0u;
+    float mn = REAL_MAX, m = REAL_MIN;
+    for (int r = 0, j = l; j < nv; ++r, j += 32) {
+        const uint4 q = reinterpret_cast<const uint4 *>(B.nbr)[j];
+        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+        float x = odf[j];
+        x = (x < thr) ? 0.0f : x;
+        mn = (mn < x) ? mn : x;
+        bool smaller = false, larger = false;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            float y = odf[(w[k >> 1] >> (16 * (k & 1))) & 0xffffu];
+            y = (y < thr) ? 0.0f : y;
+            smaller = smaller || (y < x);
+            larger = larger || (x < y);
+        }
+        if (smaller && !larger) pk |= 1u << r;
+    }
+    mn = min32(mn);
+    mn = (0.0f > mn) ? 0.0f : mn;
+    for (int r = 0, j = l; j < nv; ++r, j += 32)
+        if ((pk >> r) & 1u) {
+            const float xm = odf[j] + (-mn);
+            m = (m > xm) ? m : xm;
+        }
+    m = max32(m);
+    const float comp = (float)((double)P.rel_peak * (double)m);
+    int n = 0;
+    for (int r = 0, j0 = 0; j0 < nv; ++r, j0 += 32) {
+        const int j = j0 + l;
+        const bool keep = (j < nv) && ((pk >> r) & 1u) && ((odf[j] - mn) >= comp);
+        const uint32_t msk = half_ballot(keep);
+        if (keep) cand[n + __popc(msk & ((1u << l) - 1u))] = (uint16_t)j;
+        n += __popc(msk);
+    }
+    WAVE_SYNC();
+    return sort_dedup_peaks(P, odf, cand, sv, n);
 }
 
 // Per-slot state of the batched kernel (registers of the slot's 32 lanes).
@@ -2431,6 +2575,7 @@ __global__ void __launch_bounds__(BB_THREADS, OCC)
               off_r = (int)(S.r - smem) - slot * ss;
     const int hr = P.hr, T = P.dimt, nv = P.nverts, hr16 = P.hr16;
     const int MAXL = P.max_sline_len;
+    float *wmax = smem + BB_SLOTS * ss;  // [BB_WAVES][16] partial ODF maxima of phase E
     BbSlot st;
     st.s = -1; st.qk = 0; st.done = false; st.nb = 0; st.pass = 0;
     Philox rng;
@@ -2448,7 +2593,7 @@ __global__ void __launch_bounds__(BB_THREADS, OCC)
                 for (int m = l; m < hr16; m += 32) S.msk[m] = (m < hr) ? S.vox[B.dwi_idx[m]] : 0.0f;
             }
         }
-        const int nact = __syncthreads_count(active);
+        const int nact = __syncthreads_count(active) / 32;  // active slots (32 lanes each)
         if (nact == 0) break;
         attempts += (unsigned long long)nact;
         ++rounds;
@@ -2476,13 +2621,13 @@ __global__ void __launch_bounds__(BB_THREADS, OCC)
             }
             const float sum = tree_sum32(part);
             for (int m = 16; m; m >>= 1) cnt += shfl_xor32(cnt, m);
-            const float denom = sum / (float)cnt;
-            for (int j = l; j < T; j += 32) S.vox[j] = S.vox[j] / denom;
+            const float inv = (float)cnt / sum;
+            for (int j = l; j < T; j += 32) S.vox[j] = S.vox[j] * inv;
             WAVE_SYNC();
             for (int m = l; m < hr16; m += 32) {
                 const float v = (m < hr) ? S.vox[B.dwi_idx[m]] : 0.0f;
                 S.msk[m] = v;
-                S.h[m] = (m < hr) ? gemv_xf<(MODEL == CUSLINES_OPDT) ? 1 : 2>(v) : 0.0f;
+                S.h[m] = (m < hr) ? bb_xf<(MODEL == CUSLINES_OPDT) ? 1 : 2>(v) : 0.0f;
             }
         }
         __syncthreads();
@@ -2494,7 +2639,7 @@ __global__ void __launch_bounds__(BB_THREADS, OCC)
         // ---- phase E: the ODF on the sphere ---------------------------------
         bb_gemm<(768 / 16 + BB_WAVES - 1) / BB_WAVES>(B.odf_mat, P.dn16, P.dn16, P.nv16 / 16, smem + off_vox,
                                                       smem + off_vox, 1 << 30, smem + off_h, smem + off_h, 1 << 30,
-                                                      nv, ss);
+                                                      nv, ss, wmax);
         __syncthreads();
         // ---- phase F: peaks, closest peak, the step --------------------------
         if (active) {
@@ -2504,14 +2649,19 @@ __global__ void __launch_bounds__(BB_THREADS, OCC)
                 // fails the same way without a random draw
                 ns = st.i;
             } else {
-                float mx = REAL_MIN;
-                for (int j = l; j < nv; j += 32) mx = (mx > S.h[j]) ? mx : S.h[j];
+                // the ODF's maximum from the 8 waves' partial maxima of phase E
+                float mx = (l < BB_WAVES) ? wmax[l * 16 + slot] : REAL_MIN;
                 mx = max32(mx);
                 const float thr = (float)(PMF_THRESHOLD_P * (double)mx);
-                for (int j = l; j < nv; j += 32)
-                    if (S.h[j] < thr) S.h[j] = 0.0f;
-                WAVE_SYNC();
-                const int npk = peak_directions(P, B, S.h, S.marks, S.cand, B.verts);
+                int npk;
+                if (B.nbr) {
+                    npk = bb_peaks(P, B, S.h, thr, S.cand, B.verts);
+                } else {
+                    for (int j = l; j < nv; j += 32)
+                        if (S.h[j] < thr) S.h[j] = 0.0f;
+                    WAVE_SYNC();
+                    npk = peak_directions(P, B, S.h, S.marks, S.cand, B.verts);
+                }
                 float nx = 0.0f, ny = 0.0f, nz = 0.0f;
                 const bool ok = npk > 0 && closest_peak(P, B.verts, st.dx, st.dy, st.dz, npk, S.cand, &nx, &ny, &nz);
                 WAVE_SYNC();
@@ -2714,14 +2864,21 @@ __global__ void adam_kernel(float *__restrict__ p, const float *__restrict__ g, 
 
 // Propagators of arc length FEAT_STEPS probe steps for the 24 disc vertices
 // (the curves along which the learned proposal's inputs are tapped).
-__global__ void feature_propagators(DevParams P, float *__restrict__ out) {
+// Per disc vertex, the propagator of `steps` probe steps (FEAT_STEPS: the
+// learned proposal's features; 1: the vertex phase's probes, followed at
+// VPROP_K by the vertices' curvatures (k1, k2)).
+__global__ void feature_propagators(DevParams P, int steps, float *__restrict__ out) {
     const int l = threadIdx.x;
     if (l < DISC_VERT_CNT) {
         const float k1 = (float)((double)c_disc_vert[2 * l] * (double)P.max_curv);
         const float k2 = (float)((double)c_disc_vert[2 * l + 1] * (double)P.max_curv);
         float p[9];
-        prepare_propagator(k1, k2, (float)FEAT_STEPS * P.probe_step, p);
+        prepare_propagator(k1, k2, (steps == 1) ? P.probe_step : (float)steps * P.probe_step, p);
         for (int i = 0; i < 9; ++i) out[9 * l + i] = p[i];
+        if (steps == 1) {  // and the vertex's curvature, the disc scaled by MAX_CURVATURE
+            out[VPROP_K + 2 * l] = k1;
+            out[VPROP_K + 2 * l + 1] = k2;
+        }
     }
 }
 
@@ -2989,6 +3146,7 @@ struct cuslines_tracker {
     int num_cu = 1;
     int ptt_lanes = 0;      // PTT: lanes per seed (8, 16, 32; 0 = shared 32-lane kernel)
     float *d_fprop = nullptr;            // learned proposal: feature propagators [24][9]
+    float *d_vprop = nullptr;            // PTT vertex-phase propagators [24][9]
     uint16_t *d_mlp = nullptr;           // learned proposal (nullptr: plain getter)
     unsigned long long *d_stats = nullptr;
     int count_taps = 0;  // PTT phase 2 counts its taps and sub-steps into d_stats
@@ -3021,6 +3179,7 @@ static DevBufs bufs_of(const cuslines_tracker *t) {
     b.lut = t->d_lut;
     b.blocks = t->use_blocks ? t->d_blocks : nullptr;
     b.fprop = t->d_fprop;
+    b.vprop = t->d_vprop;
     b.mlp = t->d_mlp;
     b.rec = t->rec;
     b.rec_count = t->rec_count;
@@ -3240,7 +3399,8 @@ static int launch_boot_batch(cuslines_tracker *t, const float *seeds, int n, int
     const DevParams &d = t->dp;
     if (!boot_batch_fits(d))
         return fail(CUSLINES_EINVAL, "batched bootstrap needs hr <= 320, delta_nr <= 128 and nverts <= 768");
-    const size_t lds = sizeof(float) * (size_t)BB_SLOTS * (size_t)bb_slot_stride(d.nv32, d.t32, d.hr32);
+    const size_t lds = sizeof(float) * ((size_t)BB_SLOTS * (size_t)bb_slot_stride(d.nv32, d.t32, d.hr32) +
+                                        (size_t)BB_WAVES * 16);
     if (lds > 160 * 1024) return fail(CUSLINES_EINVAL, "batched bootstrap slots too large for LDS");
     const void *kfn = (t->boot_occ == 4) ? (const void *)phase2_boot_batch_kernel<MODEL, 4>
                                          : (const void *)phase2_boot_batch_kernel<MODEL, 2>;
@@ -3487,7 +3647,7 @@ int cuslines_tracker_destroy(cuslines_tracker *t) {
     dfree(t->d_H); dfree(t->d_R); dfree(t->d_db); dfree(t->d_dq); dfree(t->d_smat);
     dfree(t->d_hrm); dfree(t->d_fitm); dfree(t->d_odfm);
     dfree(t->d_b0); dfree(t->d_dwi); dfree(t->d_lut); dfree(t->d_work); dfree(t->d_blocks);
-    dfree(t->d_fprop); dfree(t->d_mlp); dfree(t->d_stats);
+    dfree(t->d_fprop); dfree(t->d_vprop); dfree(t->d_mlp); dfree(t->d_stats);
     if (t->stream) (void)hipStreamDestroy(t->stream);
     delete t;
     return CUSLINES_OK;
@@ -3794,6 +3954,12 @@ int cuslines_set_ptt_lut(cuslines_tracker *t, const int32_t *lut) {
     HIPC(hipMalloc(&t->d_lut, h.size() * sizeof(uint32_t)));
     HIPC(hipMemcpy(t->d_lut, h.data(), h.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     t->rank_vertex.swap(rank_vertex);
+    if (!t->d_vprop) {
+        HIPC(hipMalloc(&t->d_vprop, VPROP_FLOATS * sizeof(float)));
+        hipLaunchKernelGGL(feature_propagators, dim3(1), dim3(32), 0, t->stream, t->dp, 1, t->d_vprop);
+        HIPC(hipGetLastError());
+        HIPC(hipStreamSynchronize(t->stream));
+    }
     t->has_lut = true;
     if (t->has_data) return build_blocks(t);
     return CUSLINES_OK;
@@ -3949,7 +4115,7 @@ int cuslines_set_ptt_layout(cuslines_tracker *t, int corner_blocks) {
 static int ensure_feature_propagators(cuslines_tracker *t) {
     if (t->d_fprop) return CUSLINES_OK;
     HIPC(hipMalloc(&t->d_fprop, DISC_VERT_CNT * 9 * sizeof(float)));
-    hipLaunchKernelGGL(feature_propagators, dim3(1), dim3(32), 0, t->stream, t->dp, t->d_fprop);
+    hipLaunchKernelGGL(feature_propagators, dim3(1), dim3(32), 0, t->stream, t->dp, FEAT_STEPS, t->d_fprop);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(t->stream));
     return CUSLINES_OK;

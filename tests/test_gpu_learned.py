@@ -206,3 +206,52 @@ def test_learned_length_distribution_matches_plain(stats_runs):
     print(f"[learned] mean length {a['sline_len'].mean():.1f} vs {b['sline_len'].mean():.1f}; KS p={ks.pvalue:.3g} "
           f"(plain vs plain under another seed: p={ks0.pvalue:.3g})", flush=True)
     assert ks.pvalue > 0.05
+
+
+def test_learned_million_mixed_seeds():
+    """The learned proposal on the 1M-seed uneven-length workload (81x106x76,
+    half the seeds in FA > 0.2, half in 0.05-0.15 border voxels), trained on
+    pairs the plain getter records from 20,000 mask seeds: the same
+    streamlines per seed as the plain getter (phase 1 does not use the
+    proposal), lengths within the slot, finite points, a small share of
+    envelope misses, and length and density statistics of the plain
+    getter's (KS p > 0.01, r > 0.98)."""
+    from scipy.stats import ks_2samp
+
+    from gpustreamlines_amd import learned as LP
+    from gpustreamlines_amd import workloads as W
+
+    w = W.build("ptt")
+    seeds = W.seeds_for(w, 1_000_000, placement="mixed")
+    print("[learned] 1M mixed: recording + training", flush=True)
+    dt = _tracker(w)
+    try:
+        pairs, n = LP.record_pairs(dt, W.seeds_for(w, 20000, placement="mask", seed=7), capacity=3_000_000,
+                                   rate=0.1, rng_chunk=25000)
+        tr = LP.Trainer(dt, batch=4096)
+        tr.fit(pairs, n, epochs=2)
+        params = tr.params()
+        tr.free()
+        pairs.free()
+        print("[learned] 1M mixed: plain pass", flush=True)
+        plain = dt.track(seeds, rng_chunk=25000)
+        dt.set_proposal(LP.to_bf16(params))
+        dt.proposal_counters(reset=True)
+        print("[learned] 1M mixed: learned pass", flush=True)
+        got = dt.track(seeds, rng_chunk=25000)
+        cnt = dt.proposal_counters(reset=True)
+    finally:
+        dt.close()
+    lens, lp = got["sline_len"].astype(np.int64), plain["sline_len"].astype(np.int64)
+    assert np.array_equal(got["counts"], plain["counts"])
+    assert len(lens) == len(lp) and lens.min() >= 1 and lens.max() <= 2 * w.params["max_sline_len"] - 1
+    assert np.isfinite(got["points"]).all()
+    shape = tuple(w.params[k] for k in ("dimx", "dimy", "dimz"))
+    r = np.corrcoef(_density(plain, shape), _density(got, shape))[0, 1]
+    ks = ks_2samp(lp, lens)
+    miss = cnt[3] / max(1, cnt[0] + cnt[1])
+    print(f"[learned] 1M mixed: mean length {lp.mean():.2f} vs {lens.mean():.2f}, KS p {ks.pvalue:.3g}, "
+          f"density r {r:.5f}, envelope misses per learned sub-step {miss:.2e}", flush=True)
+    assert miss < 0.05
+    assert ks.pvalue > 0.01
+    assert r > 0.98
