@@ -228,7 +228,8 @@ def measure_end_to_end(args, w, seeds, kernel_steps):
     """The user-facing path: GPUTracker.generate_streamlines over chunks of
     CHUNK seeds (seed upload, phase 1, scan, count read, phase 2, compaction,
     point download, length filter and packing), timed after one warm-up
-    chunk.  Data preparation (PTT tiling and LUT) happens in the constructor
+    chunk; with chunks in flight the count read is gone (bounded chunks).
+    Data preparation (PTT tiling and LUT) happens in the constructor
     and is reported apart; no file is written."""
     from gpustreamlines_amd import (BootDirectionGetter, GPUTracker, ProbDirectionGetter, PttDirectionGetter)
 
@@ -244,7 +245,8 @@ def measure_end_to_end(args, w, seeds, kernel_steps):
     tr = GPUTracker(dg, data, w.stop_map, p["tc_threshold"], w.vertices, w.edges, max_angle=p["max_angle"],
                     step_size=p["step_size"], relative_peak_thresh=p["relative_peak_thresh"],
                     min_separation_angle=p["min_separation_angle"], chunk_size=CHUNK,
-                    launch_seeds=args.launch_seeds, chunk_pipeline=args.chunk_pipeline)
+                    launch_seeds=args.launch_seeds, chunk_pipeline=args.chunk_pipeline,
+                    chunk_graphs=bool(args.chunk_graphs))
     with tr:
         prep = time.perf_counter() - t0
         tr.generate_streamlines(seeds[:CHUNK])
@@ -257,8 +259,16 @@ def measure_end_to_end(args, w, seeds, kernel_steps):
           flush=True)
     return {"steps_per_s": steps / el, "s": el, "streamlines": int(len(lens)), "points": int(lens.sum()),
             "chunks": -(-len(seeds) // CHUNK), "launch_seeds": args.launch_seeds,
-            "chunk_pipeline": args.chunk_pipeline, "prep_s": prep,
+            "chunk_pipeline": args.chunk_pipeline, "chunk_graphs": bool(args.chunk_graphs), "prep_s": prep,
             "same_steps_as_kernel_pass": steps == kernel_steps}
+
+
+def kernel_name(args):
+    """The phase-2 kernel the plain pass launches (as rocprofv3 names it)."""
+    if args.getter in ("csa", "opdt") and args.boot_mode == "mfma":
+        return f"phase2_boot_batch_kernel<{MODEL_IDS[args.getter]}, {args.boot_occ or 4}>"
+    v = 4 if args.getter == "ptt" and args.ptt_stepping == "flat" else 0
+    return f"phase2_queue_kernel<{MODEL_IDS[args.getter]}, {v}>"
 
 
 def copy_calibration(nbytes):
@@ -299,8 +309,12 @@ def main():
                     help="also time GPUTracker.generate_streamlines over 25,000-seed chunks (host copies included)")
     ap.add_argument("--launch-seeds", type=int, default=250000,
                     help="end to end: seeds per launch (0: one launch per 25,000-seed chunk, the reference's protocol)")
-    ap.add_argument("--chunk-pipeline", type=int, default=4,
+    ap.add_argument("--chunk-pipeline", type=int, default=8,
                     help="end to end with one launch per chunk: chunks in flight at once (1: one after the other)")
+    ap.add_argument("--ptt-stepping", default="per-seed", choices=["flat", "per-seed"],
+                    help="PTT: one loop of sub-steps per seed slot (flat) or the reference's per-seed nesting")
+    ap.add_argument("--chunk-graphs", type=int, default=1, choices=[0, 1],
+                    help="end to end with chunks in flight: each chunk as one HIP graph launch (0: eager launches)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -329,6 +343,7 @@ def main():
     boot = args.getter in ("csa", "opdt")
     if boot:
         dt.set_boot_mode(dt.BOOT_MFMA if args.boot_mode == "mfma" else dt.BOOT_EXACT, args.boot_occ)
+    dt.set_ptt_stepping(args.ptt_stepping == "flat")
     sd = dt.alloc(seeds.nbytes)
     sd.upload(seeds)
     cd = dt.alloc(4 * (n + 1))
@@ -447,6 +462,7 @@ def main():
             "wall_s": wall,
             "proposal": "plain",
             "boot_mode": args.boot_mode if boot else None,
+            "ptt_stepping": args.ptt_stepping if args.getter == "ptt" else None,
         },
         "roofline": {
             "bound": "hbm",
@@ -455,8 +471,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": (f"phase2_boot_batch_kernel<{MODEL_IDS[args.getter]}, {args.boot_occ or 4}>"
-                       if boot and args.boot_mode == "mfma" else f"phase2_queue_kernel<{MODEL_IDS[args.getter]}, 0>"),
+            "kernel": kernel_name(args),
             "alg_bytes_per_step": alg_launch / max(steps_per_pass, 1),
             "alg_bytes_per_launch": alg_launch,
         },

@@ -132,6 +132,11 @@ class DeviceTracker:
         """PTT phase 2: lanes per seed (8, 16, 32; 0 = shared 32-lane kernel)."""
         _lib.check(self.L.cuslines_set_lanes_per_seed(self.h, int(lanes)), "set_lanes_per_seed")
 
+    def set_ptt_stepping(self, flat: bool = False):
+        """Plain PTT phase 2 on the queue: one loop of sub-steps per seed
+        slot (True) or the per-seed nesting of the reference (False, default)."""
+        _lib.check(self.L.cuslines_set_ptt_stepping(self.h, int(bool(flat))), "set_ptt_stepping")
+
     def set_ptt_layout(self, corner_blocks: bool = True):
         """PTT taps from 32-byte corner blocks (True) or 8 texel gathers."""
         _lib.check(self.L.cuslines_set_ptt_layout(self.h, int(bool(corner_blocks))), "set_ptt_layout")
@@ -286,3 +291,109 @@ class DeviceTracker:
             for b in bufs:
                 b.free()
         return out
+
+    def track_chunk(self, seeds, rng_chunk: int = 0, slot_cap: int | None = None, graph: bool = False,
+                    launches: int = 1) -> dict:
+        """``track`` as one bounded chunk (cuslines_chunk_async, or its HIP
+        graph with ``graph``; ``launches`` > 1 replays the graph that many
+        times, re-uploading the seeds before each).  ``slot_cap`` defaults to
+        8 slots per seed.  Returns ``totals`` (streamlines of the chunk,
+        slots tracked, points) and the tracked prefix: ``offsets`` [n+1],
+        ``sline_seed``/``sline_len`` [tracked], ``points``, ``point_offsets``.
+        """
+        seeds = np.ascontiguousarray(seeds, dtype=np.float32).reshape(-1, 3)
+        n = len(seeds)
+        cb = ChunkBuffers(self, n, slot_cap if slot_cap is not None else max(8 * n, 1))
+        try:
+            for _ in range(max(1, int(launches))):
+                cb.sd.upload(seeds)
+                cb.enqueue(n, rng_chunk, graph=graph)
+                res = cb.collect(n)
+        finally:
+            cb.close()
+        return res
+
+
+class ChunkBuffers:
+    """Device buffers of bounded chunks (cuslines_chunk_async) of at most
+    ``nmax`` seeds into ``cap`` slots on one handle, and the HIP graph of a
+    chunk of exactly ``nmax`` seeds (captured at its first use)."""
+
+    def __init__(self, dt: DeviceTracker, nmax: int, cap: int):
+        self.dt, self.nmax, self.cap = dt, int(nmax), int(cap)
+        nv = dt.params["nverts"]
+        L = int(dt.params.get("max_sline_len", 501))
+        ws = C.c_size_t()
+        _lib.check(dt.L.cuslines_chunk_workspace_bytes(dt.h, self.nmax, self.cap, C.byref(ws)), "chunk_workspace")
+        self.bufs = []
+        try:
+            self.sd = self._alloc(12 * self.nmax)
+            self.cd = self._alloc(4 * (self.nmax + 1))
+            self.dd = self._alloc(4 * self.nmax * nv)
+            self.od = self._alloc(4 * (self.nmax + 1))
+            self.ssd = self._alloc(4 * self.cap)
+            self.sld = self._alloc(4 * self.cap)
+            self.sl = self._alloc(4 * 3 * 2 * L * self.cap)
+            self.pod = self._alloc(8 * (self.cap + 1))
+            self.ptd = self._alloc(4 * 3 * (2 * L - 1) * self.cap)
+            self.tot = self._alloc(8 * 3)
+            self.ws = self._alloc(ws.value)
+        except Exception:
+            self.close()
+            raise
+        self.graph = None
+        self.graph_chunk = None
+
+    def _alloc(self, nbytes):
+        b = self.dt.alloc(nbytes)
+        self.bufs.append(b)
+        return b
+
+    def _args(self, n, rng_chunk):
+        P = C.c_void_p
+        return (self.dt.h, P(self.sd.ptr), int(n), int(rng_chunk), P(self.cd.ptr), P(self.dd.ptr), P(self.od.ptr),
+                self.cap, P(self.ssd.ptr), P(self.sld.ptr), P(self.sl.ptr), P(self.pod.ptr), P(self.ptd.ptr),
+                P(self.tot.ptr), P(self.ws.ptr), self.ws.nbytes)
+
+    def enqueue(self, n: int, rng_chunk: int, graph: bool = False):
+        """Enqueue the chunk of the first ``n`` seeds in ``sd`` (no host
+        read); with ``graph`` and ``n == nmax``, as a graph launch."""
+        if n > self.nmax:
+            raise ValueError("chunk larger than its buffers")
+        if graph and n == self.nmax:
+            if self.graph is None or self.graph_chunk != rng_chunk:
+                self.drop_graph()
+                g = C.c_void_p()
+                _lib.check(self.dt.L.cuslines_chunk_graph_create(*self._args(n, rng_chunk), C.byref(g)),
+                           "chunk_graph_create")
+                self.graph, self.graph_chunk = g.value, rng_chunk
+            _lib.check(self.dt.L.cuslines_graph_launch(C.c_void_p(self.graph)), "graph_launch")
+        else:
+            _lib.check(self.dt.L.cuslines_chunk_async(*self._args(n, rng_chunk)), "chunk")
+
+    def totals(self):
+        """(streamlines of the chunk, slots tracked, points), after the chunk."""
+        t = self.tot.download(np.zeros(3, dtype=np.int64))
+        return int(t[0]), int(t[1]), int(t[2])
+
+    def collect(self, n: int, with_seeds: bool = True) -> dict:
+        nsl, tracked, npts = self.totals()
+        out = {"totals": (nsl, tracked, npts)}
+        out["offsets"] = self.od.download(np.zeros(n + 1, dtype=np.int32))
+        out["sline_len"] = self.sld.download(np.zeros(tracked, dtype=np.int32))
+        if with_seeds:
+            out["sline_seed"] = self.ssd.download(np.zeros(tracked, dtype=np.int32))
+        out["point_offsets"] = self.pod.download(np.zeros(tracked + 1, dtype=np.int64))
+        out["points"] = self.ptd.download(np.zeros((npts, 3), dtype=np.float32))
+        return out
+
+    def drop_graph(self):
+        if self.graph is not None:
+            _lib.check(self.dt.L.cuslines_graph_destroy(C.c_void_p(self.graph)), "graph_destroy", hard=False)
+            self.graph = None
+
+    def close(self):
+        self.drop_graph()
+        for b in self.bufs:
+            b.free()
+        self.bufs = []

@@ -101,6 +101,7 @@ def load():
         "cuslines_set_launch": [vp, C.c_int, C.c_int, C.c_int],
         "cuslines_set_ptt_layout": [vp, C.c_int],
         "cuslines_set_lanes_per_seed": [vp, C.c_int],
+        "cuslines_set_ptt_stepping": [vp, C.c_int],
         "cuslines_set_boot_mode": [vp, C.c_int],
         "cuslines_boot_counters": [vp, C.POINTER(C.c_int64), C.c_int],
         "cuslines_set_proposal": [vp, vp],
@@ -117,6 +118,12 @@ def load():
         "cuslines_phase2_async": [vp, vp, i32, i32, vp, vp, vp, vp, vp],
         "cuslines_phase2_noalloc_async": [vp, vp, i32, i32, vp, vp, vp, vp, vp],
         "cuslines_compact_async": [vp, vp, i32, vp, vp, vp],
+        "cuslines_chunk_workspace_bytes": [vp, i32, i32, C.POINTER(sz)],
+        "cuslines_chunk_async": [vp, vp, i32, i32, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, sz],
+        "cuslines_chunk_graph_create": [vp, vp, i32, i32, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, sz,
+                                        C.POINTER(vp)],
+        "cuslines_graph_launch": [vp],
+        "cuslines_graph_destroy": [vp],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

@@ -113,6 +113,7 @@ struct DevParams {
     int ptt_nonneg;              // PTT ODF has no negative value: rejection rounds may stop early
     uint32_t rec_thresh;  // record a sub-step when its hash is below this
     int rec_cap;          // capacity of the pair buffer
+    int slot_cap;         // bounded chunk: phase 2 skips seeds whose streamlines end past this slot (0: none)
 };
 
 struct DevBufs {
@@ -1940,7 +1941,7 @@ __device__ void track_seed_ptt_g(const DevParams &P, const DevBufs &B, const Ptt
     WAVE_SYNC();
     DrawCache dc;
     dc.valid = 0;
-    const int ndir = offsets[s + 1] - offsets[s];
+    const int ndir = (P.slot_cap > 0 && offsets[s + 1] > P.slot_cap) ? 0 : offsets[s + 1] - offsets[s];
     int off = offsets[s];
     const size_t slot_f = (size_t)2 * P.max_sline_len * 3;
     for (int i = 0; i < ndir; ++i, ++off) {
@@ -2133,7 +2134,7 @@ __device__ void track_seed(const DevParams &P, const DevBufs &B, const float *sv
     const float sx = seeds[3 * s], sy = seeds[3 * s + 1], sz = seeds[3 * s + 2];
     Philox rng;
     rng.init(((uint64_t)P.seed_hi << 32) | P.seed_lo, (uint64_t)32 * (uint64_t)(s % rng_chunk) + l + 1);
-    const int ndir = offsets[s + 1] - offsets[s];
+    const int ndir = (P.slot_cap > 0 && offsets[s + 1] > P.slot_cap) ? 0 : offsets[s + 1] - offsets[s];
     int off = offsets[s];
     const size_t slot_f = (size_t)2 * P.max_sline_len * 3;
     uint32_t ntap = 0, nsub = 0;  // V == 3: this lane's taps, the seed's sub-steps
@@ -2199,6 +2200,160 @@ __device__ __forceinline__ int stage_network(const DevBufs &B, float *lds, DevBu
     return MLP_LDS_FLOATS;
 }
 
+// Plain PTT phase 2 of one 32-lane slot with the stepping loop flattened
+// (the queue kernel's V == 4).  In track_seed a slot whose streamline pass,
+// direction or seed ends leaves its loops and waits at their exit until the
+// wave's other slot leaves them too; here one iteration of a single loop is
+// one sub-step of whichever pass the slot holds, and a slot whose pass ends
+// finishes it (last point, reversal, length) and starts its next pass,
+// direction or seed at once, so the wave's two slots keep sub-stepping
+// together.  Each seed's work, its order and its random draws are those of
+// track_seed (genStreamlinesMergePtt, ptt.cu / generate_streamlines_cuda.cu
+// :484-621), so the results are identical.  The seed's index, direction
+// count, next direction, slot offset, current slot and backward length (ints)
+// and its seed point and direction (floats) live in the slot's LDS (the
+// learned proposal's activation area, which the plain getter leaves unused).
+__device__ void ptt_flat_slot(const DevParams &P, const DevBufs &B, SlotLds &S, const float *__restrict__ seeds,
+                              int nseeds, int rng_chunk, const int *__restrict__ offsets,
+                              const int *__restrict__ dirs, int *__restrict__ sline_seed,
+                              int *__restrict__ sline_len, float *__restrict__ sline, const int *__restrict__ order,
+                              int *__restrict__ work) {
+    const int l = lane32();
+    const int xcd = blockIdx.x & (QUEUES - 1);
+    const int step_frac = P.step_frac, MAXL = P.max_sline_len;
+    const int nsub_max = MAXL * step_frac;
+    const double sub = P.step_d / (double)step_frac;
+    const size_t slot_f = (size_t)2 * MAXL * 3;
+    int *si = reinterpret_cast<int *>(S.ptt.act);  // s, ndir, next direction, offsets[s], slot, nb
+    float *sf = S.ptt.act + 8;                     // seed point [0..2], direction [3..5]
+    if (l == 0) { si[1] = 0; si[2] = 0; }
+    WAVE_SYNC();
+    Philox rng;
+    int qk = 0;
+    bool have = false;
+    int pass = 0, i = 1, ri = 1, qi = 0;  // sub-step i = qi * step_frac + ri
+    float px = 0.0f, py = 0.0f, pz = 0.0f, dx = 0.0f, dy = 0.0f, dz = 0.0f;
+    float *sl = sline;
+    for (;;) {
+        if (!have) {
+            // the next direction of this seed, or of the next seeds
+            while (qk < QUEUES) {
+                const int di = si[2];
+                if (di >= si[1]) {
+                    const int qx = (xcd + qk) & (QUEUES - 1);
+                    const int lo = (int)((long long)nseeds * qx / QUEUES);
+                    const int hi = (int)((long long)nseeds * (qx + 1) / QUEUES);
+                    int q = 0;
+                    if (l == 0) q = atomicAdd(work + qx, 1);
+                    q = shfl32(q, 0);
+                    if (lo + q >= hi) { ++qk; continue; }
+                    const int s = order ? order[lo + q] : lo + q;
+                    const int nd = (P.slot_cap > 0 && offsets[s + 1] > P.slot_cap) ? 0 : offsets[s + 1] - offsets[s];
+                    rng.init(((uint64_t)P.seed_hi << 32) | P.seed_lo, (uint64_t)32 * (uint64_t)(s % rng_chunk) + l + 1);
+                    WAVE_SYNC();
+                    if (l == 0) {
+                        si[0] = s; si[1] = nd; si[2] = 0; si[3] = offsets[s];
+                        sf[0] = seeds[3 * s]; sf[1] = seeds[3 * s + 1]; sf[2] = seeds[3 * s + 2];
+                    }
+                    WAVE_SYNC();
+                    continue;
+                }
+                const int s = si[0], off = si[3] + di;
+                const float sx = sf[0], sy = sf[1], sz = sf[2];
+                const int vi = dirs[(size_t)s * P.nverts + di];
+                const float fx = B.verts[3 * vi], fy = B.verts[3 * vi + 1], fz = B.verts[3 * vi + 2];
+                float *cur = sline + (size_t)off * slot_f;
+                WAVE_SYNC();
+                if (l == 0) {
+                    sline_seed[off] = s;
+                    si[2] = di + 1; si[4] = off;
+                    sf[3] = fx; sf[4] = fy; sf[5] = fz;
+                }
+                WAVE_SYNC();
+                if (!init_frame_ptt(P, B, S.ptt, rng, fx, fy, fz, sx, sy, sz)) {
+                    if (l == 0) {
+                        sline_len[off] = 1;
+                        cur[0] = sx; cur[1] = sy; cur[2] = sz;
+                    }
+                    continue;
+                }
+                // backward pass (-dir) on the frame at [0..8]
+                pass = 0;
+                px = sx; py = sy; pz = sz;
+                dx = -fx; dy = -fy; dz = -fz;
+                sl = cur;
+                if (l == 0) { sl[0] = px; sl[1] = py; sl[2] = pz; }
+                i = 1; qi = (step_frac == 1) ? 1 : 0; ri = (step_frac == 1) ? 0 : 1;
+                have = true;
+                break;
+            }
+            if (!have) break;  // the queues are empty
+        }
+        // one sub-step of the pass (tracker's loop body)
+        bool end = i >= nsub_max;
+        if (!end) {
+            float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+            const int nd = get_direction_ptt<false>(P, B, S.ptt, S.ptt.frame + (pass ? 9 : 0), rng, dx, dy, dz, px, py,
+                                                    pz, &nx, &ny, &nz);
+            if (nd == 0) {
+                end = true;
+            } else {
+                dx = nx; dy = ny; dz = nz;
+                px = (float)__builtin_fma((double)(dx / 1.0f), sub, (double)px);
+                py = (float)__builtin_fma((double)(dy / 1.0f), sub, (double)py);
+                pz = (float)__builtin_fma((double)(dz / 1.0f), sub, (double)pz);
+                if (l == 0 && ri == 0) {
+                    float *o = sl + 3 * (size_t)qi;
+                    o[0] = px; o[1] = py; o[2] = pz;
+                }
+                const int tissue = check_point(P, B.stop, px, py, pz);
+                if (tissue == CUSLINES_ENDPOINT || tissue == CUSLINES_INVALIDPOINT || tissue == CUSLINES_OUTSIDEIMAGE) {
+                    end = true;
+                } else {
+                    ++i;
+                    if (++ri == step_frac) { ri = 0; ++qi; }
+                    end = i >= nsub_max;
+                }
+            }
+        }
+        if (!end) continue;
+        // the pass ends at sub-step i (tracker's tail)
+        int nsteps = qi;
+        if (ri != 0 && i < step_frac * (MAXL - 1)) {
+            ++nsteps;
+            if (l == 0) {
+                float *o = sl + 3 * (size_t)nsteps;
+                o[0] = px; o[1] = py; o[2] = pz;
+            }
+        }
+        const int off = si[4];
+        float *cur = sline + (size_t)off * slot_f;
+        if (pass == 0) {
+            const int nb = nsteps;
+            MEM_SYNC();
+            for (int j = l; j < nb / 2; j += 32) {
+                float *a = cur + 3 * (size_t)j, *b = cur + 3 * (size_t)(nb - 1 - j);
+                const float a0 = a[0], a1 = a[1], a2 = a[2];
+                a[0] = b[0]; a[1] = b[1]; a[2] = b[2];
+                b[0] = a0; b[1] = a1; b[2] = a2;
+            }
+            MEM_SYNC();
+            if (l == 0) si[5] = nb;
+            // forward pass (+dir) on the frame at [9..17], from the seed point at index nb - 1
+            pass = 1;
+            px = sf[0]; py = sf[1]; pz = sf[2];
+            dx = sf[3]; dy = sf[4]; dz = sf[5];
+            sl = cur + 3 * (size_t)(nb - 1);
+            if (l == 0) { sl[0] = px; sl[1] = py; sl[2] = pz; }
+            i = 1; qi = (step_frac == 1) ? 1 : 0; ri = (step_frac == 1) ? 0 : 1;
+            WAVE_SYNC();
+        } else {
+            if (l == 0) sline_len[off] = si[5] - 1 + nsteps;
+            have = false;
+        }
+    }
+}
+
 // One seed per 32-lane slot, static assignment.
 template <int MODEL, int V = 0>
 __global__ void __launch_bounds__(256) phase2_kernel(DevParams P, DevBufs B, const float *__restrict__ seeds,
@@ -2257,6 +2412,10 @@ __global__ void __launch_bounds__((MODEL == CUSLINES_PTT && V == 1) ? 512 : 256,
     const int l = lane32();
     SlotLds S;
     bind_slot<MODEL>(P, false, smem + svf + slot * slot_floats(MODEL, false, P.nv32, P.t32, P.hr32), S);
+    if constexpr (MODEL == CUSLINES_PTT && V == 4) {
+        ptt_flat_slot(P, BL, S, seeds, nseeds, rng_chunk, offsets, dirs, sline_seed, sline_len, sline, order, work);
+        return;
+    }
     // Workgroups are dispatched to the 8 XCDs round-robin.  Each XCD first
     // drains its own eighth of the (spatially sorted) seed order, so an XCD's
     // L2 serves one region of the volume, then helps with the others.
@@ -2414,21 +2573,33 @@ __device__ int bb_peaks(const DevParams &P, const DevBufs &B, const float *odf, 
     const int nv = P.nverts;
     uint32_t pk = 0u;
     float mn = REAL_MAX, m = REAL_MIN;
-    for (int r = 0, j = l; j < nv; ++r, j += 32) {
-        const uint4 q = reinterpret_cast<const uint4 *>(B.nbr)[j];
-        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
-        float x = odf[j];
-        x = (x < thr) ? 0.0f : x;
-        mn = (mn < x) ? mn : x;
-        bool smaller = false, larger = false;
+    // groups of 4 vertices per lane: the 4 adjacency rows are loaded before
+    // the first is used, so their latencies overlap
+    for (int r0 = 0, j0 = l; j0 < nv; r0 += 4, j0 += 128) {
+        uint4 qs[4];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            float y = odf[(w[k >> 1] >> (16 * (k & 1))) & 0xffffu];
-            y = (y < thr) ? 0.0f : y;
-            smaller = smaller || (y < x);
-            larger = larger || (x < y);
+        for (int u = 0; u < 4; ++u) {
+            const int j = min(j0 + 32 * u, nv - 1);  // past the end: re-read a valid row, result dropped
+            qs[u] = reinterpret_cast<const uint4 *>(B.nbr)[j];
         }
-        if (smaller && !larger) pk |= 1u << r;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int j = j0 + 32 * u;
+            if (j >= nv) break;
+            const uint32_t w[4] = {qs[u].x, qs[u].y, qs[u].z, qs[u].w};
+            float x = odf[j];
+            x = (x < thr) ? 0.0f : x;
+            mn = (mn < x) ? mn : x;
+            bool smaller = false, larger = false;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                float y = odf[(w[k >> 1] >> (16 * (k & 1))) & 0xffffu];
+                y = (y < thr) ? 0.0f : y;
+                smaller = smaller || (y < x);
+                larger = larger || (x < y);
+            }
+            if (smaller && !larger) pk |= 1u << (r0 + u);
+        }
     }
     mn = min32(mn);
     mn = (0.0f > mn) ? 0.0f : mn;
@@ -2477,7 +2648,8 @@ __device__ __forceinline__ void bb_start_pass(const DevParams &P, BbSlot &st, fl
 }
 
 // Next seed with at least one direction from the XCD-ordered queues, or
-// st.done.  Streamlines of seeds without directions need no work.
+// st.done.  Streamlines of seeds without directions need no work; in a
+// bounded chunk (P.slot_cap) neither do seeds whose slots end past the cap.
 __device__ void bb_next_seed(const DevParams &P, const DevBufs &B, BbSlot &st, Philox &rng, int nseeds,
                              const float *__restrict__ seeds, int rng_chunk, const int *__restrict__ offsets,
                              const int *__restrict__ dirs, const int *__restrict__ order, int *__restrict__ work,
@@ -2493,7 +2665,7 @@ __device__ void bb_next_seed(const DevParams &P, const DevBufs &B, BbSlot &st, P
         q = shfl32(q, 0);
         if (lo + q >= hi) { ++st.qk; continue; }
         const int s = order ? order[lo + q] : lo + q;
-        const int nd = offsets[s + 1] - offsets[s];
+        const int nd = (P.slot_cap > 0 && offsets[s + 1] > P.slot_cap) ? 0 : offsets[s + 1] - offsets[s];
         if (nd == 0) continue;
         st.s = s;
         st.ndir = nd;
@@ -3088,6 +3260,22 @@ __global__ void compact_kernel(const int *__restrict__ len, int nsl, const float
     }
 }
 
+// Totals of a bounded chunk (one thread): the chunk's streamlines
+// (offsets[n]), the slots phase 2 filled -- the largest offset <= cap, as
+// the offsets never decrease and a seed is tracked when its last slot is
+// below the cap -- and the points compacted (point_off[cap]).
+__global__ void chunk_totals(const int *__restrict__ offsets, int n, int cap, const long long *__restrict__ point_off,
+                             long long *__restrict__ totals) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (offsets[mid] <= cap) lo = mid; else hi = mid - 1;
+    }
+    totals[0] = offsets[n];
+    totals[1] = offsets[lo];
+    totals[2] = point_off[cap];
+}
+
 }  // namespace
 
 // Workspace of cuslines_phase2_noalloc_async: the spatial seed order of at
@@ -3145,6 +3333,7 @@ struct cuslines_tracker {
     int blocks_per_cu = 0;  // 0: occupancy maximum
     int num_cu = 1;
     int ptt_lanes = 0;      // PTT: lanes per seed (8, 16, 32; 0 = shared 32-lane kernel)
+    int ptt_flat = 0;       // plain PTT on the queue: one loop of sub-steps per slot (ptt_flat_slot)
     float *d_fprop = nullptr;            // learned proposal: feature propagators [24][9]
     float *d_vprop = nullptr;            // PTT vertex-phase propagators [24][9]
     uint16_t *d_mlp = nullptr;           // learned proposal (nullptr: plain getter)
@@ -3155,6 +3344,15 @@ struct cuslines_tracker {
     bool has_data = false, has_stop = false, has_sphere = false, has_boot = false, has_lut = false;
     cuslines_tracker *parent = nullptr;  // fork: the tracker whose device data it shares
     int nforks = 0;                      // live forks of this tracker
+    int ngraphs = 0;                     // live chunk graphs on this tracker (a parent: and on its forks)
+};
+
+// A captured chunk (cuslines_chunk_graph_create): its kernel nodes hold the
+// tracker's device pointers and launch settings as they were at capture.
+struct cuslines_graph {
+    cuslines_tracker *t;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
 };
 
 static float smallest_float_at_least(double x) {
@@ -3224,6 +3422,7 @@ static int check_ready(const cuslines_tracker *t) {
 static int check_owner(const cuslines_tracker *t) {
     if (t->parent) return fail(CUSLINES_ESTATE, "a fork shares its parent's data: configure the parent");
     if (t->nforks) return fail(CUSLINES_ESTATE, "live forks share this tracker's data: destroy them first");
+    if (t->ngraphs) return fail(CUSLINES_ESTATE, "live chunk graphs hold this tracker's data: destroy them first");
     return CUSLINES_OK;
 }
 
@@ -3450,6 +3649,8 @@ static int launch_phase2(cuslines_tracker *t, const float *seeds, int n, int chu
             return launch_phase2_v<MODEL, 3>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl, ws);
         if (t->ptt_lanes > 0 && t->schedule != CUSLINES_SCHED_STATIC)
             return launch_ptt_groups(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl);
+        if (t->ptt_flat && t->schedule != CUSLINES_SCHED_STATIC)
+            return launch_phase2_v<MODEL, 4>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl, ws);
     }
     return launch_phase2_v<MODEL, 0>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl, ws);
 }
@@ -3471,6 +3672,113 @@ static int run_scan(cuslines_tracker *t, const int *in, int n, TO *out, TO *bsum
     if (!bsum_ws) (void)hipFreeAsync(bsum, t->stream);
     if (e != hipSuccess) return fail(CUSLINES_EHIP, std::string("scan: ") + hipGetErrorString(e));
     return CUSLINES_OK;
+}
+
+static int phase1_models(cuslines_tracker *t, const float *seeds, int n, int chunk, int *counts, int *dirs) {
+    switch (t->p.model_type) {
+        case CUSLINES_OPDT: return launch_phase1<CUSLINES_OPDT>(t, seeds, n, chunk, counts, dirs);
+        case CUSLINES_CSA: return launch_phase1<CUSLINES_CSA>(t, seeds, n, chunk, counts, dirs);
+        case CUSLINES_PROB: return launch_phase1<CUSLINES_PROB>(t, seeds, n, chunk, counts, dirs);
+        default: return launch_phase1<CUSLINES_PTT>(t, seeds, n, chunk, counts, dirs);
+    }
+}
+
+// Phase 2 of the handle's model; `ws` as launch_phase2_v (nullptr:
+// stream-ordered allocations).
+static int phase2_models(cuslines_tracker *t, const float *seeds, int n, int chunk, const int *offsets,
+                         const int *dirs, int *sseed, int *slen, float *sl, int *ws) {
+    switch (t->p.model_type) {
+        case CUSLINES_OPDT:
+            return launch_phase2<CUSLINES_OPDT>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl, ws);
+        case CUSLINES_CSA:
+            return launch_phase2<CUSLINES_CSA>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl, ws);
+        case CUSLINES_PROB:
+            return launch_phase2<CUSLINES_PROB>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl, ws);
+        default:
+            return launch_phase2<CUSLINES_PTT>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl, ws);
+    }
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Workspace of a bounded chunk: the seed order's (order_ws_ints), then the
+// block sums of the streamline-count scan (int) and of the point scan (int64).
+static size_t chunk_ws_bytes(const DevParams &d, int n, int cap) {
+    int bl = 0;
+    return align256(order_ws_ints(n, order_buckets(d, &bl)) * sizeof(int)) +
+           align256((size_t)((n + 1023) / 1024) * sizeof(int)) + (size_t)((cap + 1023) / 1024) * sizeof(long long);
+}
+
+// The stream work of cuslines_chunk_async (arguments already checked):
+// phase 1, the scan, the lengths of the `cap` slots cleared, phase 2 of the
+// seeds that fit (DevParams::slot_cap), the point scan and compaction over
+// the `cap` slots and the totals.  No allocation and no host read.
+static int chunk_enqueue(cuslines_tracker *t, const float *seeds, int n, int chunk, int *counts, int *dirs,
+                         int *offsets, int cap, int *sseed, int *slen, float *sl, long long *poff, float *pts,
+                         long long *totals, char *ws) {
+    int bl = 0;
+    int *order_ws = reinterpret_cast<int *>(ws);
+    char *p = ws + align256(order_ws_ints(n, order_buckets(t->dp, &bl)) * sizeof(int));
+    int *scan_ws = reinterpret_cast<int *>(p);
+    long long *pscan_ws = reinterpret_cast<long long *>(p + align256((size_t)((n + 1023) / 1024) * sizeof(int)));
+    if (n > 0)
+        if (int rc = phase1_models(t, seeds, n, chunk, counts, dirs)) return rc;
+    if (int rc = run_scan<int>(t, counts, n, offsets, scan_ws)) return rc;
+    hipLaunchKernelGGL(zero_ints, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, t->stream, slen, cap);
+    HIPC(hipGetLastError());
+    if (n > 0) {
+        t->dp.slot_cap = cap;  // copied into the launches' arguments
+        const int rc = phase2_models(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl, order_ws);
+        t->dp.slot_cap = 0;
+        if (rc) return rc;
+    }
+    if (int rc = run_scan<long long>(t, slen, cap, poff, pscan_ws)) return rc;
+    const size_t slot_f = (size_t)2 * t->dp.max_sline_len * 3;
+    const unsigned grid = (unsigned)std::min<long long>(((long long)cap + 3) / 4, 65536);
+    hipLaunchKernelGGL(compact_kernel, dim3(grid), dim3(256), 0, t->stream, slen, cap, sl, slot_f, poff, pts);
+    hipLaunchKernelGGL(chunk_totals, dim3(1), dim3(1), 0, t->stream, offsets, n, cap, poff, totals);
+    HIPC(hipGetLastError());
+    return CUSLINES_OK;
+}
+
+struct chunk_args {
+    const float *seeds;
+    int n, chunk;
+    int *counts, *dirs, *offsets;
+    int cap;
+    int *sseed, *slen;
+    float *sl;
+    long long *poff;
+    float *pts;
+    long long *totals;
+    char *ws;
+};
+
+static int chunk_check(cuslines_tracker *t, const float *seeds_d, int32_t nseeds, int32_t rng_chunk, int32_t *counts_d,
+                       int32_t *dirs_d, int32_t *offsets_d, int32_t slot_cap, int32_t *sline_seed_d,
+                       int32_t *sline_len_d, float *sline_d, int64_t *point_off_d, float *points_d, int64_t *totals_d,
+                       void *ws_d, size_t ws_bytes, chunk_args *a) {
+    if (!t) return fail(CUSLINES_EINVAL, "null tracker");
+    if (nseeds < 0 || rng_chunk < 0) return fail(CUSLINES_EINVAL, "negative size");
+    if (slot_cap < 1) return fail(CUSLINES_EINVAL, "slot_cap must be at least 1");
+    if ((nseeds > 0 && (!seeds_d || !counts_d || !dirs_d)) || !offsets_d || !sline_seed_d || !sline_len_d ||
+        !sline_d || !point_off_d || !points_d || !totals_d || !ws_d)
+        return fail(CUSLINES_EINVAL, "null buffer");
+    if (int rc = check_ready(t)) return rc;
+    const size_t need = chunk_ws_bytes(t->dp, nseeds, slot_cap);
+    if (ws_bytes < need)
+        return fail(CUSLINES_EINVAL, "workspace of " + std::to_string(ws_bytes) + " bytes, " + std::to_string(need) +
+                                         " needed (cuslines_chunk_workspace_bytes)");
+    if ((reinterpret_cast<uintptr_t>(ws_d) & 255) != 0) return fail(CUSLINES_EINVAL, "workspace not 256-byte aligned");
+    *a = chunk_args{seeds_d, nseeds, rng_chunk > 0 ? rng_chunk : nseeds, counts_d, dirs_d, offsets_d, slot_cap,
+                    sline_seed_d, sline_len_d, sline_d, reinterpret_cast<long long *>(point_off_d), points_d,
+                    reinterpret_cast<long long *>(totals_d), static_cast<char *>(ws_d)};
+    return CUSLINES_OK;
+}
+
+static int chunk_run(cuslines_tracker *t, const chunk_args &a) {
+    return chunk_enqueue(t, a.seeds, a.n, a.chunk, a.counts, a.dirs, a.offsets, a.cap, a.sseed, a.slen, a.sl, a.poff,
+                         a.pts, a.totals, a.ws);
 }
 
 // Both phase-2 entry points.  noalloc: the seed-order workspace is this
@@ -3501,20 +3809,7 @@ static int phase2_entry(cuslines_tracker *t, const float *seeds_d, int32_t nseed
     }
     HIPC(hipSetDevice(t->device));
     const int chunk = rng_chunk > 0 ? rng_chunk : nseeds;
-    switch (t->p.model_type) {
-        case CUSLINES_OPDT:
-            return launch_phase2<CUSLINES_OPDT>(t, seeds_d, nseeds, chunk, offsets_d, dirs_d, sline_seed_d,
-                                                sline_len_d, sline_d, ws);
-        case CUSLINES_CSA:
-            return launch_phase2<CUSLINES_CSA>(t, seeds_d, nseeds, chunk, offsets_d, dirs_d, sline_seed_d,
-                                               sline_len_d, sline_d, ws);
-        case CUSLINES_PROB:
-            return launch_phase2<CUSLINES_PROB>(t, seeds_d, nseeds, chunk, offsets_d, dirs_d, sline_seed_d,
-                                                sline_len_d, sline_d, ws);
-        default:
-            return launch_phase2<CUSLINES_PTT>(t, seeds_d, nseeds, chunk, offsets_d, dirs_d, sline_seed_d,
-                                               sline_len_d, sline_d, ws);
-    }
+    return phase2_models(t, seeds_d, nseeds, chunk, offsets_d, dirs_d, sline_seed_d, sline_len_d, sline_d, ws);
 }
 
 extern "C" {
@@ -3633,6 +3928,7 @@ int cuslines_tracker_create(const cuslines_params *p, int device, cuslines_track
 int cuslines_tracker_destroy(cuslines_tracker *t) {
     if (!t) return CUSLINES_OK;
     if (t->nforks) return fail(CUSLINES_ESTATE, "destroy the tracker's forks first");
+    if (t->ngraphs) return fail(CUSLINES_ESTATE, "destroy the tracker's chunk graphs first");
     (void)hipSetDevice(t->device);
     if (t->stream) (void)hipStreamSynchronize(t->stream);
     if (t->parent) {  // a fork owns only its stream and counters
@@ -3661,6 +3957,7 @@ int cuslines_tracker_fork(cuslines_tracker *t, cuslines_tracker **out) {
     cuslines_tracker *f = new cuslines_tracker(*t);  // shares every device buffer pointer
     f->parent = t;
     f->nforks = 0;
+    f->ngraphs = 0;
     f->stream = nullptr;
     f->d_work = nullptr;
     f->d_stats = nullptr;  // learned statistics and tap counts stay with the parent
@@ -4031,6 +4328,13 @@ int cuslines_set_boot_mode(cuslines_tracker *t, int mode) {
     return CUSLINES_OK;
 }
 
+int cuslines_set_ptt_stepping(cuslines_tracker *t, int flat) {
+    if (!t) return fail(CUSLINES_EINVAL, "null tracker");
+    if (flat != 0 && flat != 1) return fail(CUSLINES_EINVAL, "PTT stepping must be 0 (per seed) or 1 (flat)");
+    t->ptt_flat = flat;
+    return CUSLINES_OK;
+}
+
 int cuslines_set_lanes_per_seed(cuslines_tracker *t, int lanes) {
     if (!t) return fail(CUSLINES_EINVAL, "null tracker");
     if (lanes != 0 && lanes != 8 && lanes != 16 && lanes != 32)
@@ -4335,13 +4639,7 @@ int cuslines_phase1_async(cuslines_tracker *t, const float *seeds_d, int32_t nse
     int rc = check_ready(t);
     if (rc) return rc;
     HIPC(hipSetDevice(t->device));
-    const int chunk = rng_chunk > 0 ? rng_chunk : nseeds;
-    switch (t->p.model_type) {
-        case CUSLINES_OPDT: return launch_phase1<CUSLINES_OPDT>(t, seeds_d, nseeds, chunk, counts_d, dirs_d);
-        case CUSLINES_CSA: return launch_phase1<CUSLINES_CSA>(t, seeds_d, nseeds, chunk, counts_d, dirs_d);
-        case CUSLINES_PROB: return launch_phase1<CUSLINES_PROB>(t, seeds_d, nseeds, chunk, counts_d, dirs_d);
-        default: return launch_phase1<CUSLINES_PTT>(t, seeds_d, nseeds, chunk, counts_d, dirs_d);
-    }
+    return phase1_models(t, seeds_d, nseeds, rng_chunk > 0 ? rng_chunk : nseeds, counts_d, dirs_d);
 }
 
 int cuslines_scan_async(cuslines_tracker *t, const int32_t *counts_d, int32_t nseeds, int32_t *offsets_d) {
@@ -4379,6 +4677,79 @@ int cuslines_compact_async(cuslines_tracker *t, const int32_t *sline_len_d, int3
     hipLaunchKernelGGL(compact_kernel, dim3(grid), dim3(256), 0, t->stream, sline_len_d, nslines, sline_d, slot_f,
                        reinterpret_cast<const long long *>(point_off_d), points_d);
     HIPC(hipGetLastError());
+    return CUSLINES_OK;
+}
+
+int cuslines_chunk_workspace_bytes(cuslines_tracker *t, int32_t nseeds, int32_t slot_cap, size_t *bytes) {
+    if (!t || !bytes) return fail(CUSLINES_EINVAL, "null argument");
+    if (nseeds < 0 || slot_cap < 1) return fail(CUSLINES_EINVAL, "negative size or slot_cap < 1");
+    *bytes = chunk_ws_bytes(t->dp, nseeds, slot_cap);
+    return CUSLINES_OK;
+}
+
+int cuslines_chunk_async(cuslines_tracker *t, const float *seeds_d, int32_t nseeds, int32_t rng_chunk,
+                         int32_t *counts_d, int32_t *dirs_d, int32_t *offsets_d, int32_t slot_cap,
+                         int32_t *sline_seed_d, int32_t *sline_len_d, float *sline_d, int64_t *point_off_d,
+                         float *points_d, int64_t *totals_d, void *ws_d, size_t ws_bytes) {
+    RoctxRange range_("cuslines_chunk");
+    chunk_args a;
+    if (int rc = chunk_check(t, seeds_d, nseeds, rng_chunk, counts_d, dirs_d, offsets_d, slot_cap, sline_seed_d,
+                             sline_len_d, sline_d, point_off_d, points_d, totals_d, ws_d, ws_bytes, &a))
+        return rc;
+    HIPC(hipSetDevice(t->device));
+    return chunk_run(t, a);
+}
+
+int cuslines_chunk_graph_create(cuslines_tracker *t, const float *seeds_d, int32_t nseeds, int32_t rng_chunk,
+                                int32_t *counts_d, int32_t *dirs_d, int32_t *offsets_d, int32_t slot_cap,
+                                int32_t *sline_seed_d, int32_t *sline_len_d, float *sline_d, int64_t *point_off_d,
+                                float *points_d, int64_t *totals_d, void *ws_d, size_t ws_bytes,
+                                cuslines_graph **out) {
+    RoctxRange range_("cuslines_chunk_graph_create");
+    if (!out) return fail(CUSLINES_EINVAL, "null argument");
+    *out = nullptr;
+    chunk_args a;
+    if (int rc = chunk_check(t, seeds_d, nseeds, rng_chunk, counts_d, dirs_d, offsets_d, slot_cap, sline_seed_d,
+                             sline_len_d, sline_d, point_off_d, points_d, totals_d, ws_d, ws_bytes, &a))
+        return rc;
+    HIPC(hipSetDevice(t->device));
+    HIPC(hipStreamBeginCapture(t->stream, hipStreamCaptureModeRelaxed));
+    const int rc = chunk_run(t, a);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(t->stream, &g);
+    if (rc || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc ? rc : fail(CUSLINES_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    }
+    hipGraphExec_t x = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(CUSLINES_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    }
+    *out = new cuslines_graph{t, g, x};
+    ++t->ngraphs;
+    if (t->parent) ++t->parent->ngraphs;
+    return CUSLINES_OK;
+}
+
+int cuslines_graph_launch(cuslines_graph *g) {
+    if (!g) return fail(CUSLINES_EINVAL, "null graph");
+    HIPC(hipSetDevice(g->t->device));
+    HIPC(hipGraphLaunch(g->exec, g->t->stream));
+    return CUSLINES_OK;
+}
+
+int cuslines_graph_destroy(cuslines_graph *g) {
+    if (!g) return CUSLINES_OK;
+    (void)hipSetDevice(g->t->device);
+    (void)hipStreamSynchronize(g->t->stream);  // a launch may still run
+    const hipError_t e1 = hipGraphExecDestroy(g->exec);
+    const hipError_t e2 = hipGraphDestroy(g->graph);
+    --g->t->ngraphs;
+    if (g->t->parent) --g->t->parent->ngraphs;
+    delete g;
+    if (e1 != hipSuccess || e2 != hipSuccess) return fail(CUSLINES_EHIP, "hipGraph destroy failed");
     return CUSLINES_OK;
 }
 

@@ -72,7 +72,8 @@ class GenericTracker:
                 # handles that share the device data
                 from .propagate_seeds import ChunkPipeline
 
-                pipe = ChunkPipeline(self.device_trackers[0], depth=depth, minlen=sp.minlen, maxlen=sp.maxlen)
+                pipe = ChunkPipeline(self.device_trackers[0], depth=depth, minlen=sp.minlen, maxlen=sp.maxlen,
+                                     graphs=bool(getattr(self, "chunk_graphs", True)))
                 try:
                     for pts, lens in pipe.run(seeds, gsz):
                         if len(lens):

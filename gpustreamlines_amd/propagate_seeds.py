@@ -12,13 +12,11 @@ therefore the packed float32 [n_points, 3] array of GPU ``n`` and
 
 from __future__ import annotations
 
-import ctypes as C
 import logging
 import math
 
 import numpy as np
 
-from . import _lib
 from ._globals import MAX_SLINE_LEN, REAL_SIZE, THR_X_BL, THR_X_SL
 from .array_sequence import MEGABYTE, make_array_sequence
 
@@ -196,7 +194,7 @@ class SeedBatchPropagator:
 
 
 class ChunkPipeline:
-    """The reference's chunk protocol (one phase-1/scan/phase-2 launch per
+    """The reference's chunk protocol (one phase-1/scan/phase-2 pass per
     chunk, random streams numbered per chunk) on ``depth`` handles that
     share one GPU's device data (``DeviceTracker.fork``), so consecutive
     chunks run concurrently: a chunk's phase 2 fills the GPU while the
@@ -204,74 +202,76 @@ class ChunkPipeline:
     paying the persistent kernel's tail alone (cu_propagate_seeds.py:206-233
     runs chunk after chunk on one stream).
 
-    Chunk k goes to handle k % depth.  Launching a chunk waits only for its
-    own handle's phase 1 and scan (the streamline count sizes the slots);
-    results are collected in chunk order.  Device buffers grow and are
-    reused across chunks (freeing memory would synchronise the device)."""
+    A chunk is one bounded chunk (cuslines_chunk_async): phase 1, scan,
+    phase 2 and compaction enqueued with a fixed slot capacity, with no host
+    read between them (cu_propagate_seeds.py:95-201 reads the streamline
+    count back to size the slots), and with ``graphs`` replayed as one HIP
+    graph per handle.  Chunk k goes to handle k % depth; results are
+    collected in chunk order.  A chunk whose streamlines exceed the slots
+    runs again with more (the same results).  Buffers grow and are reused
+    across chunks (freeing memory would synchronise the device)."""
 
-    def __init__(self, dt, depth: int = 4, minlen=0, maxlen=np.inf):
+    def __init__(self, dt, depth: int = 4, minlen=0, maxlen=np.inf, graphs: bool = True):
+        from ._device import ChunkBuffers
+
+        self._CB = ChunkBuffers
         self.dt = dt
         self.handles = [dt] + [dt.fork() for _ in range(max(1, int(depth)) - 1)]
-        self.bufs = [dict() for _ in self.handles]
+        self.cbs = [None] * len(self.handles)
         self.pending = [None] * len(self.handles)
         self.minlen, self.maxlen = minlen, maxlen
+        self.graphs = bool(graphs)
+        self.reruns = 0
 
-    def _buf(self, k, name, nbytes):
-        b = self.bufs[k].get(name)
-        if b is None or b.nbytes < nbytes:
-            if b is not None:
-                b.free()
-            b = self.handles[k].alloc(max(int(nbytes), 4) if b is None else max(int(nbytes), 3 * b.nbytes // 2))
-            self.bufs[k][name] = b
-        return b
+    def _cb(self, k, n, cap):
+        cb = self.cbs[k]
+        if cb is None or cb.nmax < n or cb.cap < cap:
+            if cb is not None:
+                nmax, cap = max(n, cb.nmax), max(cap, 3 * cb.cap // 2)
+                cb.close()
+            else:
+                nmax = n
+            self.cbs[k] = None
+            cb = self.cbs[k] = self._CB(self.handles[k], nmax, cap)
+        return cb
 
     def _launch(self, k, seeds, slot_hint):
-        h = self.handles[k]
         n = len(seeds)
-        nv = h.params["nverts"]
-        L = int(h.params.get("max_sline_len", MAX_SLINE_LEN))
-        sd = self._buf(k, "seeds", seeds.nbytes)
-        cd = self._buf(k, "counts", 4 * (n + 1))
-        dd = self._buf(k, "dirs", 4 * n * nv)
-        od = self._buf(k, "offsets", 4 * (n + 1))
-        sd.upload(seeds)
-        h.phase1_async(sd.ptr, n, n, cd.ptr, dd.ptr)
-        h.scan_async(cd.ptr, n, od.ptr)
-        tot = np.zeros(1, dtype=np.int32)
-        _lib.check(h.L.cuslines_memcpy_dtoh(h.h, tot.ctypes.data, C.c_void_p(od.ptr + 4 * n), 4), "dtoh")
-        nsl = int(tot[0])
-        slot_b = REAL_SIZE * 2 * 3 * L
-        ssd = self._buf(k, "sline_seed", 4 * max(nsl, slot_hint))
-        sld = self._buf(k, "sline_len", 4 * max(nsl, slot_hint))
-        sl = self._buf(k, "slots", slot_b * max(nsl, slot_hint))
-        if nsl:
-            h.phase2_async(sd.ptr, n, n, od.ptr, dd.ptr, ssd.ptr, sld.ptr, sl.ptr)
-        self.pending[k] = nsl
+        cb = self._cb(k, n, max(slot_hint, 1))
+        cb.sd.upload(seeds)
+        cb.enqueue(n, n, graph=self.graphs)
+        self.pending[k] = seeds
 
     def _collect(self, k):
-        h = self.handles[k]
-        nsl = self.pending[k]
+        seeds = self.pending[k]
         self.pending[k] = None
+        n = len(seeds)
+        cb = self.cbs[k]
+        nsl, tracked, _ = cb.totals()
+        if tracked < nsl:  # the slots were too few: the same chunk with more
+            self.reruns += 1
+            cb = self._cb(k, n, nsl)
+            cb.sd.upload(seeds)
+            cb.enqueue(n, n, graph=False)
+            nsl, tracked, _ = cb.totals()
+            assert tracked == nsl
         if not nsl:
             return np.zeros((0, 3), np.float32), np.zeros(0, np.int64)
-        lens = self.bufs[k]["sline_len"].download(np.zeros(nsl, dtype=np.int32))
-        npts = int(lens.astype(np.int64).sum())
-        pod = self._buf(k, "point_offsets", 8 * (nsl + 1))
-        ptd = self._buf(k, "points", 12 * npts)
-        h.compact_async(self.bufs[k]["sline_len"].ptr, nsl, self.bufs[k]["slots"].ptr, pod.ptr, ptd.ptr)
-        pts = ptd.download(np.zeros((npts, 3), dtype=np.float32))
-        lens = lens.astype(np.int64)
+        r = cb.collect(n, with_seeds=False)
+        pts, lens = r["points"], r["sline_len"].astype(np.int64)
         keep = (lens >= self.minlen) & (lens <= self.maxlen)
         if not keep.all():
             pts, lens = pts[np.repeat(keep, lens)], lens[keep]
         return pts, lens
 
-    def run(self, seeds, chunk: int):
-        """(points, lengths) of every chunk of ``chunk`` seeds, in order."""
+    def run(self, seeds, chunk: int, slots_per_seed: float = 8):
+        """(points, lengths) of every chunk of ``chunk`` seeds, in order.
+        ``slots_per_seed`` sizes the slots reserved up front (streamlines per
+        seed are about 1 to 8; a chunk with more runs again)."""
         seeds = np.ascontiguousarray(seeds, dtype=np.float32).reshape(-1, 3)
         nchunks = -(-len(seeds) // chunk) if len(seeds) else 0
         depth = len(self.handles)
-        hint = 8 * chunk  # slot capacity reserved up front: streamlines per seed are ~1-8
+        hint = max(1, int(slots_per_seed * chunk))
         order = []
         for c in range(nchunks):
             k = c % depth
@@ -284,10 +284,10 @@ class ChunkPipeline:
             yield self._collect(k)
 
     def close(self):
-        for k, bufs in enumerate(self.bufs):
-            for b in bufs.values():
-                b.free()
-            self.bufs[k] = {}
+        for k, cb in enumerate(self.cbs):
+            if cb is not None:
+                cb.close()
+            self.cbs[k] = None
         for h in self.handles[1:]:
             h.close()
         self.handles = self.handles[:1]

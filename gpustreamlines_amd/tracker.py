@@ -31,7 +31,7 @@ class GPUTracker(GenericTracker):
                  min_pts=0, max_pts=np.inf, relative_peak_thresh: float = 0.25,
                  min_separation_angle: float = radians(45), ngpus: int = 1, rng_seed: int = 0,
                  rng_offset: int = 0, chunk_size: int = 25000, launch_seeds: int = 250000,
-                 chunk_pipeline: int = 4):
+                 chunk_pipeline: int = 8, chunk_graphs: bool = True):
         """Same parameters as the reference GPUTracker
         (cu_tractography.py:57-109); ``step_size`` is in voxels.
         ``launch_seeds`` (not in the reference): on one GPU, chunks are
@@ -40,7 +40,9 @@ class GPUTracker(GenericTracker):
         launch).  ``chunk_pipeline`` (not in the reference): with one chunk
         per launch, this many chunks run concurrently on handles sharing the
         device data (1: one after the other); the output does not depend on
-        it."""
+        it.  ``chunk_graphs`` (not in the reference): those chunks run as one
+        HIP graph per handle (phase 1, scan, phase 2 and compaction with no
+        host read between them)."""
         _lib.load()
         self.dimx, self.dimy, self.dimz, self.dimt = dataf.shape
         if hasattr(dg, "prepare_data"):
@@ -75,6 +77,7 @@ class GPUTracker(GenericTracker):
         self.chunk_size = int(chunk_size)
         self.launch_seeds = int(launch_seeds)
         self.chunk_pipeline = int(chunk_pipeline)
+        self.chunk_graphs = bool(chunk_graphs)
         self.sphere_symm = bool(sphere_symm)
         avail = _lib.device_count()
         if self.ngpus > avail:

@@ -228,6 +228,15 @@ int cuslines_boot_counters(cuslines_tracker *t, int64_t *out2, int reset);
  * getters (which always use 32 lanes per seed). */
 int cuslines_set_lanes_per_seed(cuslines_tracker *t, int lanes);
 
+/* Plain PTT phase 2 on the work queue (lanes per seed 0): flat = 1 runs
+ * each 32-lane seed slot as one loop of sub-steps in which a slot whose
+ * streamline pass ends starts its next pass, direction or seed at once, so
+ * the two slots of a wavefront keep stepping together; flat = 0 (default)
+ * runs the reference's nesting (genStreamlinesMerge: per seed, per
+ * direction, per pass, per step), where a slot whose loop ends waits for the
+ * other slot's.  Same output either way.  No counterpart in the reference. */
+int cuslines_set_ptt_stepping(cuslines_tracker *t, int flat);
+
 /* PTT probe taps: 1 (default) reads each tap's 8 texels as one 32-byte
  * corner block, an expanded copy of the ODF built once both
  * cuslines_set_data and cuslines_set_ptt_lut have been called, when HBM
@@ -362,6 +371,56 @@ int cuslines_phase2_noalloc_async(cuslines_tracker *t, const float *seeds_d, int
  * point_off_d int64 [nslines + 1] the exclusive scan of sline_len_d. */
 int cuslines_compact_async(cuslines_tracker *t, const int32_t *sline_len_d, int32_t nslines,
                            const float *sline_d, int64_t *point_off_d, float *points_d);
+
+/*
+ * One chunk of the reference's protocol without its host round trip
+ * (cuslines/cuda_python/cu_propagate_seeds.py:95-201 reads the streamline
+ * count back between phase 1 and phase 2 to size the output slots): phase 1,
+ * the scan, phase 2 and the compaction are enqueued on the handle's stream
+ * against a fixed slot capacity.  The call reads nothing back and allocates
+ * nothing (the workspace is the caller's), so it can be captured in a HIP
+ * graph (cuslines_chunk_graph_create).  No counterpart in the reference.
+ *   seeds_d, counts_d, dirs_d, offsets_d  as cuslines_phase1_async and
+ *                cuslines_scan_async (counts_d [nseeds + 1], offsets_d [nseeds + 1])
+ *   slot_cap     slots of sline_seed_d, sline_len_d and sline_d (>= 1)
+ *   point_off_d  int64 [slot_cap + 1]: exclusive scan of the slot lengths
+ *   points_d     float32 [slot_cap * (2 * max_sline_len - 1)][3] (the most the
+ *                slots can hold); the compacted points of the tracked streamlines
+ *   totals_d     int64 [3]: streamlines of the chunk (offsets_d[nseeds]), slots
+ *                tracked, points compacted
+ *   ws_d         cuslines_chunk_workspace_bytes(nseeds, slot_cap) bytes,
+ *                256-byte aligned (cuslines_device_alloc memory is)
+ * Phase 2 tracks the seeds whose streamlines all lie below slot_cap -- a
+ * prefix of the seeds, as slots follow seed order -- with the results
+ * cuslines_phase2_async gives them; every other slot reads length 0.  When
+ * totals_d[0] > totals_d[1] the chunk did not fit: run it again with
+ * slot_cap >= totals_d[0].
+ */
+int cuslines_chunk_workspace_bytes(cuslines_tracker *t, int32_t nseeds, int32_t slot_cap, size_t *bytes);
+int cuslines_chunk_async(cuslines_tracker *t, const float *seeds_d, int32_t nseeds, int32_t rng_chunk,
+                         int32_t *counts_d, int32_t *dirs_d, int32_t *offsets_d, int32_t slot_cap,
+                         int32_t *sline_seed_d, int32_t *sline_len_d, float *sline_d, int64_t *point_off_d,
+                         float *points_d, int64_t *totals_d, void *ws_d, size_t ws_bytes);
+
+/*
+ * HIP graph of one cuslines_chunk_async call with fixed arguments, captured
+ * on the handle's stream; it replaces the per-chunk kernel launches of
+ * cuslines/cuda_python/cu_propagate_seeds.py:206-233.  The buffers' contents
+ * may change between launches (copy the next chunk's seeds into seeds_d,
+ * then launch); the launches run as the handle was set up at capture (getter
+ * data, launch shape, proposal).  cuslines_graph_launch enqueues the graph on
+ * the handle's stream.  While a graph lives, its tracker (and a fork's
+ * parent) refuses to be destroyed or to take new data (CUSLINES_ESTATE).
+ * No counterpart in the reference.
+ */
+typedef struct cuslines_graph cuslines_graph;
+int cuslines_chunk_graph_create(cuslines_tracker *t, const float *seeds_d, int32_t nseeds, int32_t rng_chunk,
+                                int32_t *counts_d, int32_t *dirs_d, int32_t *offsets_d, int32_t slot_cap,
+                                int32_t *sline_seed_d, int32_t *sline_len_d, float *sline_d, int64_t *point_off_d,
+                                float *points_d, int64_t *totals_d, void *ws_d, size_t ws_bytes,
+                                cuslines_graph **out);
+int cuslines_graph_launch(cuslines_graph *g);
+int cuslines_graph_destroy(cuslines_graph *g);
 
 #ifdef __cplusplus
 }

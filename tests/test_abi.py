@@ -81,3 +81,21 @@ def test_training_batch_limit_is_checked_without_gpu():
     assert L.cuslines_mlp_workspace_bytes(8_388_481, ctypes.byref(nbytes)) == 1  # CUSLINES_EINVAL
     assert "8,388,480" in _lib.last_error()
     assert L.cuslines_mlp_workspace_bytes(0, ctypes.byref(nbytes)) == 1
+
+
+def test_chunk_entry_points_check_arguments_without_gpu():
+    """The bounded-chunk and graph entry points refuse null handles before
+    any HIP call; destroying a null graph is a no-op."""
+    from gpustreamlines_amd import _lib
+
+    L = _lib.load()
+    nbytes = ctypes.c_size_t()
+    assert L.cuslines_chunk_workspace_bytes(None, 10, 10, ctypes.byref(nbytes)) == 1  # CUSLINES_EINVAL
+    args = [None, None, 10, 0, None, None, None, 10, None, None, None, None, None, None, None, 0]
+    assert L.cuslines_chunk_async(*args) == 1
+    assert "null tracker" in _lib.last_error()
+    g = ctypes.c_void_p()
+    assert L.cuslines_chunk_graph_create(*args, ctypes.byref(g)) == 1
+    assert g.value is None
+    assert L.cuslines_graph_launch(None) == 1
+    assert L.cuslines_graph_destroy(None) == 0

@@ -24,12 +24,14 @@ def _oracle_by_chunks(w, seeds, chunk):
     return np.concatenate(pts), np.concatenate(lens)
 
 
-@pytest.mark.parametrize("launch_seeds,pipeline", [(0, 1), (0, 4), (250000, 1)])
+@pytest.mark.parametrize("launch_seeds,pipeline,graphs", [(0, 1, False), (0, 4, True), (0, 4, False),
+                                                           (250000, 1, False)])
 @pytest.mark.parametrize("kind", ["prob", "ptt", "csa"])
-def test_gpu_tracker_generate_streamlines(small_workloads, kind, launch_seeds, pipeline):
+def test_gpu_tracker_generate_streamlines(small_workloads, kind, launch_seeds, pipeline, graphs):
     """launch_seeds 0: one chunk per launch (the reference's protocol), one
     chunk after the other (pipeline 1) or 4 chunks in flight on forked
-    handles; 250000: all chunks in one launch with chunk-relative streams."""
+    handles as bounded chunks, eager or as HIP graphs; 250000: all chunks in
+    one launch with chunk-relative streams."""
     from gpustreamlines_amd import (BootDirectionGetter, GPUTracker, ProbDirectionGetter, PttDirectionGetter)
     from gpustreamlines_amd import workloads as W
 
@@ -43,9 +45,10 @@ def test_gpu_tracker_generate_streamlines(small_workloads, kind, launch_seeds, p
         dg, data = PttDirectionGetter(), small_workloads["prob"].dataf  # raw ODF; the getter prepares it
     else:
         dg, data = BootDirectionGetter("CSA", 1.0, **{k: v for k, v in w.boot.items()}, mfma=False), w.dataf
-    print(f"[api] {kind} launch_seeds={launch_seeds} pipeline={pipeline}", flush=True)
+    print(f"[api] {kind} launch_seeds={launch_seeds} pipeline={pipeline} graphs={graphs}", flush=True)
     with GPUTracker(dg, data, w.stop_map, p["tc_threshold"], w.vertices, w.edges, step_size=p["step_size"],
-                    chunk_size=chunk, launch_seeds=launch_seeds, chunk_pipeline=pipeline) as tr:
+                    chunk_size=chunk, launch_seeds=launch_seeds, chunk_pipeline=pipeline,
+                    chunk_graphs=graphs) as tr:
         out = tr.generate_streamlines(seeds)
     pts, lens = _oracle_by_chunks(w, seeds, chunk)
     assert np.array_equal(out._lengths, lens)
