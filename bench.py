@@ -267,7 +267,7 @@ def kernel_name(args):
     """The phase-2 kernel the plain pass launches (as rocprofv3 names it)."""
     if args.getter in ("csa", "opdt") and args.boot_mode == "mfma":
         return f"phase2_boot_batch_kernel<{MODEL_IDS[args.getter]}, {args.boot_occ or 4}>"
-    v = 4 if args.getter == "ptt" and args.ptt_stepping == "flat" else 0
+    v = 4 if args.stepping == "flat" else 0
     return f"phase2_queue_kernel<{MODEL_IDS[args.getter]}, {v}>"
 
 
@@ -311,8 +311,9 @@ def main():
                     help="end to end: seeds per launch (0: one launch per 25,000-seed chunk, the reference's protocol)")
     ap.add_argument("--chunk-pipeline", type=int, default=8,
                     help="end to end with one launch per chunk: chunks in flight at once (1: one after the other)")
-    ap.add_argument("--ptt-stepping", default="per-seed", choices=["flat", "per-seed"],
-                    help="PTT: one loop of sub-steps per seed slot (flat) or the reference's per-seed nesting")
+    ap.add_argument("--stepping", default="flat", choices=["flat", "per-seed"],
+                    help="phase 2 on the queue: one loop of steps per seed slot (flat) or the reference's "
+                         "per-seed nesting")
     ap.add_argument("--chunk-graphs", type=int, default=1, choices=[0, 1],
                     help="end to end with chunks in flight: each chunk as one HIP graph launch (0: eager launches)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -343,7 +344,7 @@ def main():
     boot = args.getter in ("csa", "opdt")
     if boot:
         dt.set_boot_mode(dt.BOOT_MFMA if args.boot_mode == "mfma" else dt.BOOT_EXACT, args.boot_occ)
-    dt.set_ptt_stepping(args.ptt_stepping == "flat")
+    dt.set_stepping(args.stepping == "flat")
     sd = dt.alloc(seeds.nbytes)
     sd.upload(seeds)
     cd = dt.alloc(4 * (n + 1))
@@ -462,7 +463,7 @@ def main():
             "wall_s": wall,
             "proposal": "plain",
             "boot_mode": args.boot_mode if boot else None,
-            "ptt_stepping": args.ptt_stepping if args.getter == "ptt" else None,
+            "stepping": args.stepping,
         },
         "roofline": {
             "bound": "hbm",

@@ -132,10 +132,10 @@ class DeviceTracker:
         """PTT phase 2: lanes per seed (8, 16, 32; 0 = shared 32-lane kernel)."""
         _lib.check(self.L.cuslines_set_lanes_per_seed(self.h, int(lanes)), "set_lanes_per_seed")
 
-    def set_ptt_stepping(self, flat: bool = False):
-        """Plain PTT phase 2 on the queue: one loop of sub-steps per seed
-        slot (True) or the per-seed nesting of the reference (False, default)."""
-        _lib.check(self.L.cuslines_set_ptt_stepping(self.h, int(bool(flat))), "set_ptt_stepping")
+    def set_stepping(self, flat: bool = True):
+        """Phase 2 on the queue: one loop of steps per seed slot (True,
+        default) or the per-seed nesting of the reference."""
+        _lib.check(self.L.cuslines_set_stepping(self.h, int(bool(flat))), "set_stepping")
 
     def set_ptt_layout(self, corner_blocks: bool = True):
         """PTT taps from 32-byte corner blocks (True) or 8 texel gathers."""

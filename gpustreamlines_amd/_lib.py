@@ -101,7 +101,7 @@ def load():
         "cuslines_set_launch": [vp, C.c_int, C.c_int, C.c_int],
         "cuslines_set_ptt_layout": [vp, C.c_int],
         "cuslines_set_lanes_per_seed": [vp, C.c_int],
-        "cuslines_set_ptt_stepping": [vp, C.c_int],
+        "cuslines_set_stepping": [vp, C.c_int],
         "cuslines_set_boot_mode": [vp, C.c_int],
         "cuslines_boot_counters": [vp, C.POINTER(C.c_int64), C.c_int],
         "cuslines_set_proposal": [vp, vp],

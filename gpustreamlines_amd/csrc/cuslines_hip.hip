@@ -2200,38 +2200,41 @@ __device__ __forceinline__ int stage_network(const DevBufs &B, float *lds, DevBu
     return MLP_LDS_FLOATS;
 }
 
-// Plain PTT phase 2 of one 32-lane slot with the stepping loop flattened
-// (the queue kernel's V == 4).  In track_seed a slot whose streamline pass,
-// direction or seed ends leaves its loops and waits at their exit until the
-// wave's other slot leaves them too; here one iteration of a single loop is
-// one sub-step of whichever pass the slot holds, and a slot whose pass ends
-// finishes it (last point, reversal, length) and starts its next pass,
-// direction or seed at once, so the wave's two slots keep sub-stepping
+// Phase 2 of one 32-lane slot with the stepping loop flattened (the queue
+// kernel's V == 4).  In track_seed a slot whose streamline pass, direction
+// or seed ends leaves its loops and waits at their exit until the wave's
+// other slot leaves them too; here one iteration of a single loop is one
+// step (PTT: sub-step) of whichever pass the slot holds, and a slot whose
+// pass ends finishes it (last point, reversal, length) and starts its next
+// pass, direction or seed at once, so the wave's two slots keep stepping
 // together.  Each seed's work, its order and its random draws are those of
-// track_seed (genStreamlinesMergePtt, ptt.cu / generate_streamlines_cuda.cu
-// :484-621), so the results are identical.  The seed's index, direction
-// count, next direction, slot offset, current slot and backward length (ints)
-// and its seed point and direction (floats) live in the slot's LDS (the
-// learned proposal's activation area, which the plain getter leaves unused).
-__device__ void ptt_flat_slot(const DevParams &P, const DevBufs &B, SlotLds &S, const float *__restrict__ seeds,
-                              int nseeds, int rng_chunk, const int *__restrict__ offsets,
-                              const int *__restrict__ dirs, int *__restrict__ sline_seed,
-                              int *__restrict__ sline_len, float *__restrict__ sline, const int *__restrict__ order,
-                              int *__restrict__ work) {
+// track_seed (genStreamlinesMerge{Prob,Boot,Ptt}, boot.cu:826-980,
+// generate_streamlines_cuda.cu:464-621), so the results are identical.
+// `st` (FLAT_STATE_FLOATS of the slot's LDS) holds the seed's index,
+// direction count, next direction, first slot, current slot and backward
+// length (ints), its seed point and the direction (floats).
+constexpr int FLAT_STATE_FLOATS = 16;
+
+template <int MODEL>
+__device__ void flat_slot(const DevParams &P, const DevBufs &B, const float *sv, SlotLds &S, float *st,
+                          const float *__restrict__ seeds, int nseeds, int rng_chunk, const int *__restrict__ offsets,
+                          const int *__restrict__ dirs, int *__restrict__ sline_seed, int *__restrict__ sline_len,
+                          float *__restrict__ sline, const int *__restrict__ order, int *__restrict__ work) {
+    constexpr bool PTT = MODEL == CUSLINES_PTT;
     const int l = lane32();
     const int xcd = blockIdx.x & (QUEUES - 1);
-    const int step_frac = P.step_frac, MAXL = P.max_sline_len;
+    const int step_frac = PTT ? P.step_frac : 1, MAXL = P.max_sline_len;
     const int nsub_max = MAXL * step_frac;
     const double sub = P.step_d / (double)step_frac;
     const size_t slot_f = (size_t)2 * MAXL * 3;
-    int *si = reinterpret_cast<int *>(S.ptt.act);  // s, ndir, next direction, offsets[s], slot, nb
-    float *sf = S.ptt.act + 8;                     // seed point [0..2], direction [3..5]
+    int *si = reinterpret_cast<int *>(st);  // s, ndir, next direction, offsets[s], slot, nb
+    float *sf = st + 8;                     // seed point [0..2], direction [3..5]
     if (l == 0) { si[1] = 0; si[2] = 0; }
     WAVE_SYNC();
     Philox rng;
     int qk = 0;
     bool have = false;
-    int pass = 0, i = 1, ri = 1, qi = 0;  // sub-step i = qi * step_frac + ri
+    int pass = 0, i = 1, ri = 1, qi = 0;  // step i = qi * step_frac + ri
     float px = 0.0f, py = 0.0f, pz = 0.0f, dx = 0.0f, dy = 0.0f, dz = 0.0f;
     float *sl = sline;
     for (;;) {
@@ -2270,14 +2273,16 @@ __device__ void ptt_flat_slot(const DevParams &P, const DevBufs &B, SlotLds &S, 
                     sf[3] = fx; sf[4] = fy; sf[5] = fz;
                 }
                 WAVE_SYNC();
-                if (!init_frame_ptt(P, B, S.ptt, rng, fx, fy, fz, sx, sy, sz)) {
-                    if (l == 0) {
-                        sline_len[off] = 1;
-                        cur[0] = sx; cur[1] = sy; cur[2] = sz;
+                if constexpr (PTT) {
+                    if (!init_frame_ptt(P, B, S.ptt, rng, fx, fy, fz, sx, sy, sz)) {
+                        if (l == 0) {
+                            sline_len[off] = 1;
+                            cur[0] = sx; cur[1] = sy; cur[2] = sz;
+                        }
+                        continue;
                     }
-                    continue;
                 }
-                // backward pass (-dir) on the frame at [0..8]
+                // backward pass (-dir; PTT: on the frame at [0..8])
                 pass = 0;
                 px = sx; py = sy; pz = sz;
                 dx = -fx; dy = -fy; dz = -fz;
@@ -2289,12 +2294,18 @@ __device__ void ptt_flat_slot(const DevParams &P, const DevBufs &B, SlotLds &S, 
             }
             if (!have) break;  // the queues are empty
         }
-        // one sub-step of the pass (tracker's loop body)
+        // one step of the pass (tracker's loop body)
         bool end = i >= nsub_max;
         if (!end) {
             float nx = 0.0f, ny = 0.0f, nz = 0.0f;
-            const int nd = get_direction_ptt<false>(P, B, S.ptt, S.ptt.frame + (pass ? 9 : 0), rng, dx, dy, dz, px, py,
-                                                    pz, &nx, &ny, &nz);
+            int nd;
+            if constexpr (MODEL == CUSLINES_PROB)
+                nd = get_direction_prob<false>(P, B, sv, S.prob, rng, dx, dy, dz, px, py, pz, &nx, &ny, &nz);
+            else if constexpr (PTT)
+                nd = get_direction_ptt<false>(P, B, S.ptt, S.ptt.frame + (pass ? 9 : 0), rng, dx, dy, dz, px, py, pz,
+                                              &nx, &ny, &nz);
+            else
+                nd = get_direction_boot<MODEL>(P, B, sv, S.boot, rng, 5, dx, dy, dz, px, py, pz, &nx, &ny, &nz);
             if (nd == 0) {
                 end = true;
             } else {
@@ -2317,7 +2328,7 @@ __device__ void ptt_flat_slot(const DevParams &P, const DevBufs &B, SlotLds &S, 
             }
         }
         if (!end) continue;
-        // the pass ends at sub-step i (tracker's tail)
+        // the pass ends at step i (tracker's tail)
         int nsteps = qi;
         if (ri != 0 && i < step_frac * (MAXL - 1)) {
             ++nsteps;
@@ -2339,7 +2350,7 @@ __device__ void ptt_flat_slot(const DevParams &P, const DevBufs &B, SlotLds &S, 
             }
             MEM_SYNC();
             if (l == 0) si[5] = nb;
-            // forward pass (+dir) on the frame at [9..17], from the seed point at index nb - 1
+            // forward pass (+dir; PTT: on the frame at [9..17]), from the seed point at index nb - 1
             pass = 1;
             px = sf[0]; py = sf[1]; pz = sf[2];
             dx = sf[3]; dy = sf[4]; dz = sf[5];
@@ -2388,10 +2399,14 @@ __global__ void __launch_bounds__(256) phase2_kernel(DevParams P, DevBufs B, con
 // index.
 // Plain PTT is held to 128 VGPRs (4 waves per SIMD): the few spills cost less
 // than the latency hiding gained (+8% at 200k seeds).  Bootstrap likewise
-// (4 waves per SIMD, which its 5 KB of LDS per slot allows: +13% CSA).
+// (4 waves per SIMD, which its 5 KB of LDS per slot allows: +13% CSA).  The
+// flat PTT loop (V == 4) needs about 170 VGPRs and at 128 spills inside its
+// loop, so it runs at 3 waves per SIMD (168 VGPRs, no spills).
 template <int MODEL, int V = 0>
 __global__ void __launch_bounds__((MODEL == CUSLINES_PTT && V == 1) ? 512 : 256,
-                                  (MODEL == CUSLINES_PTT && V != 2) ? 4 : (MODEL <= CUSLINES_PROB ? 4 : 1))
+                                  (MODEL == CUSLINES_PTT && V == 4)   ? 3
+                                  : (MODEL == CUSLINES_PTT && V != 2) ? 4
+                                                                      : (MODEL <= CUSLINES_PROB ? 4 : 1))
     phase2_queue_kernel(DevParams P, DevBufs B, const float *__restrict__ seeds,
                                                            int nseeds, int rng_chunk, const int *__restrict__ offsets,
                                                            const int *__restrict__ dirs, int *__restrict__ sline_seed,
@@ -2411,9 +2426,11 @@ __global__ void __launch_bounds__((MODEL == CUSLINES_PTT && V == 1) ? 512 : 256,
     const int slot = threadIdx.x >> 5;
     const int l = lane32();
     SlotLds S;
-    bind_slot<MODEL>(P, false, smem + svf + slot * slot_floats(MODEL, false, P.nv32, P.t32, P.hr32), S);
-    if constexpr (MODEL == CUSLINES_PTT && V == 4) {
-        ptt_flat_slot(P, BL, S, seeds, nseeds, rng_chunk, offsets, dirs, sline_seed, sline_len, sline, order, work);
+    const int stride = slot_floats(MODEL, false, P.nv32, P.t32, P.hr32) + (V == 4 ? FLAT_STATE_FLOATS : 0);
+    bind_slot<MODEL>(P, false, smem + svf + slot * stride, S);
+    if constexpr (V == 4) {
+        flat_slot<MODEL>(P, BL, sv, S, smem + svf + slot * stride + stride - FLAT_STATE_FLOATS, seeds, nseeds,
+                         rng_chunk, offsets, dirs, sline_seed, sline_len, sline, order, work);
         return;
     }
     // Workgroups are dispatched to the 8 XCDs round-robin.  Each XCD first
@@ -3333,7 +3350,7 @@ struct cuslines_tracker {
     int blocks_per_cu = 0;  // 0: occupancy maximum
     int num_cu = 1;
     int ptt_lanes = 0;      // PTT: lanes per seed (8, 16, 32; 0 = shared 32-lane kernel)
-    int ptt_flat = 0;       // plain PTT on the queue: one loop of sub-steps per slot (ptt_flat_slot)
+    int flat = 1;           // phase 2 on the queue: one loop of steps per seed slot (flat_slot)
     float *d_fprop = nullptr;            // learned proposal: feature propagators [24][9]
     float *d_vprop = nullptr;            // PTT vertex-phase propagators [24][9]
     uint16_t *d_mlp = nullptr;           // learned proposal (nullptr: plain getter)
@@ -3555,7 +3572,8 @@ static int launch_phase2_v(cuslines_tracker *t, const float *seeds, int n, int c
     const int nslots = threads / 32;
     const size_t vf = ((MODEL == CUSLINES_PROB) ? (size_t)((3 * d.nverts + 3) & ~3) : 0) +
                       ((V == 1) ? (size_t)MLP_LDS_FLOATS : 0);
-    const size_t lds = sizeof(float) * (vf + (size_t)nslots * slot_floats(MODEL, false, d.nv32, d.t32, d.hr32));
+    const size_t lds = sizeof(float) * (vf + (size_t)nslots * (slot_floats(MODEL, false, d.nv32, d.t32, d.hr32) +
+                                                             (V == 4 ? FLAT_STATE_FLOATS : 0)));
     if (lds > 160 * 1024) return fail(CUSLINES_EINVAL, "sphere too large for LDS");
     if (t->schedule == CUSLINES_SCHED_STATIC) {
         HIPC(hipFuncSetAttribute((const void *)phase2_kernel<MODEL, V>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -3649,9 +3667,9 @@ static int launch_phase2(cuslines_tracker *t, const float *seeds, int n, int chu
             return launch_phase2_v<MODEL, 3>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl, ws);
         if (t->ptt_lanes > 0 && t->schedule != CUSLINES_SCHED_STATIC)
             return launch_ptt_groups(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl);
-        if (t->ptt_flat && t->schedule != CUSLINES_SCHED_STATIC)
-            return launch_phase2_v<MODEL, 4>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl, ws);
     }
+    if (t->flat && t->schedule != CUSLINES_SCHED_STATIC)
+        return launch_phase2_v<MODEL, 4>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl, ws);
     return launch_phase2_v<MODEL, 0>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl, ws);
 }
 
@@ -4328,10 +4346,10 @@ int cuslines_set_boot_mode(cuslines_tracker *t, int mode) {
     return CUSLINES_OK;
 }
 
-int cuslines_set_ptt_stepping(cuslines_tracker *t, int flat) {
+int cuslines_set_stepping(cuslines_tracker *t, int flat) {
     if (!t) return fail(CUSLINES_EINVAL, "null tracker");
-    if (flat != 0 && flat != 1) return fail(CUSLINES_EINVAL, "PTT stepping must be 0 (per seed) or 1 (flat)");
-    t->ptt_flat = flat;
+    if (flat != 0 && flat != 1) return fail(CUSLINES_EINVAL, "stepping must be 0 (per seed) or 1 (flat)");
+    t->flat = flat;
     return CUSLINES_OK;
 }
 

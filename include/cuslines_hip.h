@@ -228,14 +228,16 @@ int cuslines_boot_counters(cuslines_tracker *t, int64_t *out2, int reset);
  * getters (which always use 32 lanes per seed). */
 int cuslines_set_lanes_per_seed(cuslines_tracker *t, int lanes);
 
-/* Plain PTT phase 2 on the work queue (lanes per seed 0): flat = 1 runs
- * each 32-lane seed slot as one loop of sub-steps in which a slot whose
- * streamline pass ends starts its next pass, direction or seed at once, so
- * the two slots of a wavefront keep stepping together; flat = 0 (default)
- * runs the reference's nesting (genStreamlinesMerge: per seed, per
- * direction, per pass, per step), where a slot whose loop ends waits for the
- * other slot's.  Same output either way.  No counterpart in the reference. */
-int cuslines_set_ptt_stepping(cuslines_tracker *t, int flat);
+/* Phase 2 on the work queue (the probabilistic getter, the exact bootstrap
+ * kernel and plain PTT with lanes per seed 0): flat = 1 (default) runs each
+ * 32-lane seed slot as one loop of steps (PTT: sub-steps) in which a slot
+ * whose streamline pass ends starts its next pass, direction or seed at
+ * once, so the two slots of a wavefront keep stepping together; flat = 0
+ * runs the reference's nesting (genStreamlinesMerge{Boot,Prob}_k: per seed,
+ * per direction, per pass, per step), where a slot whose loop ends waits
+ * for the other slot's.  Same output either way.  No counterpart in the
+ * reference. */
+int cuslines_set_stepping(cuslines_tracker *t, int flat);
 
 /* PTT probe taps: 1 (default) reads each tap's 8 texels as one 32-byte
  * corner block, an expanded copy of the ODF built once both

@@ -24,7 +24,7 @@ def _run_both(w, seeds, rng_chunk=0, launch=(0, 0, 0), blocks=True, lanes=0, fla
         dt.set_launch(*launch)
         dt.set_ptt_layout(blocks)
         dt.set_lanes_per_seed(lanes)
-        dt.set_ptt_stepping(flat)
+        dt.set_stepping(flat)
         dt.upload(w.dataf, w.stop_map, w.vertices, w.edges, boot=w.boot, ptt_lut=w.ptt_lut)
         g = dt.track(seeds, rng_chunk=rng_chunk, keep_dirs=True)
     finally:
@@ -59,15 +59,16 @@ def test_track_bit_exact(small_workloads, kind):
 
 
 @pytest.mark.parametrize("flat", [False, True])
-def test_ptt_stepping_modes_bit_exact(small_workloads, flat):
-    """Plain PTT on the queue with the reference's per-seed loop nesting and
-    with one flat loop of sub-steps per slot (a slot whose pass ends starts
-    its next pass, direction or seed at once); mixed seeds, so slots of a
-    wave end their streamlines at different times.  Chunk-relative streams
-    of 40 seeds."""
+@pytest.mark.parametrize("kind", ["ptt", "prob", "csa", "opdt"])
+def test_stepping_modes_bit_exact(small_workloads, kind, flat):
+    """Phase 2 on the queue with the reference's per-seed loop nesting and
+    with one flat loop of steps per slot (a slot whose pass ends starts its
+    next pass, direction or seed at once); mixed seeds, so slots of a wave
+    end their streamlines at different times.  Chunk-relative streams of 40
+    seeds; bootstrap on the exact kernel."""
     from gpustreamlines_amd import workloads as W
 
-    w = small_workloads["ptt"]
+    w = small_workloads[kind]
     seeds = W.seeds_for(w, 96, placement="mixed", seed=31)
     g, o = _run_both(w, seeds, rng_chunk=40, flat=flat)
     assert len(o["sline_len"]) > len(seeds)
