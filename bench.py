@@ -510,6 +510,10 @@ def main():
     else:
         res["cpu_baseline"] = None
     res["config"]["total_s"] = elapsed()
+    # the timed passes and the whole run against the 600 s a driver allows one bench command
+    timed_s = res["ms_per_step"] * res["steps"] / 1e3
+    res["consistency"] = {"timed_s": timed_s, "ms_per_step_x_steps_s": timed_s, "run_s": res["config"]["total_s"],
+                          "run_limit_s": 600.0, "fits_in_driver_run": res["config"]["total_s"] < 600.0}
     print(json.dumps(res), flush=True)
 
 

@@ -2185,14 +2185,14 @@ __device__ void track_seed(const DevParams &P, const DevBufs &B, const float *sv
     }
 }
 
-// The learned kernels (V == 1) copy the proposal network (IL_NPARAM bf16,
+// The learned kernels (V == 1, 5) copy the proposal network (IL_NPARAM bf16,
 // lane-interleaved) into the first MLP_LDS_FLOATS floats of `lds` and point
 // Bl.mlp there; every slot of the workgroup reads it from LDS.  Returns the
 // floats used.  The caller synchronises the workgroup before use.
 template <int V>
 __device__ __forceinline__ int stage_network(const DevBufs &B, float *lds, DevBufs &Bl) {
     Bl = B;
-    if (V != 1) return 0;
+    if (V != 1 && V != 5) return 0;
     const uint4 *src = reinterpret_cast<const uint4 *>(B.mlp);
     uint4 *dst = reinterpret_cast<uint4 *>(lds);
     for (int i = threadIdx.x; i < IL_NPARAM / 8; i += blockDim.x) dst[i] = src[i];
@@ -2215,7 +2215,7 @@ __device__ __forceinline__ int stage_network(const DevBufs &B, float *lds, DevBu
 // length (ints), its seed point and the direction (floats).
 constexpr int FLAT_STATE_FLOATS = 16;
 
-template <int MODEL>
+template <int MODEL, bool LEARNED = false>
 __device__ void flat_slot(const DevParams &P, const DevBufs &B, const float *sv, SlotLds &S, float *st,
                           const float *__restrict__ seeds, int nseeds, int rng_chunk, const int *__restrict__ offsets,
                           const int *__restrict__ dirs, int *__restrict__ sline_seed, int *__restrict__ sline_len,
@@ -2237,6 +2237,11 @@ __device__ void flat_slot(const DevParams &P, const DevBufs &B, const float *sv,
     int pass = 0, i = 1, ri = 1, qi = 0;  // step i = qi * step_frac + ri
     float px = 0.0f, py = 0.0f, pz = 0.0f, dx = 0.0f, dy = 0.0f, dz = 0.0f;
     float *sl = sline;
+    // learned proposal (tracker's per-pass state): proposal on, its tries,
+    // fallbacks to the plain sub-step, plain sub-steps and envelope misses
+    bool learned = LEARNED;
+    int ltries = 0, lfalls = 0, nplain = 0;
+    uint32_t lmiss = 0;
     for (;;) {
         if (!have) {
             // the next direction of this seed, or of the next seeds
@@ -2289,6 +2294,7 @@ __device__ void flat_slot(const DevParams &P, const DevBufs &B, const float *sv,
                 sl = cur;
                 if (l == 0) { sl[0] = px; sl[1] = py; sl[2] = pz; }
                 i = 1; qi = (step_frac == 1) ? 1 : 0; ri = (step_frac == 1) ? 0 : 1;
+                if (LEARNED) { learned = true; ltries = lfalls = nplain = 0; lmiss = 0; }
                 have = true;
                 break;
             }
@@ -2299,12 +2305,23 @@ __device__ void flat_slot(const DevParams &P, const DevBufs &B, const float *sv,
         if (!end) {
             float nx = 0.0f, ny = 0.0f, nz = 0.0f;
             int nd;
-            if constexpr (MODEL == CUSLINES_PROB)
+            if constexpr (MODEL == CUSLINES_PROB) {
                 nd = get_direction_prob<false>(P, B, sv, S.prob, rng, dx, dy, dz, px, py, pz, &nx, &ny, &nz);
-            else if constexpr (PTT)
+            } else if constexpr (PTT && LEARNED) {
+                int fell = 0;
+                nd = get_direction_ptt_learned(P, B, S.ptt, S.ptt.frame + (pass ? 9 : 0), rng, learned, dx, dy, dz, px,
+                                               py, pz, &nx, &ny, &nz, &fell, &lmiss);
+                if (learned) {
+                    ++ltries;
+                    lfalls += fell;
+                    if (ltries >= 16 && 4 * lfalls > ltries) learned = false;
+                } else {
+                    ++nplain;
+                }
+            } else if constexpr (PTT) {
                 nd = get_direction_ptt<false>(P, B, S.ptt, S.ptt.frame + (pass ? 9 : 0), rng, dx, dy, dz, px, py, pz,
                                               &nx, &ny, &nz);
-            else
+            } else
                 nd = get_direction_boot<MODEL>(P, B, sv, S.boot, rng, 5, dx, dy, dz, px, py, pz, &nx, &ny, &nz);
             if (nd == 0) {
                 end = true;
@@ -2329,6 +2346,12 @@ __device__ void flat_slot(const DevParams &P, const DevBufs &B, const float *sv,
         }
         if (!end) continue;
         // the pass ends at step i (tracker's tail)
+        if (LEARNED && B.stats && l == 0) {
+            atomicAdd(B.stats, (unsigned long long)(ltries - lfalls));
+            atomicAdd(B.stats + 1, (unsigned long long)lfalls);
+            atomicAdd(B.stats + 2, (unsigned long long)nplain);
+            atomicAdd(B.stats + 3, (unsigned long long)lmiss);
+        }
         int nsteps = qi;
         if (ri != 0 && i < step_frac * (MAXL - 1)) {
             ++nsteps;
@@ -2357,6 +2380,7 @@ __device__ void flat_slot(const DevParams &P, const DevBufs &B, const float *sv,
             sl = cur + 3 * (size_t)(nb - 1);
             if (l == 0) { sl[0] = px; sl[1] = py; sl[2] = pz; }
             i = 1; qi = (step_frac == 1) ? 1 : 0; ri = (step_frac == 1) ? 0 : 1;
+            if (LEARNED) { learned = true; ltries = lfalls = nplain = 0; lmiss = 0; }
             WAVE_SYNC();
         } else {
             if (l == 0) sline_len[off] = si[5] - 1 + nsteps;
@@ -2401,12 +2425,16 @@ __global__ void __launch_bounds__(256) phase2_kernel(DevParams P, DevBufs B, con
 // than the latency hiding gained (+8% at 200k seeds).  Bootstrap likewise
 // (4 waves per SIMD, which its 5 KB of LDS per slot allows: +13% CSA).  The
 // flat PTT loop (V == 4) needs about 170 VGPRs and at 128 spills inside its
-// loop, so it runs at 3 waves per SIMD (168 VGPRs, no spills).
+// loop, so it runs at 3 waves per SIMD (168 VGPRs, no spills); its learned
+// form (V == 5) likewise, in 768-thread workgroups (3 waves per SIMD in one
+// workgroup, so one staged copy of the network serves 24 slots).
 template <int MODEL, int V = 0>
-__global__ void __launch_bounds__((MODEL == CUSLINES_PTT && V == 1) ? 512 : 256,
-                                  (MODEL == CUSLINES_PTT && V == 4)   ? 3
-                                  : (MODEL == CUSLINES_PTT && V != 2) ? 4
-                                                                      : (MODEL <= CUSLINES_PROB ? 4 : 1))
+__global__ void __launch_bounds__((MODEL == CUSLINES_PTT && V == 1)   ? 512
+                                  : (MODEL == CUSLINES_PTT && V == 5) ? 768
+                                                                      : 256,
+                                  (MODEL == CUSLINES_PTT && (V == 4 || V == 5)) ? 3
+                                  : (MODEL == CUSLINES_PTT && V != 2)           ? 4
+                                                                                : (MODEL <= CUSLINES_PROB ? 4 : 1))
     phase2_queue_kernel(DevParams P, DevBufs B, const float *__restrict__ seeds,
                                                            int nseeds, int rng_chunk, const int *__restrict__ offsets,
                                                            const int *__restrict__ dirs, int *__restrict__ sline_seed,
@@ -2426,11 +2454,12 @@ __global__ void __launch_bounds__((MODEL == CUSLINES_PTT && V == 1) ? 512 : 256,
     const int slot = threadIdx.x >> 5;
     const int l = lane32();
     SlotLds S;
-    const int stride = slot_floats(MODEL, false, P.nv32, P.t32, P.hr32) + (V == 4 ? FLAT_STATE_FLOATS : 0);
+    const int stride =
+        slot_floats(MODEL, false, P.nv32, P.t32, P.hr32) + ((V == 4 || V == 5) ? FLAT_STATE_FLOATS : 0);
     bind_slot<MODEL>(P, false, smem + svf + slot * stride, S);
-    if constexpr (V == 4) {
-        flat_slot<MODEL>(P, BL, sv, S, smem + svf + slot * stride + stride - FLAT_STATE_FLOATS, seeds, nseeds,
-                         rng_chunk, offsets, dirs, sline_seed, sline_len, sline, order, work);
+    if constexpr (V == 4 || V == 5) {
+        flat_slot<MODEL, V == 5>(P, BL, sv, S, smem + svf + slot * stride + stride - FLAT_STATE_FLOATS, seeds,
+                                 nseeds, rng_chunk, offsets, dirs, sline_seed, sline_len, sline, order, work);
         return;
     }
     // Workgroups are dispatched to the 8 XCDs round-robin.  Each XCD first
@@ -3569,11 +3598,12 @@ static int launch_phase2_v(cuslines_tracker *t, const float *seeds, int n, int c
     // learned: 512-thread workgroups of the queue kernel share one staged
     // copy of the network (50 KB of LDS) among 16 slots
     if (V == 1 && t->threads <= 0 && t->schedule != CUSLINES_SCHED_STATIC) threads = 512;
+    if (V == 5 && t->threads <= 0) threads = 768;
     const int nslots = threads / 32;
     const size_t vf = ((MODEL == CUSLINES_PROB) ? (size_t)((3 * d.nverts + 3) & ~3) : 0) +
-                      ((V == 1) ? (size_t)MLP_LDS_FLOATS : 0);
+                      ((V == 1 || V == 5) ? (size_t)MLP_LDS_FLOATS : 0);
     const size_t lds = sizeof(float) * (vf + (size_t)nslots * (slot_floats(MODEL, false, d.nv32, d.t32, d.hr32) +
-                                                             (V == 4 ? FLAT_STATE_FLOATS : 0)));
+                                                             ((V == 4 || V == 5) ? FLAT_STATE_FLOATS : 0)));
     if (lds > 160 * 1024) return fail(CUSLINES_EINVAL, "sphere too large for LDS");
     if (t->schedule == CUSLINES_SCHED_STATIC) {
         HIPC(hipFuncSetAttribute((const void *)phase2_kernel<MODEL, V>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -3659,6 +3689,8 @@ static int launch_phase2(cuslines_tracker *t, const float *seeds, int n, int chu
             return launch_boot_batch<MODEL>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl, ws);
     }
     if constexpr (MODEL == CUSLINES_PTT) {
+        if (t->d_mlp && t->flat && t->schedule != CUSLINES_SCHED_STATIC)
+            return launch_phase2_v<MODEL, 5>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl, ws);
         if (t->d_mlp)
             return launch_phase2_v<MODEL, 1>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl, ws);
         if (t->rec)

@@ -21,7 +21,10 @@ def _tracker(w):
     return dt
 
 
-def test_learned_bit_exact_vs_oracle(small_workloads):
+@pytest.mark.parametrize("flat", [True, False])
+def test_learned_bit_exact_vs_oracle(small_workloads, flat):
+    """Both stepping modes: one loop of sub-steps per slot (the learned
+    proposal's per-pass state kept across the loop) and per-seed nesting."""
     from gpustreamlines_amd import learned as LP
     from gpustreamlines_amd import workloads as W
     from oracle.oracle import Oracle
@@ -29,9 +32,10 @@ def test_learned_bit_exact_vs_oracle(small_workloads):
     w = small_workloads["ptt"]
     seeds = W.seeds_for(w, 48, placement="mixed")
     wts = LP.to_bf16(LP.init_params(11))
-    print("[learned] bit-exact, 48 seeds", flush=True)
+    print(f"[learned] bit-exact, 48 seeds, flat={flat}", flush=True)
     dt = _tracker(w)
     try:
+        dt.set_stepping(flat)
         dt.set_proposal(wts)
         g = dt.track(seeds, keep_dirs=True)
         gstats = dt.proposal_counters()
