@@ -267,6 +267,9 @@ def kernel_name(args):
     """The phase-2 kernel the plain pass launches (as rocprofv3 names it)."""
     if args.getter in ("csa", "opdt") and args.boot_mode == "mfma":
         return f"phase2_boot_batch_kernel<{MODEL_IDS[args.getter]}, {args.boot_occ or 4}>"
+    if args.getter == "ptt" and args.ptt_lanes:
+        kind = "phase2_ptt_flat_kernel" if args.stepping == "flat" else "phase2_ptt_queue_kernel"
+        return f"{kind}<{args.ptt_lanes}>"
     v = 4 if args.stepping == "flat" else 0
     return f"phase2_queue_kernel<{MODEL_IDS[args.getter]}, {v}>"
 
@@ -311,6 +314,8 @@ def main():
                     help="end to end: seeds per launch (0: one launch per 25,000-seed chunk, the reference's protocol)")
     ap.add_argument("--chunk-pipeline", type=int, default=8,
                     help="end to end with one launch per chunk: chunks in flight at once (1: one after the other)")
+    ap.add_argument("--ptt-lanes", type=int, default=0, choices=[0, 8, 16, 32],
+                    help="PTT: lanes per seed (0: the shared 32-lane kernel)")
     ap.add_argument("--stepping", default="flat", choices=["flat", "per-seed"],
                     help="phase 2 on the queue: one loop of steps per seed slot (flat) or the reference's "
                          "per-seed nesting")
@@ -345,6 +350,8 @@ def main():
     if boot:
         dt.set_boot_mode(dt.BOOT_MFMA if args.boot_mode == "mfma" else dt.BOOT_EXACT, args.boot_occ)
     dt.set_stepping(args.stepping == "flat")
+    if args.getter == "ptt":
+        dt.set_lanes_per_seed(args.ptt_lanes)
     sd = dt.alloc(seeds.nbytes)
     sd.upload(seeds)
     cd = dt.alloc(4 * (n + 1))
@@ -464,6 +471,7 @@ def main():
             "proposal": "plain",
             "boot_mode": args.boot_mode if boot else None,
             "stepping": args.stepping,
+            "ptt_lanes": args.ptt_lanes if args.getter == "ptt" else None,
         },
         "roofline": {
             "bound": "hbm",

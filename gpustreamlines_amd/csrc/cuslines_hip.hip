@@ -526,20 +526,22 @@ __device__ __forceinline__ Tap tap_begin(const DevParams &P, const DevBufs &B, u
     return t;
 }
 
+// Linear filter of a corner block (texel order x fastest, then y, z).
+__device__ __forceinline__ float corner_blend(float a, float b, float g, const float4 &lo, const float4 &hi) {
+    const float a0 = 1.0f - a, b0 = 1.0f - b, g0 = 1.0f - g;
+    float acc = ((a0 * b0) * g0) * lo.x;
+    acc = __builtin_fmaf((a * b0) * g0, lo.y, acc);
+    acc = __builtin_fmaf((a0 * b) * g0, lo.z, acc);
+    acc = __builtin_fmaf((a * b) * g0, lo.w, acc);
+    acc = __builtin_fmaf((a0 * b0) * g, hi.x, acc);
+    acc = __builtin_fmaf((a * b0) * g, hi.y, acc);
+    acc = __builtin_fmaf((a0 * b) * g, hi.z, acc);
+    acc = __builtin_fmaf((a * b) * g, hi.w, acc);
+    return acc;
+}
+
 __device__ __forceinline__ float tap_end(const DevParams &P, const DevBufs &B, const Tap &t) {
-    if (t.fast) {
-        const float a = t.a, b = t.b, g = t.g;
-        const float a0 = 1.0f - a, b0 = 1.0f - b, g0 = 1.0f - g;
-        float acc = ((a0 * b0) * g0) * t.lo.x;
-        acc = __builtin_fmaf((a * b0) * g0, t.lo.y, acc);
-        acc = __builtin_fmaf((a0 * b) * g0, t.lo.z, acc);
-        acc = __builtin_fmaf((a * b) * g0, t.lo.w, acc);
-        acc = __builtin_fmaf((a0 * b0) * g, t.hi.x, acc);
-        acc = __builtin_fmaf((a * b0) * g, t.hi.y, acc);
-        acc = __builtin_fmaf((a0 * b) * g, t.hi.z, acc);
-        acc = __builtin_fmaf((a * b) * g, t.hi.w, acc);
-        return acc;
-    }
+    if (t.fast) return corner_blend(t.a, t.b, t.g, t.lo, t.hi);
     const int idx = (int)(t.lv & 0xffffu);
     return tiled_sample(P, B.data, t.xq, t.yq, t.zq, idx & (P.nx - 1), (idx >> P.log2x) & (P.ny - 1),
                         idx >> (P.log2x + P.log2y));
@@ -548,6 +550,30 @@ __device__ __forceinline__ float tap_end(const DevParams &P, const DevBufs &B, c
 __device__ __forceinline__ float tap_sample(const DevParams &P, const DevBufs &B, uint32_t lv, float px, float py,
                                             float pz) {
     return tap_end(P, B, tap_begin(P, B, lv, px, py, pz));
+}
+
+// A tap for loops that hold several in flight: tap_begin's weights and
+// block read, with a cell that has no block gathered at once (its value in
+// lo.x), so a tap in flight holds no more than its weights and block.
+struct TapB {
+    float a, b, g;
+    float4 lo, hi;
+    bool fast;
+};
+
+__device__ __forceinline__ TapB tapb_begin(const DevParams &P, const DevBufs &B, uint32_t lv, float px, float py,
+                                           float pz) {
+    const Tap t = tap_begin(P, B, lv, px, py, pz);
+    TapB r;
+    r.a = t.a; r.b = t.b; r.g = t.g;
+    r.lo = t.lo; r.hi = t.hi;
+    r.fast = t.fast;
+    if (!t.fast) r.lo.x = tap_end(P, B, t);
+    return r;
+}
+
+__device__ __forceinline__ float tapb_end(const TapB &t) {
+    return t.fast ? corner_blend(t.a, t.b, t.g, t.lo, t.hi) : t.lo.x;
 }
 
 __device__ __forceinline__ float interp4(const DevParams &P, const DevBufs &B, float px, float py,
@@ -1726,21 +1752,67 @@ __device__ int get_direction_ptt_g(const DevParams &P, const DevBufs &B, const P
     const float first_val = interp4(P, B, px, py, pz, fr[0], fr[1], fr[2]);
     // vertex phase: virtual lanes 0..23
     bool found = false;
-    for (int v = p; v < DISC_VERT_CNT; v += L) {
-        float pf[9];
-        if (INIT) {
-            VLane vl{key0, key1, sub0 + (uint32_t)v, S.rk[v], S.rc[v]};
-            probe_frame_init(fr, vl, dc, pf);
-            S.rk[v] = vl.k;
-            S.rc[v] = vl.c;
-        } else {
+    constexpr int NC = (DISC_VERT_CNT + L - 1) / L;  // curves per lane
+    if (INIT || NC > 2) {
+        for (int v = p; v < DISC_VERT_CNT; v += L) {
+            float pf[9];
+            if (INIT) {
+                VLane vl{key0, key1, sub0 + (uint32_t)v, S.rk[v], S.rc[v]};
+                probe_frame_init(fr, vl, dc, pf);
+                S.rk[v] = vl.k;
+                S.rc[v] = vl.c;
+            } else {
 #pragma unroll
-            for (int i = 0; i < 9; ++i) pf[i] = fr[i];
+                for (int i = 0; i < 9; ++i) pf[i] = fr[i];
+            }
+            const float sup = support_prop(P, B, first_val, px, py, pz, K.vprop + 9 * v, pf, __builtin_inff());
+            const bool ok = !(sup < P.min_support_f);
+            S.vert_pdf[v] = ok ? sup : 0.0f;
+            found |= ok;
         }
-        const float sup = support_prop(P, B, first_val, px, py, pz, K.vprop + 9 * v, pf, __builtin_inff());
-        const bool ok = !(sup < P.min_support_f);
-        S.vert_pdf[v] = ok ? sup : 0.0f;
-        found |= ok;
+    } else {
+        // this lane's NC <= 2 curves v = p, p + L probed in lockstep: each
+        // probe step issues every curve's LUT read, then every curve's block
+        // read, then blends, so the curves' load latencies overlap (with 3
+        // curves per lane the registers of three taps in flight spill, which
+        // costs more than it hides).  Each curve's arithmetic is
+        // support_prop's, in its order.
+        float pf[NC][9], qx[NC], qy[NC], qz[NC], sup[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) pf[c][i] = fr[i];
+            qx[c] = px; qy[c] = py; qz[c] = pz;
+            sup[c] = first_val;
+        }
+        for (int i = 0; i < P.probe_quality; ++i) {
+            uint32_t lv[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                if (p + c * L < DISC_VERT_CNT) {
+                    float direc[3];
+                    propagate_frame(K.vprop + 9 * (p + c * L), pf[c], direc);
+                    qx[c] += direc[0]; qy[c] += direc[1]; qz[c] += direc[2];
+                    lv[c] = B.lut[lut_index(P, pf[c][0], pf[c][1], pf[c][2])];
+                }
+            }
+            TapB t[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+                if (p + c * L < DISC_VERT_CNT) t[c] = tapb_begin(P, B, lv[c], qx[c], qy[c], qz[c]);
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+                if (p + c * L < DISC_VERT_CNT) sup[c] += tapb_end(t[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int v = p + c * L;
+            if (v < DISC_VERT_CNT) {
+                const bool ok = !(sup[c] < P.min_support_f);
+                S.vert_pdf[v] = ok ? sup[c] : 0.0f;
+                found |= ok;
+            }
+        }
     }
     if (G::ballot(found) == 0u) return 0;
     WAVE_SYNC();
@@ -1978,6 +2050,188 @@ __device__ void track_seed_ptt_g(const DevParams &P, const DevBufs &B, const Ptt
         }
         if (p == 0) sline_len[off] = nb - 1 + nf;
     }
+}
+
+// track_seed_ptt_g's work for a stream of seeds with the stepping loop
+// flattened, as flat_slot does for 32-lane slots: one loop iteration is one
+// sub-step of whichever pass the L-lane group holds, and a group whose pass
+// ends finishes it and starts its next pass, direction or seed at once, so
+// the 64/L seeds of a wavefront keep sub-stepping together.  Seeds come from
+// the XCD-ordered queues over the spatial order.  `st` (16 floats of the
+// group's LDS) holds the seed's index, direction count, next direction,
+// first slot, current slot, backward length and stream base (ints), its
+// seed point and the direction (floats).
+template <int L>
+__device__ void flat_group_ptt(const DevParams &P, const DevBufs &B, const PttConst &K, PttG &S, float *st,
+                               const float *__restrict__ seeds, int nseeds, int rng_chunk,
+                               const int *__restrict__ offsets, const int *__restrict__ dirs,
+                               int *__restrict__ sline_seed, int *__restrict__ sline_len, float *__restrict__ sline,
+                               const int *__restrict__ order, int *__restrict__ work) {
+    using G = Grp<L>;
+    const int p = G::lane();
+    const int xcd = blockIdx.x & (QUEUES - 1);
+    const int step_frac = P.step_frac, MAXL = P.max_sline_len;
+    const int nsub_max = MAXL * step_frac;
+    const double sub = P.step_d / (double)step_frac;
+    const size_t slot_f = (size_t)2 * MAXL * 3;
+    const uint32_t key0 = P.seed_lo, key1 = P.seed_hi;
+    int *si = reinterpret_cast<int *>(st);  // s, ndir, next direction, offsets[s], slot, nb, sub0
+    float *sf = st + 8;                     // seed point [0..2], direction [3..5]
+    if (p == 0) { si[1] = 0; si[2] = 0; }
+    WAVE_SYNC();
+    DrawCache dc;
+    dc.valid = 0;
+    int qk = 0;
+    bool have = false;
+    int pass = 0, i = 1, ri = 1, qi = 0;  // sub-step i = qi * step_frac + ri
+    float px = 0.0f, py = 0.0f, pz = 0.0f, dx = 0.0f, dy = 0.0f, dz = 0.0f;
+    float *sl = sline;
+    for (;;) {
+        if (!have) {
+            while (qk < QUEUES) {
+                const int di = si[2];
+                if (di >= si[1]) {
+                    const int qx = (xcd + qk) & (QUEUES - 1);
+                    const int lo = (int)((long long)nseeds * qx / QUEUES);
+                    const int hi = (int)((long long)nseeds * (qx + 1) / QUEUES);
+                    int q = 0;
+                    if (p == 0) q = atomicAdd(work + qx, 1);
+                    q = G::shfl(q, 0);
+                    if (lo + q >= hi) { ++qk; continue; }
+                    const int s = order ? order[lo + q] : lo + q;
+                    const int nd = (P.slot_cap > 0 && offsets[s + 1] > P.slot_cap) ? 0 : offsets[s + 1] - offsets[s];
+                    WAVE_SYNC();
+                    for (int v = p; v < 32; v += L) { S.rk[v] = 0u; S.rc[v] = -1; }
+                    if (p == 0) {
+                        si[0] = s; si[1] = nd; si[2] = 0; si[3] = offsets[s];
+                        si[6] = (int)(32u * (uint32_t)(s % rng_chunk) + 1u);
+                        sf[0] = seeds[3 * s]; sf[1] = seeds[3 * s + 1]; sf[2] = seeds[3 * s + 2];
+                    }
+                    dc.valid = 0;
+                    WAVE_SYNC();
+                    continue;
+                }
+                const int s = si[0], off = si[3] + di;
+                const uint32_t sub0 = (uint32_t)si[6];
+                const float sx = sf[0], sy = sf[1], sz = sf[2];
+                const int vi = dirs[(size_t)s * P.nverts + di];
+                const float fx = B.verts[3 * vi], fy = B.verts[3 * vi + 1], fz = B.verts[3 * vi + 2];
+                float *cur = sline + (size_t)off * slot_f;
+                WAVE_SYNC();
+                if (p == 0) {
+                    sline_seed[off] = s;
+                    si[2] = di + 1; si[4] = off;
+                    sf[3] = fx; sf[4] = fy; sf[5] = fz;
+                }
+                WAVE_SYNC();
+                if (!init_frame_ptt_g<L>(P, B, K, S, key0, key1, sub0, dc, fx, fy, fz, sx, sy, sz)) {
+                    if (p == 0) {
+                        sline_len[off] = 1;
+                        cur[0] = sx; cur[1] = sy; cur[2] = sz;
+                    }
+                    continue;
+                }
+                pass = 0;
+                px = sx; py = sy; pz = sz;
+                dx = -fx; dy = -fy; dz = -fz;
+                sl = cur;
+                if (p == 0) { sl[0] = px; sl[1] = py; sl[2] = pz; }
+                i = 1; qi = (step_frac == 1) ? 1 : 0; ri = (step_frac == 1) ? 0 : 1;
+                have = true;
+                break;
+            }
+            if (!have) break;  // the queues are empty
+        }
+        // one sub-step of the pass (tracker_ptt_g's loop body)
+        bool end = i >= nsub_max;
+        if (!end) {
+            float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+            const int nd = get_direction_ptt_g<L, false>(P, B, K, S, S.frame + (pass ? 9 : 0), key0, key1,
+                                                         (uint32_t)si[6], dc, dx, dy, dz, px, py, pz, &nx, &ny, &nz);
+            if (nd == 0) {
+                end = true;
+            } else {
+                dx = nx; dy = ny; dz = nz;
+                px = (float)__builtin_fma((double)(dx / 1.0f), sub, (double)px);
+                py = (float)__builtin_fma((double)(dy / 1.0f), sub, (double)py);
+                pz = (float)__builtin_fma((double)(dz / 1.0f), sub, (double)pz);
+                if (p == 0 && ri == 0) {
+                    float *o = sl + 3 * (size_t)qi;
+                    o[0] = px; o[1] = py; o[2] = pz;
+                }
+                const int tissue = check_point(P, B.stop, px, py, pz);
+                if (tissue == CUSLINES_ENDPOINT || tissue == CUSLINES_INVALIDPOINT || tissue == CUSLINES_OUTSIDEIMAGE) {
+                    end = true;
+                } else {
+                    ++i;
+                    if (++ri == step_frac) { ri = 0; ++qi; }
+                    end = i >= nsub_max;
+                }
+            }
+        }
+        if (!end) continue;
+        int nsteps = qi;
+        if (ri != 0 && i < step_frac * (MAXL - 1)) {
+            ++nsteps;
+            if (p == 0) {
+                float *o = sl + 3 * (size_t)nsteps;
+                o[0] = px; o[1] = py; o[2] = pz;
+            }
+        }
+        const int off = si[4];
+        float *cur = sline + (size_t)off * slot_f;
+        if (pass == 0) {
+            const int nb = nsteps;
+            MEM_SYNC();
+            for (int j = p; j < nb / 2; j += L) {
+                float *a = cur + 3 * (size_t)j, *b = cur + 3 * (size_t)(nb - 1 - j);
+                const float a0 = a[0], a1 = a[1], a2 = a[2];
+                a[0] = b[0]; a[1] = b[1]; a[2] = b[2];
+                b[0] = a0; b[1] = a1; b[2] = a2;
+            }
+            MEM_SYNC();
+            if (p == 0) si[5] = nb;
+            pass = 1;
+            px = sf[0]; py = sf[1]; pz = sf[2];
+            dx = sf[3]; dy = sf[4]; dz = sf[5];
+            sl = cur + 3 * (size_t)(nb - 1);
+            if (p == 0) { sl[0] = px; sl[1] = py; sl[2] = pz; }
+            i = 1; qi = (step_frac == 1) ? 1 : 0; ri = (step_frac == 1) ? 0 : 1;
+            WAVE_SYNC();
+        } else {
+            if (p == 0) sline_len[off] = si[5] - 1 + nsteps;
+            have = false;
+        }
+    }
+}
+
+// flat_group_ptt over workgroups of 256 threads (256/L seed groups).
+template <int L>
+__global__ void __launch_bounds__(256, 3) phase2_ptt_flat_kernel(DevParams P, DevBufs B,
+                                                                 const float *__restrict__ seeds, int nseeds,
+                                                                 int rng_chunk, const int *__restrict__ offsets,
+                                                                 const int *__restrict__ dirs,
+                                                                 int *__restrict__ sline_seed,
+                                                                 int *__restrict__ sline_len,
+                                                                 float *__restrict__ sline,
+                                                                 const int *__restrict__ order,
+                                                                 int *__restrict__ work) {
+    extern __shared__ float smem[];
+    PttConst K;
+    K.vprop = smem;
+    K.vk = smem + 24 * 9;
+    ptt_const_init(P, K);
+    __syncthreads();
+    const int g = threadIdx.x / L;
+    float *base = smem + PTTC_FLOATS + g * PTTG_FLOATS;
+    PttG S;
+    S.frame = base;
+    S.vert_pdf = base + 18;
+    S.face_cdf = base + 48;
+    S.rk = reinterpret_cast<uint32_t *>(base + 80);
+    S.rc = reinterpret_cast<int *>(base + 112);
+    flat_group_ptt<L>(P, B, K, S, base + 144, seeds, nseeds, rng_chunk, offsets, dirs, sline_seed, sline_len, sline,
+                      order, work);
 }
 
 // Persistent PTT phase 2 with L lanes per seed.
@@ -3488,12 +3742,18 @@ static int launch_phase1(cuslines_tracker *t, const float *seeds, int n, int chu
     return CUSLINES_OK;
 }
 
+static int spatial_order(cuslines_tracker *t, const float *seeds, int n, int **order, int *ws);
+
+// PTT with L lanes per seed: with flat stepping (t->flat, the default) the
+// flat grouped kernel over the XCD-ordered queues of the spatial seed order
+// (`ws` as launch_phase2_v), else the per-seed kernel over one counter.
 template <int L>
 static int launch_ptt_l(cuslines_tracker *t, int threads, const float *seeds, int n, int chunk, const int *offsets,
-                        const int *dirs, int *sseed, int *slen, float *sl) {
+                        const int *dirs, int *sseed, int *slen, float *sl, int *ws) {
     const DevParams &d = t->dp;
     const size_t lds = sizeof(float) * (size_t)(PTTC_FLOATS + (threads / L) * PTTG_FLOATS);
-    const void *kfn = (const void *)phase2_ptt_queue_kernel<L>;
+    const bool flat = t->flat != 0;
+    const void *kfn = flat ? (const void *)phase2_ptt_flat_kernel<L> : (const void *)phase2_ptt_queue_kernel<L>;
     HIPC(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int per_cu = 0;
     HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, threads, lds));
@@ -3503,20 +3763,29 @@ static int launch_ptt_l(cuslines_tracker *t, int threads, const float *seeds, in
     const long long need = ((long long)n * L + threads - 1) / threads;
     if (grid > need) grid = need;
     if (grid < 1) grid = 1;
+    int *order = nullptr;
+    if (flat)
+        if (int rc = spatial_order(t, seeds, n, &order, ws)) return rc;
     hipLaunchKernelGGL(zero_ints, dim3(1), dim3(QUEUES), 0, t->stream, t->d_work, QUEUES);
-    hipLaunchKernelGGL(phase2_ptt_queue_kernel<L>, dim3((unsigned)grid), dim3(threads), lds, t->stream, d, bufs_of(t),
-                       seeds, n, chunk, offsets, dirs, sseed, slen, sl, t->d_work);
-    HIPC(hipGetLastError());
+    if (flat)
+        hipLaunchKernelGGL(phase2_ptt_flat_kernel<L>, dim3((unsigned)grid), dim3(threads), lds, t->stream, d,
+                           bufs_of(t), seeds, n, chunk, offsets, dirs, sseed, slen, sl, order, t->d_work);
+    else
+        hipLaunchKernelGGL(phase2_ptt_queue_kernel<L>, dim3((unsigned)grid), dim3(threads), lds, t->stream, d,
+                           bufs_of(t), seeds, n, chunk, offsets, dirs, sseed, slen, sl, t->d_work);
+    const hipError_t e = hipGetLastError();
+    if (flat && !ws) (void)hipFreeAsync(order, t->stream);
+    if (e != hipSuccess) return fail(CUSLINES_EHIP, std::string("phase 2 (PTT groups): ") + hipGetErrorString(e));
     return CUSLINES_OK;
 }
 
 static int launch_ptt_groups(cuslines_tracker *t, const float *seeds, int n, int chunk, const int *offsets,
-                             const int *dirs, int *sseed, int *slen, float *sl) {
+                             const int *dirs, int *sseed, int *slen, float *sl, int *ws) {
     const int threads = t->threads > 0 ? t->threads : 256;
     switch (t->ptt_lanes) {
-        case 8: return launch_ptt_l<8>(t, threads, seeds, n, chunk, offsets, dirs, sseed, slen, sl);
-        case 16: return launch_ptt_l<16>(t, threads, seeds, n, chunk, offsets, dirs, sseed, slen, sl);
-        default: return launch_ptt_l<32>(t, threads, seeds, n, chunk, offsets, dirs, sseed, slen, sl);
+        case 8: return launch_ptt_l<8>(t, threads, seeds, n, chunk, offsets, dirs, sseed, slen, sl, ws);
+        case 16: return launch_ptt_l<16>(t, threads, seeds, n, chunk, offsets, dirs, sseed, slen, sl, ws);
+        default: return launch_ptt_l<32>(t, threads, seeds, n, chunk, offsets, dirs, sseed, slen, sl, ws);
     }
 }
 
@@ -3698,7 +3967,7 @@ static int launch_phase2(cuslines_tracker *t, const float *seeds, int n, int chu
         if (t->count_taps)
             return launch_phase2_v<MODEL, 3>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl, ws);
         if (t->ptt_lanes > 0 && t->schedule != CUSLINES_SCHED_STATIC)
-            return launch_ptt_groups(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl);
+            return launch_ptt_groups(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl, ws);
     }
     if (t->flat && t->schedule != CUSLINES_SCHED_STATIC)
         return launch_phase2_v<MODEL, 4>(t, seeds, n, chunk, offsets, dirs, sseed, slen, sl, ws);

@@ -97,15 +97,17 @@ def test_ptt_texel_gather_path_bit_exact(small_workloads):
     _assert_same(g, o)
 
 
+@pytest.mark.parametrize("flat", [True, False])
 @pytest.mark.parametrize("lanes", [8, 16, 32])
-def test_ptt_lanes_per_seed_bit_exact(small_workloads, lanes):
-    """PTT with 8, 16 or 32 lanes per seed (virtual random streams) equals the
-    shared 32-lane kernel and the oracle."""
+def test_ptt_lanes_per_seed_bit_exact(small_workloads, lanes, flat):
+    """PTT with 8, 16 or 32 lanes per seed (virtual random streams), flat
+    stepping over the spatial order or the per-seed nesting, equals the
+    oracle; chunk-relative streams of 24 seeds."""
     from gpustreamlines_amd import workloads as W
 
     w = small_workloads["ptt"]
-    seeds = W.seeds_for(w, 40, placement="mixed", seed=13)
-    g, o = _run_both(w, seeds, lanes=lanes)
+    seeds = W.seeds_for(w, 72, placement="mixed", seed=13)
+    g, o = _run_both(w, seeds, rng_chunk=24, lanes=lanes, flat=flat)
     _assert_same(g, o)
 
 
