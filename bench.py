@@ -308,8 +308,11 @@ def main():
                     help="MFMA bootstrap kernel variant: waves per SIMD (0 = library default)")
     ap.add_argument("--proposal", default="both", choices=["plain", "learned", "both"],
                     help="PTT: time the plain getter, the learned proposal, or both (value = plain)")
-    ap.add_argument("--end-to-end", action="store_true",
-                    help="also time GPUTracker.generate_streamlines over 25,000-seed chunks (host copies included)")
+    ap.add_argument("--end-to-end", dest="end_to_end", action="store_true", default=True,
+                    help="also time GPUTracker.generate_streamlines over 25,000-seed chunks (host copies included; "
+                         "the default)")
+    ap.add_argument("--no-end-to-end", dest="end_to_end", action="store_false",
+                    help="skip the end-to-end timing")
     ap.add_argument("--launch-seeds", type=int, default=250000,
                     help="end to end: seeds per launch (0: one launch per 25,000-seed chunk, the reference's protocol)")
     ap.add_argument("--chunk-pipeline", type=int, default=8,
@@ -404,6 +407,8 @@ def main():
     after = cpu_reserve + 10.0
     if learned_leg:
         after += 45.0 + LEARNED_PASSES * t_pass
+    if args.end_to_end:  # data preparation, a warm-up chunk and the chunks at about the kernel's rate
+        after += 25.0 + 1.4 * t_pass
     fit = int((args.budget - elapsed() - after) // max(t_pass, 1e-3))
     n_steps = max(min(args.steps, 3), min(args.steps, fit))
     times = []

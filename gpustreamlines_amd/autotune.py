@@ -7,8 +7,10 @@ tracker's stream, times it with HIP events, and applies the fastest.
 
 Tuned knobs: schedule (persistent queue or static), threads per block (seed
 slots per workgroup), resident blocks per CU (the occupancy maximum or
-fewer workgroups, each keeping its LDS slots) and, for PTT, lanes per seed
-(seeds per wavefront).  For the batched MFMA bootstrap kernel, whose
+fewer workgroups, each keeping its LDS slots), the stepping of the queue
+kernels (one flat loop of steps per seed slot, or the reference's per-seed
+nesting) and, for PTT, lanes per seed (seeds per wavefront).  For the
+batched MFMA bootstrap kernel, whose
 workgroup is 16 seed slots of LDS and one product tile column each, the
 knobs are its register/LDS tiling variant (2 waves per SIMD with 256
 registers, or 4 with 128 and two workgroups per CU) and the workgroups per
@@ -55,22 +57,25 @@ class _Events:
 def candidates(model_type: int, boot_mode: int = 0):
     if model_type == 3:  # PTT
         for lanes, threads in itertools.product((0, 16, 8), (256, 128)):
-            yield dict(schedule=0, threads=threads, blocks_per_cu=0, lanes=lanes)
-        yield dict(schedule=0, threads=256, blocks_per_cu=2, lanes=0)
-        yield dict(schedule=1, threads=256, blocks_per_cu=0, lanes=0)
+            yield dict(schedule=0, threads=threads, blocks_per_cu=0, lanes=lanes, flat=True)
+        yield dict(schedule=0, threads=256, blocks_per_cu=2, lanes=0, flat=True)
+        yield dict(schedule=0, threads=256, blocks_per_cu=0, lanes=0, flat=False)
+        yield dict(schedule=1, threads=256, blocks_per_cu=0, lanes=0, flat=False)
     elif model_type in (0, 1) and boot_mode == 1:  # batched MFMA bootstrap
         for occ, bpc in ((4, 0), (4, 1), (2, 0)):
-            yield dict(schedule=0, threads=0, blocks_per_cu=bpc, lanes=0, waves_per_simd=occ)
+            yield dict(schedule=0, threads=0, blocks_per_cu=bpc, lanes=0, waves_per_simd=occ, flat=True)
     else:
         threads = (128, 64) if model_type in (0, 1) else (256, 128, 64)
         for t in threads:
-            yield dict(schedule=0, threads=t, blocks_per_cu=0, lanes=0)
-        yield dict(schedule=0, threads=threads[0], blocks_per_cu=2, lanes=0)
-        yield dict(schedule=1, threads=threads[0], blocks_per_cu=0, lanes=0)
+            yield dict(schedule=0, threads=t, blocks_per_cu=0, lanes=0, flat=True)
+        yield dict(schedule=0, threads=threads[0], blocks_per_cu=2, lanes=0, flat=True)
+        yield dict(schedule=0, threads=threads[0], blocks_per_cu=0, lanes=0, flat=False)
+        yield dict(schedule=1, threads=threads[0], blocks_per_cu=0, lanes=0, flat=False)
 
 
 def apply(dt, cfg):
     dt.set_launch(cfg["schedule"], cfg["threads"], cfg["blocks_per_cu"])
+    dt.set_stepping(cfg.get("flat", True))
     if dt.params["model_type"] == 3:
         dt.set_lanes_per_seed(cfg["lanes"])
     if "waves_per_simd" in cfg:
