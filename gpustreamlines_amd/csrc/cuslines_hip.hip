@@ -2873,42 +2873,85 @@ __device__ int bb_peaks(const DevParams &P, const DevBufs &B, const float *odf, 
     const int nv = P.nverts;
     uint32_t pk = 0u;
     float mn = REAL_MAX, m = REAL_MIN;
-    // groups of 4 vertices per lane: the 4 adjacency rows are loaded before
-    // the first is used, so their latencies overlap
-    for (int r0 = 0, j0 = l; j0 < nv; r0 += 4, j0 += 128) {
-        uint4 qs[4];
+    if (thr > 0.0f) {
+        // With thr > 0 the thresholded test reduces to the raw values: a
+        // vertex whose value x is below thr (or NaN) reads as 0 and has no
+        // strictly smaller thresholded neighbour, so it is no peak; for x >=
+        // thr, a neighbour y is smaller after thresholding iff y < x and
+        // larger iff y > x.  So x is a peak iff x >= thr, the minimum of its
+        // raw neighbours is below x and their maximum is not above it
+        // (fminf / fmaxf pass over a NaN neighbour as the comparisons do).
+        // The largest peak value is kept on the way: max(x + c) over the
+        // peaks is max(x) + c, rounding being monotone.
+        float pmax = REAL_MIN;
+        for (int r0 = 0, j0 = l; j0 < nv; r0 += 4, j0 += 128) {
+            uint4 qs[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int j = min(j0 + 32 * u, nv - 1);  // past the end: re-read a valid row, result dropped
-            qs[u] = reinterpret_cast<const uint4 *>(B.nbr)[j];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int j = j0 + 32 * u;
-            if (j >= nv) break;
-            const uint32_t w[4] = {qs[u].x, qs[u].y, qs[u].z, qs[u].w};
-            float x = odf[j];
-            x = (x < thr) ? 0.0f : x;
-            mn = (mn < x) ? mn : x;
-            bool smaller = false, larger = false;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                float y = odf[(w[k >> 1] >> (16 * (k & 1))) & 0xffffu];
-                y = (y < thr) ? 0.0f : y;
-                smaller = smaller || (y < x);
-                larger = larger || (x < y);
+            for (int u = 0; u < 4; ++u) {
+                const int j = min(j0 + 32 * u, nv - 1);  // past the end: re-read a valid row, result dropped
+                qs[u] = reinterpret_cast<const uint4 *>(B.nbr)[j];
             }
-            if (smaller && !larger) pk |= 1u << (r0 + u);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = j0 + 32 * u;
+                if (j >= nv) break;
+                const uint32_t w[4] = {qs[u].x, qs[u].y, qs[u].z, qs[u].w};
+                const float x = odf[j];
+                const float tx = (x < thr) ? 0.0f : x;
+                mn = (mn < tx) ? mn : tx;
+                float lo = odf[w[0] & 0xffffu], hi = lo;
+#pragma unroll
+                for (int k = 1; k < 8; ++k) {
+                    const float y = odf[(w[k >> 1] >> (16 * (k & 1))) & 0xffffu];
+                    lo = fminf(lo, y);
+                    hi = fmaxf(hi, y);
+                }
+                if (x >= thr && lo < x && !(hi > x)) {
+                    pk |= 1u << (r0 + u);
+                    pmax = fmaxf(pmax, x);
+                }
+            }
         }
+        mn = min32(mn);
+        mn = (0.0f > mn) ? 0.0f : mn;
+        pmax = max32(pmax);
+        if (pmax != REAL_MIN) m = pmax + (-mn);
+    } else {
+        for (int r0 = 0, j0 = l; j0 < nv; r0 += 4, j0 += 128) {
+            uint4 qs[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = min(j0 + 32 * u, nv - 1);
+                qs[u] = reinterpret_cast<const uint4 *>(B.nbr)[j];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = j0 + 32 * u;
+                if (j >= nv) break;
+                const uint32_t w[4] = {qs[u].x, qs[u].y, qs[u].z, qs[u].w};
+                float x = odf[j];
+                x = (x < thr) ? 0.0f : x;
+                mn = (mn < x) ? mn : x;
+                bool smaller = false, larger = false;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    float y = odf[(w[k >> 1] >> (16 * (k & 1))) & 0xffffu];
+                    y = (y < thr) ? 0.0f : y;
+                    smaller = smaller || (y < x);
+                    larger = larger || (x < y);
+                }
+                if (smaller && !larger) pk |= 1u << (r0 + u);
+            }
+        }
+        mn = min32(mn);
+        mn = (0.0f > mn) ? 0.0f : mn;
+        for (int r = 0, j = l; j < nv; ++r, j += 32)
+            if ((pk >> r) & 1u) {
+                const float xm = odf[j] + (-mn);
+                m = (m > xm) ? m : xm;
+            }
+        m = max32(m);
     }
-    mn = min32(mn);
-    mn = (0.0f > mn) ? 0.0f : mn;
-    for (int r = 0, j = l; j < nv; ++r, j += 32)
-        if ((pk >> r) & 1u) {
-            const float xm = odf[j] + (-mn);
-            m = (m > xm) ? m : xm;
-        }
-    m = max32(m);
     const float comp = (float)((double)P.rel_peak * (double)m);
     int n = 0;
     for (int r = 0, j0 = 0; j0 < nv; ++r, j0 += 32) {
