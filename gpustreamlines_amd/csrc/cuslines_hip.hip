@@ -198,25 +198,37 @@ __device__ int peak_directions(const DevParams &P, const DevBufs &B, const float
     mn = (0.0f > mn) ? 0.0f : mn;
     if (B.nbr) {
         // Vertex-centred form of the edge marks below: a vertex is a peak when
-        // some neighbour is strictly smaller and none strictly larger.  Bit r
-        // of pk: vertex l + 32 r (nverts <= 1024).
+        // some neighbour is strictly smaller and none strictly larger, i.e.
+        // when the smallest neighbour is below it and the largest not above
+        // it (fminf / fmaxf pass over a NaN neighbour as the comparisons do).
+        // Bit r of pk: vertex l + 32 r (nverts <= 1024).  Groups of 4
+        // vertices per lane: the 4 adjacency rows are loaded before the first
+        // is used, so their latencies overlap.
         uint32_t pk = 0u;
         float m = REAL_MIN;
-        for (int r = 0, j = l; j < nv; ++r, j += 32) {
-            const uint4 q = reinterpret_cast<const uint4 *>(B.nbr)[j];
-            const uint32_t w[4] = {q.x, q.y, q.z, q.w};
-            const float x = odf[j];
-            bool smaller = false, larger = false;
+        for (int r0 = 0, j0 = l; j0 < nv; r0 += 4, j0 += 128) {
+            uint4 qs[4];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const float y = odf[(w[k >> 1] >> (16 * (k & 1))) & 0xffffu];
-                smaller = smaller || (y < x);
-                larger = larger || (x < y);
-            }
-            if (smaller && !larger) {
-                pk |= 1u << r;
-                const float xm = x + (-mn);
-                m = (m > xm) ? m : xm;
+            for (int u = 0; u < 4; ++u)
+                qs[u] = reinterpret_cast<const uint4 *>(B.nbr)[min(j0 + 32 * u, nv - 1)];  // past the end: dropped
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = j0 + 32 * u;
+                if (j >= nv) break;
+                const uint32_t w[4] = {qs[u].x, qs[u].y, qs[u].z, qs[u].w};
+                const float x = odf[j];
+                float lo = odf[w[0] & 0xffffu], hi = lo;
+#pragma unroll
+                for (int k = 1; k < 8; ++k) {
+                    const float y = odf[(w[k >> 1] >> (16 * (k & 1))) & 0xffffu];
+                    lo = fminf(lo, y);
+                    hi = fmaxf(hi, y);
+                }
+                if (lo < x && !(hi > x)) {
+                    pk |= 1u << (r0 + u);
+                    const float xm = x + (-mn);
+                    m = (m > xm) ? m : xm;
+                }
             }
         }
         m = max32(m);
