@@ -304,6 +304,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--boot-mode", default="mfma", choices=["mfma", "exact"],
                     help="bootstrap getters: batched MFMA kernel or the bit-exact per-seed kernel")
+    ap.add_argument("--boot-full-products", action="store_true",
+                    help="MFMA bootstrap: the full [H; R] product instead of H.d = U (U^T d) and R.d from it")
     ap.add_argument("--boot-occ", type=int, default=0, choices=[0, 2, 4],
                     help="MFMA bootstrap kernel variant: waves per SIMD (0 = library default)")
     ap.add_argument("--proposal", default="both", choices=["plain", "learned", "both"],
@@ -351,7 +353,8 @@ def main():
     dt.upload(w.dataf, w.stop_map, w.vertices, w.edges, boot=w.boot, ptt_lut=w.ptt_lut)
     boot = args.getter in ("csa", "opdt")
     if boot:
-        dt.set_boot_mode(dt.BOOT_MFMA if args.boot_mode == "mfma" else dt.BOOT_EXACT, args.boot_occ)
+        dt.set_boot_mode(dt.BOOT_MFMA if args.boot_mode == "mfma" else dt.BOOT_EXACT, args.boot_occ,
+                         full_products=args.boot_full_products)
     dt.set_stepping(args.stepping == "flat")
     if args.getter == "ptt":
         dt.set_lanes_per_seed(args.ptt_lanes)
@@ -475,6 +478,7 @@ def main():
             "wall_s": wall,
             "proposal": "plain",
             "boot_mode": args.boot_mode if boot else None,
+            "boot_lowrank": dt.boot_lowrank() if boot else None,
             "stepping": args.stepping,
             "ptt_lanes": args.ptt_lanes if args.getter == "ptt" else None,
         },

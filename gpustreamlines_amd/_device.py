@@ -170,13 +170,24 @@ class DeviceTracker:
 
     BOOT_EXACT, BOOT_MFMA = 0, 1
 
-    def set_boot_mode(self, mode: int, waves_per_simd: int = 0):
+    BOOT_FULL_PRODUCTS = 0x10000
+
+    def set_boot_mode(self, mode: int, waves_per_simd: int = 0, full_products: bool = False):
         """Bootstrap phase-2 kernel: BOOT_EXACT (bit-identical to the CPU
         oracle) or BOOT_MFMA (16 seeds in lockstep, matrix products on the
         matrix cores; equal within float rounding).  waves_per_simd (2 or 4,
-        0 = the default 4) picks the MFMA kernel's occupancy variant."""
-        _lib.check(self.L.cuslines_set_boot_mode(self.h, int(mode) | (int(waves_per_simd) << 8)), "set_boot_mode")
+        0 = the default 4) picks the MFMA kernel's occupancy variant;
+        full_products keeps the full [H; R] product where the low-rank form
+        (H = U U^T, R its leveraged centred residual matrix) would apply."""
+        m = int(mode) | (int(waves_per_simd) << 8) | (self.BOOT_FULL_PRODUCTS if full_products else 0)
+        _lib.check(self.L.cuslines_set_boot_mode(self.h, m), "set_boot_mode")
         self.boot_mode = int(mode)
+
+    def boot_lowrank(self) -> int:
+        """Rank of H used by the MFMA kernel's low-rank form (0: full products)."""
+        r = C.c_int32()
+        _lib.check(self.L.cuslines_boot_lowrank(self.h, C.byref(r)), "boot_lowrank")
+        return int(r.value)
 
     def boot_counters(self, reset: bool = True) -> np.ndarray:
         """MFMA bootstrap kernel: [slot attempts, workgroup rounds] since the

@@ -114,6 +114,7 @@ struct DevParams {
     uint32_t rec_thresh;  // record a sub-step when its hash is below this
     int rec_cap;          // capacity of the pair buffer
     int slot_cap;         // bounded chunk: phase 2 skips seeds whose streamlines end past this slot (0: none)
+    int rk16;             // batched bootstrap: rank of H rounded up to 16 when H.d = U (U^T d) (0: [H; R] product)
 };
 
 struct DevBufs {
@@ -124,6 +125,7 @@ struct DevBufs {
     const uint16_t *nbr;  // [nverts][8] neighbours, padded with the vertex itself (nullptr: edge path)
     const float *H, *R, *db, *dq, *smat;
     const float *hr_mat, *fit_mat, *odf_mat;  // batched bootstrap: [H;R], [dq|-db], smat, zero padded
+    const float *ut_mat, *u_mat, *invlev;      // batched bootstrap, low-rank form: U^T, U, 1/sqrt(1 - H_ii)
     const int *b0;
     const int *dwi_idx;
     const uint32_t *lut;  // PTT LUT entry: vertex | (direction rank << 16)
@@ -3125,11 +3127,34 @@ __global__ void __launch_bounds__(BB_THREADS, OCC)
         attempts += (unsigned long long)nact;
         ++rounds;
         // ---- phase B: H.d and R.d for the 16 slots --------------------------
-        bb_gemm<(2 * 320 / 16 + BB_WAVES - 1) / BB_WAVES>(B.hr_mat, hr16, hr16, 2 * hr16 / 16, smem + off_msk,
-                                                          smem + off_msk, hr16, smem + off_h, smem + off_r, hr16,
-                                                          2 * hr16, ss);
+        if (P.rk16 > 0) {
+            // low-rank form: z = U^T d into the R.d region, then H.d = U z;
+            // R.d follows in phase C
+            bb_gemm<1>(B.ut_mat, hr16, hr16, P.rk16 / 16, smem + off_msk, smem + off_msk, hr16, smem + off_r,
+                       smem + off_r, 1 << 30, P.rk16, ss);
+            __syncthreads();
+            bb_gemm<(320 / 16 + BB_WAVES - 1) / BB_WAVES>(B.u_mat, P.rk16, P.rk16, hr16 / 16, smem + off_r,
+                                                          smem + off_r, 1 << 30, smem + off_h, smem + off_h, 1 << 30,
+                                                          hr16, ss);
+        } else {
+            bb_gemm<(2 * 320 / 16 + BB_WAVES - 1) / BB_WAVES>(B.hr_mat, hr16, hr16, 2 * hr16 / 16, smem + off_msk,
+                                                              smem + off_msk, hr16, smem + off_h, smem + off_r, hr16,
+                                                              2 * hr16, ss);
+        }
         __syncthreads();
         // ---- phase C: residual bootstrap, normalisation, model transform -----
+        if (active && data && P.rk16 > 0) {
+            // R.d = v - mean(v), v = (d - H.d) / sqrt(1 - H_ii)
+            float part = 0.0f;
+            for (int j = l; j < hr; j += 32) {
+                const float v = (S.msk[j] - S.h[j]) * B.invlev[j];
+                S.r[j] = v;
+                part = part + v;
+            }
+            const float mean = tree_sum32(part) / (float)hr;
+            for (int j = l; j < hr; j += 32) S.r[j] = S.r[j] - mean;
+            WAVE_SYNC();
+        }
         if (active && data) {
             for (int j = l; j < hr; j += 32) {
                 const int src = (int)(rng.next() % (uint32_t)hr);
@@ -3674,6 +3699,9 @@ struct cuslines_tracker {
     uint16_t *d_nbr = nullptr;  // vertex adjacency, 8 per vertex (nullptr: a degree above 8 or > 1024 vertices)
     float *d_H = nullptr, *d_R = nullptr, *d_db = nullptr, *d_dq = nullptr, *d_smat = nullptr;
     float *d_hrm = nullptr, *d_fitm = nullptr, *d_odfm = nullptr;  // batched bootstrap matrices
+    float *d_ut = nullptr, *d_u = nullptr, *d_invlev = nullptr;   // low-rank form of [H; R] (rk16 > 0)
+    int rk16 = 0, hrank = 0;                                      // rank of H and rounded to 16 (0: not that form)
+    int boot_full = 0;                                            // batched bootstrap: always the [H; R] product
     int boot_mode = CUSLINES_BOOT_EXACT;
     int boot_occ = 4;  // batched bootstrap: waves per SIMD of the kernel variant
     int *d_b0 = nullptr, *d_dwi = nullptr;
@@ -3727,6 +3755,7 @@ static DevBufs bufs_of(const cuslines_tracker *t) {
     b.nbr = t->d_nbr;
     b.H = t->d_H; b.R = t->d_R; b.db = t->d_db; b.dq = t->d_dq; b.smat = t->d_smat;
     b.hr_mat = t->d_hrm; b.fit_mat = t->d_fitm; b.odf_mat = t->d_odfm;
+    b.ut_mat = t->d_ut; b.u_mat = t->d_u; b.invlev = t->d_invlev;
     b.b0 = t->d_b0;
     b.dwi_idx = t->d_dwi;
     b.lut = t->d_lut;
@@ -3967,7 +3996,8 @@ static bool boot_batch_fits(const DevParams &d) {
 template <int MODEL>
 static int launch_boot_batch(cuslines_tracker *t, const float *seeds, int n, int chunk, const int *offsets,
                              const int *dirs, int *sseed, int *slen, float *sl, int *ws) {
-    const DevParams &d = t->dp;
+    DevParams d = t->dp;
+    d.rk16 = t->boot_full ? 0 : t->rk16;  // H.d = U (U^T d) and R.d from it
     if (!boot_batch_fits(d))
         return fail(CUSLINES_EINVAL, "batched bootstrap needs hr <= 320, delta_nr <= 128 and nverts <= 768");
     const size_t lds = sizeof(float) * ((size_t)BB_SLOTS * (size_t)bb_slot_stride(d.nv32, d.t32, d.hr32) +
@@ -4316,6 +4346,7 @@ int cuslines_tracker_destroy(cuslines_tracker *t) {
     dfree(t->d_data); dfree(t->d_stop); dfree(t->d_verts); dfree(t->d_edges); dfree(t->d_nbr);
     dfree(t->d_H); dfree(t->d_R); dfree(t->d_db); dfree(t->d_dq); dfree(t->d_smat);
     dfree(t->d_hrm); dfree(t->d_fitm); dfree(t->d_odfm);
+    dfree(t->d_ut); dfree(t->d_u); dfree(t->d_invlev);
     dfree(t->d_b0); dfree(t->d_dwi); dfree(t->d_lut); dfree(t->d_work); dfree(t->d_blocks);
     dfree(t->d_fprop); dfree(t->d_vprop); dfree(t->d_mlp); dfree(t->d_stats);
     if (t->stream) (void)hipStreamDestroy(t->stream);
@@ -4353,6 +4384,99 @@ int cuslines_tracker_fork(cuslines_tracker *t, cuslines_tracker **out) {
 
 // PTT corner blocks from the transposed ODF and the LUT's direction ranks
 // (both must be present).  Without room for them taps gather their 8 texels.
+// The low-rank form of the bootstrap's [H; R] products.  H is the hat
+// matrix of the signal basis, a symmetric projection of rank r (the basis
+// size): H = U U^T with U (hr x r) orthonormal, and R = lcr(H) =
+// L (I - H) - 1 mu^T with L = diag(1 / sqrt(1 - H_ii)) (1 where H_ii >= 1)
+// and mu the column means of L (I - H) (boot_utils.py:86-94, DIPY's
+// lcr_matrix), so R.d = v - mean(v) with v = L (d - H.d).  U comes from
+// Gram-Schmidt (twice, in double) over H's columns; the form is used only
+// when U U^T and lcr(H) reproduce the given H and R to 1e-5.  Returns the
+// rank (0 when the matrices do not have this form), U row-major [hr][r] and
+// the leverage factors.
+static int hat_lowrank(const float *H, const float *R, int hr, std::vector<double> &U, std::vector<double> &invlev) {
+    std::vector<std::vector<double>> basis;
+    double hmax = 0.0;
+    for (size_t i = 0; i < (size_t)hr * hr; ++i) hmax = std::max(hmax, std::fabs((double)H[i]));
+    if (hmax == 0.0) return 0;
+    std::vector<double> v(hr);
+    for (int j = 0; j < hr && (int)basis.size() < 64; ++j) {
+        double n0 = 0.0;
+        for (int i = 0; i < hr; ++i) { v[i] = H[(size_t)i * hr + j]; n0 += v[i] * v[i]; }
+        if (n0 == 0.0) continue;
+        for (int pass = 0; pass < 2; ++pass)
+            for (const auto &u : basis) {
+                double dot = 0.0;
+                for (int i = 0; i < hr; ++i) dot += u[i] * v[i];
+                for (int i = 0; i < hr; ++i) v[i] -= dot * u[i];
+            }
+        double n = 0.0;
+        for (int i = 0; i < hr; ++i) n += v[i] * v[i];
+        if (n > 1e-6 * n0 && n > 1e-12) {
+            const double inv = 1.0 / std::sqrt(n);
+            for (int i = 0; i < hr; ++i) v[i] *= inv;
+            basis.push_back(v);
+        }
+    }
+    const int r = (int)basis.size();
+    if (r == 0 || r >= hr || r > 64) return 0;
+    // A column added late can have a small residual, which the float32
+    // rounding of H then dominates: refine the basis twice by U <- orth(H U)
+    // (H is a projection, so H U lies in its range and is close to
+    // orthonormal; Gram-Schmidt on it amplifies nothing)
+    for (int it = 0; it < 2; ++it) {
+        std::vector<std::vector<double>> nb;
+        for (int k = 0; k < r; ++k) {
+            for (int i = 0; i < hr; ++i) {
+                double acc = 0.0;
+                for (int j = 0; j < hr; ++j) acc += (double)H[(size_t)i * hr + j] * basis[k][j];
+                v[i] = acc;
+            }
+            for (int pass = 0; pass < 2; ++pass)
+                for (const auto &u : nb) {
+                    double dot = 0.0;
+                    for (int i = 0; i < hr; ++i) dot += u[i] * v[i];
+                    for (int i = 0; i < hr; ++i) v[i] -= dot * u[i];
+                }
+            double n = 0.0;
+            for (int i = 0; i < hr; ++i) n += v[i] * v[i];
+            if (n < 0.25) return 0;  // H U lost a direction: not a projection
+            const double inv = 1.0 / std::sqrt(n);
+            for (int i = 0; i < hr; ++i) v[i] *= inv;
+            nb.push_back(v);
+        }
+        basis.swap(nb);
+    }
+    // H = U U^T ?
+    for (int i = 0; i < hr; ++i)
+        for (int j = 0; j < hr; ++j) {
+            double acc = 0.0;
+            for (int k = 0; k < r; ++k) acc += basis[k][i] * basis[k][j];
+            if (std::fabs(acc - (double)H[(size_t)i * hr + j]) > 1e-5 * hmax) return 0;
+        }
+    // R = lcr(H) ?
+    invlev.assign(hr, 1.0);
+    for (int i = 0; i < hr; ++i) {
+        const double hii = H[(size_t)i * hr + i];
+        if (hii < 1.0) invlev[i] = 1.0 / std::sqrt(1.0 - hii);
+    }
+    double rmax = 1.0;
+    for (size_t i = 0; i < (size_t)hr * hr; ++i) rmax = std::max(rmax, std::fabs((double)R[i]));
+    for (int j = 0; j < hr; ++j) {
+        double mu = 0.0;
+        for (int i = 0; i < hr; ++i) mu += ((i == j ? 1.0 : 0.0) - (double)H[(size_t)i * hr + j]) * invlev[i];
+        mu /= hr;
+        for (int i = 0; i < hr; ++i) {
+            const double lij = ((i == j ? 1.0 : 0.0) - (double)H[(size_t)i * hr + j]) * invlev[i] - mu;
+            if (std::fabs(lij - (double)R[(size_t)i * hr + j]) > 1e-5 * rmax) return 0;
+        }
+    }
+    U.assign((size_t)hr * r, 0.0);
+    for (int i = 0; i < hr; ++i)
+        for (int k = 0; k < r; ++k) U[(size_t)i * r + k] = basis[k][i];
+    return r;
+}
+
 static int build_blocks(cuslines_tracker *t) {
     const DevParams &d = t->dp;
     dfree(t->d_blocks);
@@ -4554,9 +4678,24 @@ int cuslines_set_boot_matrices(cuslines_tracker *t, const float *H, const float 
             }
         for (int i = 0; i < nv; ++i)
             for (int j = 0; j < dn; ++j) odf[(size_t)i * dn16 + j] = sampling_matrix[(size_t)i * dn + j];
-        float **bdst[3] = {&t->d_hrm, &t->d_fitm, &t->d_odfm};
-        const std::vector<float> *bsrc[3] = {&hrm, &fit, &odf};
-        for (int k = 0; k < 3; ++k) {
+        // low-rank form: U^T [rk16][hr16], U [hr16][rk16], 1/lev [hr16], zero padded
+        std::vector<double> U, invlev;
+        const int r = hat_lowrank(H, R, hr, U, invlev);
+        const int rk16 = (r + 15) & ~15;
+        std::vector<float> ut((size_t)std::max(rk16, 16) * hr16, 0.0f), um((size_t)hr16 * std::max(rk16, 16), 0.0f),
+            il((size_t)hr16, 0.0f);
+        for (int i = 0; i < hr && r > 0; ++i) {
+            for (int k = 0; k < r; ++k) {
+                ut[(size_t)k * hr16 + i] = (float)U[(size_t)i * r + k];
+                um[(size_t)i * rk16 + k] = (float)U[(size_t)i * r + k];
+            }
+            il[i] = (float)invlev[i];
+        }
+        t->rk16 = r > 0 ? rk16 : 0;
+        t->hrank = r;
+        float **bdst[6] = {&t->d_hrm, &t->d_fitm, &t->d_odfm, &t->d_ut, &t->d_u, &t->d_invlev};
+        const std::vector<float> *bsrc[6] = {&hrm, &fit, &odf, &ut, &um, &il};
+        for (int k = 0; k < 6; ++k) {
             dfree(*bdst[k]);
             *bdst[k] = nullptr;
             HIPC(hipMalloc(bdst[k], bsrc[k]->size() * sizeof(float)));
@@ -4695,10 +4834,18 @@ int cuslines_set_boot_mode(cuslines_tracker *t, int mode) {
         return fail(CUSLINES_EINVAL, "the bootstrap mode is for OPDT and CSA trackers");
     if ((mode & 0xff) != CUSLINES_BOOT_EXACT && (mode & 0xff) != CUSLINES_BOOT_MFMA)
         return fail(CUSLINES_EINVAL, "unknown mode");
-    const int occ = mode >> 8;
+    const int occ = (mode >> 8) & 0xff;
     if (occ != 0 && occ != 2 && occ != 4) return fail(CUSLINES_EINVAL, "waves per SIMD must be 2 or 4");
+    if ((mode & ~(0xffff | CUSLINES_BOOT_FULL_PRODUCTS)) != 0) return fail(CUSLINES_EINVAL, "unknown mode flag");
     t->boot_mode = mode & 0xff;
     t->boot_occ = occ ? occ : 4;
+    t->boot_full = (mode & CUSLINES_BOOT_FULL_PRODUCTS) ? 1 : 0;
+    return CUSLINES_OK;
+}
+
+int cuslines_boot_lowrank(cuslines_tracker *t, int32_t *rank) {
+    if (!t || !rank) return fail(CUSLINES_EINVAL, "null argument");
+    *rank = (t->has_boot && !t->boot_full) ? t->hrank : 0;
     return CUSLINES_OK;
 }
 

@@ -198,9 +198,11 @@ int cuslines_set_launch(cuslines_tracker *t, int schedule, int threads, int bloc
 enum {
     CUSLINES_BOOT_EXACT = 0,  /* one seed per 32 lanes, the reference's lane-strided
                                  dot products: bit-identical to the CPU oracle    */
-    CUSLINES_BOOT_MFMA = 1    /* 16 seeds per workgroup in lockstep, the refit and
+    CUSLINES_BOOT_MFMA = 1,   /* 16 seeds per workgroup in lockstep, the refit and
                                  ODF projection as fp32 MFMA products batched over
                                  the 16 seeds: equal within float rounding         */
+    CUSLINES_BOOT_FULL_PRODUCTS = 0x10000  /* flag: MFMA mode with the full
+                                 [H; R] product even where the low-rank form holds */
 };
 
 /* Select the bootstrap phase-2 kernel (default CUSLINES_BOOT_EXACT).  Replaces
@@ -211,8 +213,19 @@ enum {
  * and nverts <= 768 (phase 2 returns CUSLINES_EINVAL otherwise) and the
  * queue schedule (CUSLINES_SCHED_STATIC keeps the exact kernel).  Bits 8..15
  * of `mode` choose the MFMA kernel's occupancy, 2 or 4 (default) waves per
- * SIMD (the autotuner's knob). */
+ * SIMD (the autotuner's knob).
+ * When H is a projection H = U U^T of rank r <= 64 and R its leveraged centred
+ * residual matrix (boot_utils.py:86-94: R = L (I - H) - column means, L =
+ * diag(1 / sqrt(1 - H_ii))), as cuslines_set_boot_matrices checks to 1e-5, the
+ * MFMA kernel computes H.d = U (U^T d) and R.d = v - mean(v), v = L (d - H.d),
+ * instead of the full [H; R] product (fewer products, another rounding);
+ * CUSLINES_BOOT_FULL_PRODUCTS in `mode` keeps the full product. */
 int cuslines_set_boot_mode(cuslines_tracker *t, int mode);
+
+/* The rank r of H that the MFMA bootstrap kernel's low-rank form uses, or 0
+ * when H and R do not have that form or CUSLINES_BOOT_FULL_PRODUCTS is set.
+ * No counterpart in the reference. */
+int cuslines_boot_lowrank(cuslines_tracker *t, int32_t *rank);
 
 /* Work counters of the MFMA bootstrap kernel since the last reset: out[0]
  * getter attempts of all slots (each one refit and ODF projection, i.e. one

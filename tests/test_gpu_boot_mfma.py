@@ -13,14 +13,39 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def _track(w, seeds, mode, rng_chunk=0, occ=0):
+def _track(w, seeds, mode, rng_chunk=0, occ=0, full=False):
     from gpustreamlines_amd._device import DeviceTracker
 
     dt = DeviceTracker(w.params, device=0)
     try:
         dt.upload(w.dataf, w.stop_map, w.vertices, w.edges, boot=w.boot)
-        dt.set_boot_mode(mode, occ)
+        dt.set_boot_mode(mode, occ, full_products=full)
         return dt.track(seeds, rng_chunk=rng_chunk)
+    finally:
+        dt.close()
+
+
+@pytest.mark.parametrize("kind", ["csa", "opdt"])
+def test_lowrank_form_detected(kind, small_workloads):
+    """The getter matrices (H the hat matrix of the order-6 basis, R its
+    leveraged centred residuals) have the low-rank form, of rank 28; a
+    perturbed R does not, and the full-products flag turns it off."""
+    from gpustreamlines_amd._device import DeviceTracker
+
+    w = small_workloads[kind]
+    dt = DeviceTracker(w.params, device=0)
+    try:
+        dt.upload(w.dataf, w.stop_map, w.vertices, w.edges, boot=w.boot)
+        dt.set_boot_mode(1)
+        assert dt.boot_lowrank() == 28
+        dt.set_boot_mode(1, full_products=True)
+        assert dt.boot_lowrank() == 0
+        bad = dict(w.boot)
+        bad["R"] = np.array(bad["R"], dtype=np.float32, copy=True)
+        bad["R"][3, 5] += 1e-3
+        dt.upload(w.dataf, w.stop_map, w.vertices, w.edges, boot=bad)
+        dt.set_boot_mode(1)
+        assert dt.boot_lowrank() == 0
     finally:
         dt.close()
 
@@ -53,15 +78,15 @@ def _compare(a, b, shape, min_identical):
 
 
 @pytest.mark.parametrize("kind", ["csa", "opdt"])
-@pytest.mark.parametrize("occ", [2, 4])
-def test_mfma_boot_matches_oracle_small(kind, occ, small_workloads):
+@pytest.mark.parametrize("occ,full", [(2, False), (4, False), (4, True)])
+def test_mfma_boot_matches_oracle_small(kind, occ, full, small_workloads):
     from gpustreamlines_amd import workloads as W
     from oracle.oracle import Oracle
 
     w = small_workloads[kind]
     seeds = W.seeds_for(w, 400, placement="mixed", seed=13)
-    print(f"[boot-mfma] {kind} occ={occ} small seeds={len(seeds)}", flush=True)
-    g = _track(w, seeds, 1, rng_chunk=0, occ=occ)
+    print(f"[boot-mfma] {kind} occ={occ} full={full} small seeds={len(seeds)}", flush=True)
+    g = _track(w, seeds, 1, rng_chunk=0, occ=occ, full=full)
     o = Oracle(w.params, w.dataf, w.stop_map, w.vertices, w.edges, boot=w.boot).track(seeds, rng_chunk=0)
     from conftest import packed_points
 
