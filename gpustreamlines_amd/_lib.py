@@ -105,6 +105,7 @@ def load():
         "cuslines_set_boot_mode": [vp, C.c_int],
         "cuslines_boot_counters": [vp, C.POINTER(C.c_int64), C.c_int],
         "cuslines_boot_lowrank": [vp, C.POINTER(i32)],
+        "cuslines_hat_lowrank": [vp, vp, i32, C.POINTER(i32)],
         "cuslines_set_proposal": [vp, vp],
         "cuslines_set_recording": [vp, vp, i32, vp, C.c_float],
         "cuslines_proposal_stats": [vp, C.POINTER(C.c_int64), C.c_int],

@@ -4843,6 +4843,14 @@ int cuslines_set_boot_mode(cuslines_tracker *t, int mode) {
     return CUSLINES_OK;
 }
 
+int cuslines_hat_lowrank(const float *H, const float *R, int32_t hr, int32_t *rank) {
+    if (!H || !R || !rank) return fail(CUSLINES_EINVAL, "null argument");
+    if (hr < 1) return fail(CUSLINES_EINVAL, "hr must be positive");
+    std::vector<double> U, invlev;
+    *rank = hat_lowrank(H, R, hr, U, invlev);
+    return CUSLINES_OK;
+}
+
 int cuslines_boot_lowrank(cuslines_tracker *t, int32_t *rank) {
     if (!t || !rank) return fail(CUSLINES_EINVAL, "null argument");
     *rank = (t->has_boot && !t->boot_full) ? t->hrank : 0;

@@ -227,6 +227,12 @@ int cuslines_set_boot_mode(cuslines_tracker *t, int mode);
  * No counterpart in the reference. */
 int cuslines_boot_lowrank(cuslines_tracker *t, int32_t *rank);
 
+/* The same check on host arrays, without a tracker or a GPU: *rank = r when
+ * H (hr x hr, row-major) is U U^T of rank r <= 64 < hr and R its leveraged
+ * centred residual matrix, to 1e-5; else 0.  No counterpart in the
+ * reference. */
+int cuslines_hat_lowrank(const float *H, const float *R, int32_t hr, int32_t *rank);
+
 /* Work counters of the MFMA bootstrap kernel since the last reset: out[0]
  * getter attempts of all slots (each one refit and ODF projection, i.e. one
  * column of the batched products), out[1] lockstep rounds of all workgroups
