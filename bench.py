@@ -263,10 +263,11 @@ def measure_end_to_end(args, w, seeds, kernel_steps):
             "same_steps_as_kernel_pass": steps == kernel_steps}
 
 
-def kernel_name(args):
+def kernel_name(args, lowrank=False):
     """The phase-2 kernel the plain pass launches (as rocprofv3 names it)."""
     if args.getter in ("csa", "opdt") and args.boot_mode == "mfma":
-        return f"phase2_boot_batch_kernel<{MODEL_IDS[args.getter]}, {args.boot_occ or 4}>"
+        lr = "true" if lowrank else "false"
+        return f"phase2_boot_batch_kernel<{MODEL_IDS[args.getter]}, {args.boot_occ or 4}, {lr}>"
     if args.getter == "ptt" and args.ptt_lanes:
         kind = "phase2_ptt_flat_kernel" if args.stepping == "flat" else "phase2_ptt_queue_kernel"
         return f"{kind}<{args.ptt_lanes}>"
@@ -489,7 +490,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": kernel_name(args),
+            "kernel": kernel_name(args, boot and dt.boot_lowrank() > 0),
             "alg_bytes_per_step": alg_launch / max(steps_per_pass, 1),
             "alg_bytes_per_launch": alg_launch,
         },

@@ -3085,7 +3085,9 @@ __device__ void bb_end_pass(const DevParams &P, const DevBufs &B, BbSlot &st, Ph
 // workgroup per CU; 4: 128 VGPRs with spills and two workgroups per CU, the
 // default: 99.8 against 71.9 M steps/s for CSA at 1M seeds, as the two
 // workgroups' lockstep phases overlap).
-template <int MODEL, int OCC>
+// LR: the low-rank form of the [H; R] product (P.rk16 > 0), a separate
+// instantiation so that each form's kernel carries only its own registers.
+template <int MODEL, int OCC, bool LR>
 __global__ void __launch_bounds__(BB_THREADS, OCC)
     phase2_boot_batch_kernel(DevParams P, DevBufs B, const float *__restrict__ seeds, int nseeds, int rng_chunk,
                              const int *__restrict__ offsets, const int *__restrict__ dirs,
@@ -3127,7 +3129,7 @@ __global__ void __launch_bounds__(BB_THREADS, OCC)
         attempts += (unsigned long long)nact;
         ++rounds;
         // ---- phase B: H.d and R.d for the 16 slots --------------------------
-        if (P.rk16 > 0) {
+        if (LR) {
             // low-rank form: z = U^T d into the R.d region, then H.d = U z;
             // R.d follows in phase C
             bb_gemm<1>(B.ut_mat, hr16, hr16, P.rk16 / 16, smem + off_msk, smem + off_msk, hr16, smem + off_r,
@@ -3143,7 +3145,7 @@ __global__ void __launch_bounds__(BB_THREADS, OCC)
         }
         __syncthreads();
         // ---- phase C: residual bootstrap, normalisation, model transform -----
-        if (active && data && P.rk16 > 0) {
+        if (LR && active && data) {
             // R.d = v - mean(v), v = (d - H.d) / sqrt(1 - H_ii)
             float part = 0.0f;
             for (int j = l; j < hr; j += 32) {
@@ -4003,8 +4005,12 @@ static int launch_boot_batch(cuslines_tracker *t, const float *seeds, int n, int
     const size_t lds = sizeof(float) * ((size_t)BB_SLOTS * (size_t)bb_slot_stride(d.nv32, d.t32, d.hr32) +
                                         (size_t)BB_WAVES * 16);
     if (lds > 160 * 1024) return fail(CUSLINES_EINVAL, "batched bootstrap slots too large for LDS");
-    const void *kfn = (t->boot_occ == 4) ? (const void *)phase2_boot_batch_kernel<MODEL, 4>
-                                         : (const void *)phase2_boot_batch_kernel<MODEL, 2>;
+    const bool lr = d.rk16 > 0;
+    const void *kfn = (t->boot_occ == 4)
+                          ? (lr ? (const void *)phase2_boot_batch_kernel<MODEL, 4, true>
+                                : (const void *)phase2_boot_batch_kernel<MODEL, 4, false>)
+                          : (lr ? (const void *)phase2_boot_batch_kernel<MODEL, 2, true>
+                                : (const void *)phase2_boot_batch_kernel<MODEL, 2, false>);
     HIPC(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int per_cu = 0;
     HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, BB_THREADS, lds));
@@ -4021,14 +4027,20 @@ static int launch_boot_batch(cuslines_tracker *t, const float *seeds, int n, int
     int *order = nullptr;
     if (int rc = spatial_order(t, seeds, n, &order, ws)) return rc;
     hipLaunchKernelGGL(zero_ints, dim3(1), dim3(QUEUES), 0, t->stream, t->d_work, QUEUES);
-    if (t->boot_occ == 4)
-        hipLaunchKernelGGL((phase2_boot_batch_kernel<MODEL, 4>), dim3((unsigned)grid), dim3(BB_THREADS), lds,
-                           t->stream, d, bufs_of(t), seeds, n, chunk, offsets, dirs, sseed, slen, sl, order,
-                           t->d_work);
+    const DevBufs bb = bufs_of(t);
+    const dim3 g((unsigned)grid), b(BB_THREADS);
+    if (t->boot_occ == 4 && lr)
+        hipLaunchKernelGGL((phase2_boot_batch_kernel<MODEL, 4, true>), g, b, lds, t->stream, d, bb, seeds, n, chunk,
+                           offsets, dirs, sseed, slen, sl, order, t->d_work);
+    else if (t->boot_occ == 4)
+        hipLaunchKernelGGL((phase2_boot_batch_kernel<MODEL, 4, false>), g, b, lds, t->stream, d, bb, seeds, n, chunk,
+                           offsets, dirs, sseed, slen, sl, order, t->d_work);
+    else if (lr)
+        hipLaunchKernelGGL((phase2_boot_batch_kernel<MODEL, 2, true>), g, b, lds, t->stream, d, bb, seeds, n, chunk,
+                           offsets, dirs, sseed, slen, sl, order, t->d_work);
     else
-        hipLaunchKernelGGL((phase2_boot_batch_kernel<MODEL, 2>), dim3((unsigned)grid), dim3(BB_THREADS), lds,
-                           t->stream, d, bufs_of(t), seeds, n, chunk, offsets, dirs, sseed, slen, sl, order,
-                           t->d_work);
+        hipLaunchKernelGGL((phase2_boot_batch_kernel<MODEL, 2, false>), g, b, lds, t->stream, d, bb, seeds, n, chunk,
+                           offsets, dirs, sseed, slen, sl, order, t->d_work);
     hipError_t e = hipGetLastError();
     if (!ws) (void)hipFreeAsync(order, t->stream);
     if (e != hipSuccess) return fail(CUSLINES_EHIP, std::string("phase 2 (batched): ") + hipGetErrorString(e));
