@@ -19,9 +19,11 @@ from its published closed form:
 * hat matrix ``H = U U^T`` (thin SVD of ``B``) and leveraged centred
   residuals ``R = (I-H)/sqrt(1-diag(H)) - column means`` (Chung et al. 2006).
 
-Exact agreement with DIPY's numerics is not claimed (parity unpinned: DIPY is
-absent); the kernels consume whatever matrices are passed, and both the GPU
-path and the CPU oracle receive the same arrays.
+DIPY is absent, so agreement with its numbers is not tested directly; the
+tests pin each matrix against the closed form above (harmonics of orders 0-4
+written out, the fits' inverse relation to the basis, the leveraged
+residuals).  The kernels consume whatever matrices are passed, and both the
+GPU path and the CPU oracle receive the same arrays.
 """
 
 from __future__ import annotations
