@@ -387,7 +387,9 @@ class ChunkBuffers:
         t = self.tot.download(np.zeros(3, dtype=np.int64))
         return int(t[0]), int(t[1]), int(t[2])
 
-    def collect(self, n: int, with_seeds: bool = True) -> dict:
+    def collect(self, n: int, with_seeds: bool = True, points_out=None) -> dict:
+        """The chunk's results on the host; the points go into
+        ``points_out`` (float32 [npts, 3], C-contiguous) when given."""
         nsl, tracked, npts = self.totals()
         out = {"totals": (nsl, tracked, npts)}
         out["offsets"] = self.od.download(np.zeros(n + 1, dtype=np.int32))
@@ -395,7 +397,11 @@ class ChunkBuffers:
         if with_seeds:
             out["sline_seed"] = self.ssd.download(np.zeros(tracked, dtype=np.int32))
         out["point_offsets"] = self.pod.download(np.zeros(tracked + 1, dtype=np.int64))
-        out["points"] = self.ptd.download(np.zeros((npts, 3), dtype=np.float32))
+        if points_out is None:
+            points_out = np.zeros((npts, 3), dtype=np.float32)
+        elif points_out.shape != (npts, 3) or points_out.dtype != np.float32:
+            raise ValueError(f"points_out must be float32 [{npts}, 3]")
+        out["points"] = self.ptd.download(points_out)
         return out
 
     def drop_graph(self):

@@ -48,6 +48,77 @@ class StreamlineArray:
             yield self[i]
 
 
+class PackedSink:
+    """One growing host buffer that the chunks of a tracking run download
+    their packed points into, in chunk order, so the packed output needs no
+    concatenation pass over all points.
+
+    ``take(npts, nseeds)`` appends room for a chunk's points and returns it
+    (a view the caller fills at once: a later ``take`` may move the buffer,
+    copying what was filled); ``add(pts, lens, nseeds)`` records the chunk's
+    lengths, and copies ``pts`` in when it is not the view ``take`` gave.
+    With ``total_seeds`` known, the first growth sizes the buffer for the
+    whole run from the points per seed seen so far (15 % headroom); pages
+    past the points written are never touched, so the headroom costs
+    address space, not memory."""
+
+    HEADROOM = 1.15
+
+    def __init__(self, total_seeds: int = 0):
+        self.total_seeds = int(total_seeds)
+        self._buf = np.empty((0, 3), dtype=np.float32)
+        self._n = 0
+        self._seeds = 0
+        self._open = []
+        self._lens = []
+        self.grows = 0
+
+    def _grow(self, need):
+        cap = max(need, len(self._buf) + len(self._buf) // 2)
+        if 0 < self._seeds < self.total_seeds:
+            cap = max(need, int(need / self._seeds * self.total_seeds * self.HEADROOM) + 1)
+        new = np.empty((cap, 3), dtype=np.float32)
+        new[: self._n] = self._buf[: self._n]
+        self._buf = new
+        self.grows += 1
+
+    def take(self, npts: int, nseeds: int = 0) -> np.ndarray:
+        npts = int(npts)
+        self._seeds += int(nseeds)
+        need = self._n + npts
+        if need > len(self._buf):
+            self._grow(need)
+        view = self._buf[self._n: need]
+        self._n = need
+        self._open.append(view)
+        return view
+
+    def add(self, pts, lens, nseeds: int = 0):
+        lens = np.asarray(lens, dtype=np.int64)
+        if self._open:
+            if pts is not self._open[0]:
+                raise RuntimeError("add the points of the chunks in the order they were taken")
+            self._open.pop(0)
+        else:
+            pts = np.asarray(pts, dtype=np.float32).reshape(-1, 3)
+            self.take(len(pts), nseeds)[:] = pts
+            self._open.pop()
+        self._lens.append(lens)
+
+    def result(self, minlen=0, maxlen=np.inf) -> StreamlineArray:
+        """The packed streamlines whose point count lies in [minlen, maxlen]."""
+        if self._open:
+            raise RuntimeError("points taken but never added")
+        lens = np.concatenate(self._lens) if self._lens else np.zeros(0, dtype=np.int64)
+        if int(lens.sum()) != self._n:
+            raise RuntimeError("lengths do not cover the points")
+        data = self._buf[: self._n]
+        keep = (lens >= minlen) & (lens <= maxlen)
+        if not keep.all():
+            data, lens = data[np.repeat(keep, lens)], lens[keep]
+        return StreamlineArray.from_packed(data, lens)
+
+
 def make_array_sequence(iterable, buffer_size=0):
     try:
         from nibabel.streamlines.array_sequence import ArraySequence

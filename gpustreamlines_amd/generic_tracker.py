@@ -12,7 +12,7 @@ import logging
 
 import numpy as np
 
-from .array_sequence import StreamlineArray
+from .array_sequence import PackedSink, StreamlineArray
 
 logger = logging.getLogger("GPUStreamlines")
 
@@ -52,10 +52,12 @@ class GenericTracker:
         nchunks = (seeds.shape[0] + global_chunk_sz - 1) // global_chunk_sz
         return global_chunk_sz, nchunks
 
-    def _packed_chunks(self, seeds):
-        """Per seed chunk: (points [n, 3] float32 voxel coordinates, lengths
-        int64) of the streamlines kept by the min/max length filter, packed as
-        the device compaction returns them."""
+    def _raw_chunks(self, seeds, sink=None):
+        """Per launch and GPU: (points [n, 3] float32 voxel coordinates,
+        lengths int64, seeds) as the device compaction returns them, before
+        the min/max length filter.  With ``sink`` (a PackedSink) the points
+        are downloaded into it and the caller adds each part to it before
+        asking for the next."""
         seeds = np.asarray(seeds)
         gsz, nchunks = self._divide_chunks(seeds)
         sp = self.seed_propagator
@@ -72,36 +74,56 @@ class GenericTracker:
                 # handles that share the device data
                 from .propagate_seeds import ChunkPipeline
 
-                pipe = ChunkPipeline(self.device_trackers[0], depth=depth, minlen=sp.minlen, maxlen=sp.maxlen,
+                pipe = ChunkPipeline(self.device_trackers[0], depth=depth,
                                      graphs=bool(getattr(self, "chunk_graphs", True)))
                 try:
-                    for pts, lens in pipe.run(seeds, gsz):
-                        if len(lens):
-                            yield pts, lens
+                    for c, (pts, lens) in enumerate(pipe.run(seeds, gsz, sink=sink)):
+                        yield pts, lens, len(seeds[c * gsz:(c + 1) * gsz])
                 finally:
                     pipe.close()
                 return
+        from .propagate_seeds import seed_shares
+
         sp.rng_chunk = gsz if per > 1 else None
-        for idx in range(0, nchunks, per):
-            sp.propagate(seeds[idx * gsz: (idx + per) * gsz])
-            for ii in range(sp.ngpus):
-                pts, lens = sp.slines[ii], sp.sline_lens[ii]
-                if lens is None or len(lens) == 0:
-                    continue
-                keep = (lens >= sp.minlen) & (lens <= sp.maxlen)
-                if keep.all():
-                    yield pts, lens.astype(np.int64)
-                elif keep.any():
-                    # points are packed in streamline order: one mask over all of them
-                    yield pts[np.repeat(keep, lens)], lens[keep].astype(np.int64)
+        direct = sink is not None and hasattr(sp, "sink")
+        if direct:
+            sp.sink = sink
+        try:
+            for idx in range(0, nchunks, per):
+                part = seeds[idx * gsz: (idx + per) * gsz]
+                sp.propagate(part)
+                shares = seed_shares(len(part), sp.ngpus)
+                for ii in range(sp.ngpus):
+                    lens = sp.sline_lens[ii]
+                    if lens is not None:
+                        yield sp.slines[ii], lens.astype(np.int64), shares[ii][1]
+        finally:
+            if direct:
+                sp.sink = None
+
+    def _packed_chunks(self, seeds):
+        """Per seed chunk: (points [n, 3] float32 voxel coordinates, lengths
+        int64) of the streamlines kept by the min/max length filter, packed as
+        the device compaction returns them."""
+        sp = self.seed_propagator
+        for pts, lens, _ in self._raw_chunks(seeds):
+            keep = (lens >= sp.minlen) & (lens <= sp.maxlen)
+            if keep.all():
+                if len(lens):
+                    yield pts, lens
+            elif keep.any():
+                # points are packed in streamline order: one mask over all of them
+                yield pts[np.repeat(keep, lens)], lens[keep]
 
     def generate_streamlines(self, seeds) -> StreamlineArray:
-        """All kept streamlines of ``seeds`` (voxel coordinates), packed."""
-        parts = list(self._packed_chunks(seeds))
-        if not parts:
-            return StreamlineArray()
-        return StreamlineArray.from_packed(np.concatenate([p for p, _ in parts]),
-                                           np.concatenate([n for _, n in parts]))
+        """All kept streamlines of ``seeds`` (voxel coordinates), packed: the
+        chunks download into one buffer (PackedSink), filtered once at the
+        end."""
+        sp = self.seed_propagator
+        sink = PackedSink(len(seeds))
+        for pts, lens, n in self._raw_chunks(seeds, sink):
+            sink.add(pts, lens, n)
+        return sink.result(sp.minlen, sp.maxlen)
 
     def generate_sft(self, seeds, ref_img):
         """StatefulTractogram in voxel space (generic_tracker.py:75-93), built

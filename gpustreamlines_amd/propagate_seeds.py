@@ -58,6 +58,9 @@ class SeedBatchPropagator:
         # chunk per launch, as the reference); set when one launch carries
         # several chunks (GenericTracker, one GPU).
         self.rng_chunk = None
+        # When set (an array_sequence.PackedSink), each GPU's points are
+        # downloaded straight into it instead of into arrays of their own.
+        self.sink = None
         self._reset_buffers()
 
     def _reset_buffers(self):
@@ -143,7 +146,10 @@ class SeedBatchPropagator:
                 self.slineSeed_d[ii].download(seeds_of)
             npts = int(lens.astype(np.int64).sum())
             offs = np.zeros(nsl + 1, dtype=np.int64)
-            pts = np.zeros((npts, 3), dtype=np.float32)
+            if self.sink is not None:
+                pts = self.sink.take(npts, seed_shares(self.nseeds, self.ngpus)[ii][1])
+            else:
+                pts = np.zeros((npts, 3), dtype=np.float32)
             if nsl:
                 pod = dt.alloc(8 * (nsl + 1))
                 ptd = dt.alloc(12 * max(npts, 1))
@@ -242,19 +248,22 @@ class ChunkPipeline:
         cb.enqueue(n, n, graph=self.graphs)
         self.pending[k] = seeds
 
-    def _collect(self, k):
+    def _collect(self, k, sink=None):
         seeds = self.pending[k]
         self.pending[k] = None
         n = len(seeds)
         cb = self.cbs[k]
-        nsl, tracked, _ = cb.totals()
+        nsl, tracked, npts = cb.totals()
         if tracked < nsl:  # the slots were too few: the same chunk with more
             self.reruns += 1
             cb = self._cb(k, n, nsl)
             cb.sd.upload(seeds)
             cb.enqueue(n, n, graph=False)
-            nsl, tracked, _ = cb.totals()
+            nsl, tracked, npts = cb.totals()
             assert tracked == nsl
+        if sink is not None:
+            r = cb.collect(n, with_seeds=False, points_out=sink.take(npts, n))
+            return r["points"], r["sline_len"].astype(np.int64)
         if not nsl:
             return np.zeros((0, 3), np.float32), np.zeros(0, np.int64)
         r = cb.collect(n, with_seeds=False)
@@ -264,10 +273,13 @@ class ChunkPipeline:
             pts, lens = pts[np.repeat(keep, lens)], lens[keep]
         return pts, lens
 
-    def run(self, seeds, chunk: int, slots_per_seed: float = 8):
+    def run(self, seeds, chunk: int, slots_per_seed: float = 8, sink=None):
         """(points, lengths) of every chunk of ``chunk`` seeds, in order.
         ``slots_per_seed`` sizes the slots reserved up front (streamlines per
-        seed are about 1 to 8; a chunk with more runs again)."""
+        seed are about 1 to 8; a chunk with more runs again).  With ``sink``
+        (an array_sequence.PackedSink) each chunk's points are downloaded
+        into it and yielded unfiltered, for the caller to ``sink.add`` before
+        the next chunk; the sink's result applies the length filter."""
         seeds = np.ascontiguousarray(seeds, dtype=np.float32).reshape(-1, 3)
         nchunks = -(-len(seeds) // chunk) if len(seeds) else 0
         depth = len(self.handles)
@@ -276,12 +288,12 @@ class ChunkPipeline:
         for c in range(nchunks):
             k = c % depth
             if self.pending[k] is not None:
-                yield self._collect(k)
+                yield self._collect(k, sink)
                 order.pop(0)
             self._launch(k, seeds[c * chunk:(c + 1) * chunk], hint)
             order.append(k)
         for k in order:
-            yield self._collect(k)
+            yield self._collect(k, sink)
 
     def close(self):
         for k, cb in enumerate(self.cbs):

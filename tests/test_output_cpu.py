@@ -75,6 +75,86 @@ def test_generate_streamlines_chunks_and_filter(minlen, maxlen):
         np.testing.assert_array_equal(a, b)
 
 
+class SinkPropagator(FakePropagator):
+    """The same streamlines, its points written into the caller's
+    PackedSink when one is set (as the device propagator downloads them)."""
+
+    def __init__(self, **kw):
+        super().__init__(**kw)
+        self.sink = None
+
+    def propagate(self, seeds):
+        super().propagate(seeds)
+        if self.sink is not None:
+            dest = self.sink.take(len(self.slines[0]), len(seeds))
+            dest[:] = self.slines[0]
+            self.slines = [dest]
+
+
+@pytest.mark.parametrize("minlen,maxlen", [(0, np.inf), (100, 400)])
+def test_generate_streamlines_into_a_sink(minlen, maxlen, monkeypatch):
+    """Points downloaded straight into one buffer give the same output; the
+    buffer is sized once from the first chunk's points per seed (every later
+    chunk fits in the 15 % headroom here) and never moved again."""
+    from gpustreamlines_amd import array_sequence
+
+    seeds = _seeds(40)
+    tr = Tracker(chunk_size=8, minlen=minlen, maxlen=maxlen)
+    tr.seed_propagator = SinkPropagator(minlen=minlen, maxlen=maxlen)
+    sinks = []
+
+    class Spy(array_sequence.PackedSink):
+        def __init__(self, n):
+            super().__init__(n)
+            sinks.append(self)
+
+    from gpustreamlines_amd import generic_tracker
+
+    monkeypatch.setattr(generic_tracker, "PackedSink", Spy)
+    got = tr.generate_streamlines(seeds)
+    want = _expected(seeds, 8, minlen, maxlen)
+    assert len(got) == len(want)
+    for a, b in zip(got, want):
+        np.testing.assert_array_equal(a, b)
+    assert tr.seed_propagator.sink is None
+    assert sinks[0].grows <= 2
+
+
+def test_packed_sink_growth_order_and_misuse():
+    from gpustreamlines_amd.array_sequence import PackedSink
+
+    rng = np.random.default_rng(1)
+    sink = PackedSink(total_seeds=100)
+    parts = []
+    for k in range(10):
+        lens = rng.integers(1, 30, size=rng.integers(0, 12))
+        pts = rng.normal(size=(int(lens.sum()), 3)).astype(np.float32)
+        parts.append((pts, lens))
+        if k % 2:
+            sink.take(len(pts), 10)[:] = pts  # a producer that downloads into the sink
+            sink.add(sink._open[0], lens, 10)
+        else:
+            sink.add(pts, lens, 10)  # a producer with its own array: copied in
+    out = sink.result()
+    np.testing.assert_array_equal(out._data, np.concatenate([p for p, _ in parts]))
+    np.testing.assert_array_equal(out._lengths, np.concatenate([n for _, n in parts]))
+    np.testing.assert_array_equal(out._offsets[1:], np.cumsum(out._lengths)[:-1])
+    kept = sink.result(minlen=10, maxlen=20)
+    assert ((kept._lengths >= 10) & (kept._lengths <= 20)).all()
+    assert sum(len(s) for s in kept) == kept._lengths.sum() == len(kept._data)
+    # a view taken and not added, or added out of order, is refused
+    s2 = PackedSink()
+    a, b = s2.take(3), s2.take(2)
+    with pytest.raises(RuntimeError):
+        s2.add(b, [2])
+    with pytest.raises(RuntimeError):
+        s2.result()
+    s2.add(a, [3])
+    s2.add(b, [1, 1])
+    assert len(s2.result()) == 3
+    assert len(PackedSink().result()) == 0
+
+
 class FakeTrxFile:
     """The calls the TRX writer makes on trx.trx_file_memmap.TrxFile."""
 
