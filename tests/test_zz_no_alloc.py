@@ -3,7 +3,6 @@ allocates nothing: its seed-order workspace is the library's static
 per-device array) against cuslines_phase2_async on the same inputs, on two
 calls in a row, the second with fewer seeds.  Same kernels, so bit-exact."""
 
-import ctypes as C
 
 import numpy as np
 import pytest
