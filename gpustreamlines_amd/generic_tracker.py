@@ -69,16 +69,21 @@ class GenericTracker:
         if self._ngpus() == 1:
             per = max(1, int(getattr(self, "launch_seeds", 0) or 0) // gsz)
             depth = int(getattr(self, "chunk_pipeline", 1) or 1)
-            if per == 1 and depth > 1 and nchunks > 1:
-                # one launch per chunk, consecutive chunks overlapping on
-                # handles that share the device data
+            if depth > 1 and nchunks > per:
+                # consecutive launches overlapping on handles that share the
+                # device data: one chunk per launch with `depth` in flight,
+                # or `per` chunks per launch (chunk-relative streams) with
+                # two in flight, so one launch's points are downloaded while
+                # the next one tracks
                 from .propagate_seeds import ChunkPipeline
 
-                pipe = ChunkPipeline(self.device_trackers[0], depth=depth,
-                                     graphs=bool(getattr(self, "chunk_graphs", True)))
+                step = per * gsz
+                pipe = ChunkPipeline(self.device_trackers[0], depth=depth if per == 1 else 2,
+                                     graphs=bool(getattr(self, "chunk_graphs", True)),
+                                     rng_chunk=gsz if per > 1 else None)
                 try:
-                    for c, (pts, lens) in enumerate(pipe.run(seeds, gsz, sink=sink)):
-                        yield pts, lens, len(seeds[c * gsz:(c + 1) * gsz])
+                    for c, (pts, lens) in enumerate(pipe.run(seeds, step, sink=sink)):
+                        yield pts, lens, len(seeds[c * step:(c + 1) * step])
                 finally:
                     pipe.close()
                 return

@@ -217,11 +217,15 @@ class ChunkPipeline:
     runs again with more (the same results).  Buffers grow and are reused
     across chunks (freeing memory would synchronise the device)."""
 
-    def __init__(self, dt, depth: int = 4, minlen=0, maxlen=np.inf, graphs: bool = True):
+    def __init__(self, dt, depth: int = 4, minlen=0, maxlen=np.inf, graphs: bool = True, rng_chunk=None):
         from ._device import ChunkBuffers
 
         self._CB = ChunkBuffers
         self.dt = dt
+        # random streams restart every rng_chunk seeds of a chunk (None: once
+        # per chunk, the reference's protocol; several reference chunks per
+        # launch otherwise, as SeedBatchPropagator.rng_chunk)
+        self.rng_chunk = rng_chunk
         self.handles = [dt] + [dt.fork() for _ in range(max(1, int(depth)) - 1)]
         self.cbs = [None] * len(self.handles)
         self.pending = [None] * len(self.handles)
@@ -245,7 +249,7 @@ class ChunkPipeline:
         n = len(seeds)
         cb = self._cb(k, n, max(slot_hint, 1))
         cb.sd.upload(seeds)
-        cb.enqueue(n, n, graph=self.graphs)
+        cb.enqueue(n, self.rng_chunk or n, graph=self.graphs)
         self.pending[k] = seeds
 
     def _collect(self, k, sink=None):
@@ -258,7 +262,7 @@ class ChunkPipeline:
             self.reruns += 1
             cb = self._cb(k, n, nsl)
             cb.sd.upload(seeds)
-            cb.enqueue(n, n, graph=False)
+            cb.enqueue(n, self.rng_chunk or n, graph=False)
             nsl, tracked, npts = cb.totals()
             assert tracked == nsl
         if sink is not None:

@@ -39,8 +39,10 @@ class GPUTracker(GenericTracker):
         chunk-relative, so the output does not depend on it (0: one chunk per
         launch).  ``chunk_pipeline`` (not in the reference): with one chunk
         per launch, this many chunks run concurrently on handles sharing the
-        device data (1: one after the other); the output does not depend on
-        it.  ``chunk_graphs`` (not in the reference): those chunks run as one
+        device data; with grouped launches, two launches run at once, so one
+        launch's points are downloaded while the next one tracks (1: one
+        after the other); the output does not depend on it.
+        ``chunk_graphs`` (not in the reference): those launches run as one
         HIP graph per handle (phase 1, scan, phase 2 and compaction with no
         host read between them)."""
         _lib.load()

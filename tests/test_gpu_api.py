@@ -25,13 +25,14 @@ def _oracle_by_chunks(w, seeds, chunk):
 
 
 @pytest.mark.parametrize("launch_seeds,pipeline,graphs", [(0, 1, False), (0, 4, True), (0, 4, False),
-                                                           (250000, 1, False)])
+                                                           (250000, 1, False), (64, 2, True), (64, 3, False)])
 @pytest.mark.parametrize("kind", ["prob", "ptt", "csa"])
 def test_gpu_tracker_generate_streamlines(small_workloads, kind, launch_seeds, pipeline, graphs):
     """launch_seeds 0: one chunk per launch (the reference's protocol), one
     chunk after the other (pipeline 1) or 4 chunks in flight on forked
     handles as bounded chunks, eager or as HIP graphs; 250000: all chunks in
-    one launch with chunk-relative streams."""
+    one launch with chunk-relative streams; 64 with a pipeline: launches of
+    two chunks (chunk-relative streams) as bounded chunks, two in flight."""
     from gpustreamlines_amd import (BootDirectionGetter, GPUTracker, ProbDirectionGetter, PttDirectionGetter)
     from gpustreamlines_amd import workloads as W
 

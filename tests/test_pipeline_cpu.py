@@ -46,12 +46,14 @@ class FakeChunk:
         self.graph = None
         self.closed = False
         self.enqueued = []
+        self.rng_chunks = []
         FakeChunk.made.append(self)
 
     def enqueue(self, n, rng_chunk, graph=False):
         assert not self.closed and n <= self.nmax
         seeds = self.sd.data[:n]
         self.enqueued.append((n, graph))
+        self.rng_chunks.append(rng_chunk)
         if graph:
             self.graph = "g"
         sl = [s for seed in seeds for s in _slines_of(seed)]
@@ -147,3 +149,19 @@ def test_pipeline_into_a_sink_and_length_filter():
     pipe.close()
     np.testing.assert_array_equal(np.concatenate([o[1] for o in out]), lens[keep])
     np.testing.assert_array_equal(np.concatenate([o[0] for o in out]), pts[np.repeat(keep, lens)])
+
+
+def test_pipeline_passes_chunk_relative_streams():
+    """Launches of several reference chunks keep the streams chunk-relative:
+    every launch (and rerun) gets rng_chunk; without it a launch is one
+    chunk (rng_chunk = its seed count)."""
+    seeds = _seeds(25)
+    pipe = _pipe(2, rng_chunk=4)
+    list(pipe.run(seeds, 12, slots_per_seed=0.3))
+    pipe.close()
+    assert pipe.reruns > 0
+    assert {r for cb in FakeChunk.made for r in cb.rng_chunks} == {4}
+    pipe = _pipe(2)
+    list(pipe.run(seeds, 12))
+    pipe.close()
+    assert sorted(r for cb in FakeChunk.made for r in cb.rng_chunks) == [1, 12, 12]
