@@ -41,7 +41,9 @@ class GPUTracker(GenericTracker):
         per launch, this many chunks run concurrently on handles sharing the
         device data; with grouped launches, two launches run at once, so one
         launch's points are downloaded while the next one tracks (1: one
-        after the other); the output does not depend on it.
+        after the other, with slots sized from the read-back streamline
+        count: the least memory; two 250,000-seed launches in flight hold
+        about 98 GB of worst-case slots); the output does not depend on it.
         ``chunk_graphs`` (not in the reference): those launches run as one
         HIP graph per handle (phase 1, scan, phase 2 and compaction with no
         host read between them)."""
