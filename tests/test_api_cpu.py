@@ -64,6 +64,18 @@ def test_tracker_rejects_mismatched_sphere():
                    0.1, np.zeros((4, 3), np.float32), np.zeros((1, 2), np.int32))
 
 
+def test_tracker_rejects_more_gpus_than_present():
+    """Asking for more GPUs than the machine has raises RuntimeError before
+    any device memory is touched (cu_tractography.py:161-164 raises the same
+    type); without any GPU the failed device count is that error too."""
+    from gpustreamlines_amd import GPUTracker, ProbDirectionGetter
+
+    v = np.eye(3, dtype=np.float32)
+    with pytest.raises(RuntimeError):
+        GPUTracker(ProbDirectionGetter(), np.zeros((2, 2, 2, 3), np.float32), np.zeros((2, 2, 2), np.float32),
+                   0.1, v, np.array([[0, 1], [1, 2]], np.int32), ngpus=10**6)
+
+
 def test_streamline_array():
     a = StreamlineArray([np.zeros((3, 3)), np.ones((2, 3))])
     assert len(a) == 2 and a[1].shape == (2, 3) and a._data.shape == (5, 3)

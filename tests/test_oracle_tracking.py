@@ -87,3 +87,47 @@ def test_straight_fibre_lengths():
         assert pts[:, 0].min() > 3.0 and pts[:, 0].max() < 25.0
         steps = np.linalg.norm(np.diff(pts, axis=0), axis=1)
         np.testing.assert_allclose(steps, step, rtol=1e-4)
+
+
+@pytest.mark.parametrize("kind", ["prob", "csa", "opdt"])
+def test_seeds_outside_the_volume_have_no_directions(small_workloads, kind):
+    """A seed with a coordinate below -0.5 or at/after dim - 0.5 finds no
+    direction: the trilinear read returns -1 (tracking_helpers.cu:12-16) and
+    the getter reports 0 (generate_streamlines_cuda.cu, boot.cu phase 1).
+    The same points moved just inside are read, so the bound, not the
+    data, decides.  (PTT's phase 1 samples the tiled texture at the seed
+    instead, where neighbouring tiles can bleed in, so it is left out.)"""
+    w = small_workloads[kind]
+    dims = np.array(w.dataf.shape[:3], dtype=np.float32)
+    c = (dims - 1) / 2
+    out, inside = [], []
+    for ax in range(3):
+        for lo in (True, False):
+            p = c.copy()
+            q = c.copy()
+            if lo:
+                p[ax], q[ax] = np.nextafter(np.float32(-0.5), np.float32(-1)), np.float32(-0.5)
+            else:
+                p[ax], q[ax] = dims[ax] - 0.5, np.nextafter(dims[ax] - np.float32(0.5), np.float32(0))
+            out.append(p)
+            inside.append(q)
+    o = _oracle(w)
+    r = o.track(np.array(out, dtype=np.float32))
+    assert not r["counts"].any() and len(r["sline_len"]) == 0
+    for p, q in zip(out, inside):  # the bound itself: -1 just outside, 0 just inside
+        assert o.trilinear(p)[0] == -1
+        assert o.trilinear(q)[0] == 0
+
+
+def test_constant_odf_has_no_peaks(small_workloads):
+    """An ODF equal on every vertex has no local maximum: no edge marks
+    either end (tracking_helpers.cu:109-131), so the peak list is empty,
+    at zero and at any constant; one raised vertex is the single peak."""
+    w = small_workloads["csa"]
+    o = _oracle(w)
+    nv = w.params["nverts"]
+    assert len(o.peaks(np.zeros(nv, np.float32))) == 0
+    assert len(o.peaks(np.full(nv, 0.7, np.float32))) == 0
+    odf = np.full(nv, 0.7, np.float32)
+    odf[5] = 1.0
+    assert list(o.peaks(odf)) == [5]
