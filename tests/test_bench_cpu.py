@@ -1,5 +1,6 @@
-"""bench.py's host-side contract, checked without a GPU: the one-GPU guard and
-the algorithmic bytes per step that its roofline figure divides by time."""
+"""bench.py's host-side contract, checked without a GPU: the one-GPU guard,
+the algorithmic bytes per step that its roofline figure divides by time, and
+the committed traffic summaries its default lines report."""
 
 import os
 import subprocess
@@ -31,3 +32,20 @@ def test_algorithmic_bytes_per_step():
     assert per_sub == 193 * 36 + 32
     # the reference's work adds one full round of 32 candidates x 8 probes
     assert bench.ptt_reference_bytes_per_step({"step_frac": 10}) == 161_972
+
+
+def test_default_lines_find_their_measured_traffic():
+    """The committed FETCH_SIZE/WRITE_SIZE summaries are matched by workload
+    and kernel name; the default PTT line and the default prob and CSA
+    lines must find theirs, or `roofline.traffic` would silently be null."""
+    import types
+
+    sys.path.insert(0, ROOT)
+    import bench
+
+    for getter, lowrank in (("ptt", False), ("prob", False), ("csa", True)):
+        args = types.SimpleNamespace(getter=getter, boot_mode="mfma", boot_occ=0, ptt_lanes=0, stepping="flat")
+        workload = f"{getter}_1000000_seeds_mixed_81x106x76_step0.25vox"
+        t = bench.measured_traffic(workload, bench.kernel_name(args, lowrank))
+        assert t is not None, (getter, bench.kernel_name(args, lowrank))
+        assert t.get("traffic_bytes_unscaled", t["traffic_bytes"]) > 0
