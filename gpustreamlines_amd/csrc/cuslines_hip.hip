@@ -3083,7 +3083,8 @@ __device__ void bb_end_pass(const DevParams &P, const DevBufs &B, BbSlot &st, Ph
 
 // OCC: waves per SIMD the kernel is compiled for (2: 256 VGPRs and one
 // workgroup per CU; 4: 128 VGPRs with spills and two workgroups per CU, the
-// default: 99.8 against 71.9 M steps/s for CSA at 1M seeds, as the two
+// default: 99.8 against 71.9 M steps/s for CSA at 1M seeds when the two were
+// compared, before the min/max peak test and the low-rank form, as the two
 // workgroups' lockstep phases overlap).
 // LR: the low-rank form of the [H; R] product (P.rk16 > 0), a separate
 // instantiation so that each form's kernel carries only its own registers.
