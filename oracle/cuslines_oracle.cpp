@@ -21,6 +21,17 @@
  * float64 libm.  Where the reference compares a float against a macro double
  * literal the comparison is done in double, as C++ promotion does there.
  *
+ * Parity unpinned (nothing available here fixes them):
+ *  - normals: curand_normal's own algorithm is not published; the normals
+ *    here are Box-Muller over this file's log/sin/cos (pinned to their own
+ *    float64 formula and to the uniforms, which rocRAND's known-answer
+ *    vectors of the same Philox4x32-10 API pin), so they are not cuRAND's
+ *    bits.
+ *  - texture filtering: the CUDA guide fixes the 9-bit fixed-point weights
+ *    (8 fractional bits), the half-texel shift and the border colour; the
+ *    rounding of the fraction to 8 bits (nearest, ties to even, here) is
+ *    not documented.
+ *
  * Deliberate deviations (undefined behaviour in the reference):
  *  - boot tracking: the reference writes the peak list of a non-initial step
  *    into a one-element shared slot (boot.cu:773-789 passes
